@@ -1,0 +1,191 @@
+/*
+ * staq_dic.h — C ABI of the MI355X (gfx950) implementation of the STAQ-DIC
+ * hot path: ZNCC integer search, 6-DOF and 2-DOF (augmented-Lagrangian)
+ * IC-GN subset fitting, the bicubic gather, and the ZNSSD adjoint.
+ *
+ * Conventions (identical to the MATLAB reference, so MEX gateways pass mxArray
+ * data straight through):
+ *   - images are column-major, dim 1 = x (H entries), dim 2 = y (W entries),
+ *     i.e. the arrays after the reference's transpose on load
+ *     (func_quadtree/ReadImageQuadtree.m:78); coordinates are 1-based;
+ *   - coordinatesFEM is N x 2 column-major (x then y);
+ *   - U is 2N interleaved [u1 v1 u2 v2 ...], F is 4N [u_x v_x u_y v_y] per node,
+ *     Hpar is N x 21 column-major (packed lower triangle of the 6x6 Hessian,
+ *     funICGNQuadtree.m:448);
+ *   - status is the reference's ConvItPerEle code: 1..100 iterations used,
+ *     101 not converged / left the image / NaN, 102 >= 40 % masked,
+ *     103 featureless, -1 an error the reference catches (LocalICGNQuadtree.m:92-97).
+ *
+ * Every call returns 0 on success and a negative STAQ_E* code on a fatal
+ * error (bad arguments, out of memory, HIP failure); staq_last_error() then
+ * describes it.  Per-subset failures are never fatal: they are status codes.
+ * The caller owns every host buffer; outputs are preallocated by the caller.
+ * A context is not re-entrant; distinct contexts may be used from distinct
+ * threads.  Host-pointer calls are synchronous; *_dev and staq_batch_* calls
+ * take device pointers / device-resident batches and are asynchronous on the
+ * given stream (a hipStream_t passed as void*, NULL = the context stream).
+ */
+#ifndef STAQ_DIC_H
+#define STAQ_DIC_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+    STAQ_OK = 0,
+    STAQ_EINVAL = -1,  /* bad argument                         */
+    STAQ_ENOMEM = -2,  /* device or host allocation failed     */
+    STAQ_EHIP = -3,    /* HIP runtime error                    */
+    STAQ_ENOFRAME = -4 /* frame slot not set                   */
+};
+
+typedef struct staq_ctx staq_ctx;
+typedef struct staq_batch staq_batch;
+
+typedef struct staq_params {
+    double tol;      /* IC-GN stopping threshold (LocalICGNQuadtree tol argument)        */
+    double mu;       /* AL penalty of Subproblem 1 (Subpb1Quadtree mu)                   */
+    int max_iter;    /* 100 in the reference (funICGNQuadtree.m:59)                      */
+    int plain;       /* 1: uniform-grid twins funICGN / Subpb1 (no node rounding,
+                        whole-window statistics, DfDx==0 mask rule)                      */
+    int block_size;  /* workgroup size 64/128/256/512; 0 = tuned or default              */
+    int persistent;  /* 1: persistent grid pulling subsets from a device work queue      */
+} staq_params;
+
+/* Defaults of the reference's drivers: tol 1e-2 (main_ALDIC_Quadtree_v2.m:212),
+ * mu 1e-3 (:256), max_iter 100. */
+void staq_params_default(staq_params *p);
+
+/* ---- context ---------------------------------------------------------- */
+int staq_ctx_create(staq_ctx **ctx, int device);
+void staq_ctx_destroy(staq_ctx *ctx);
+const char *staq_last_error(const staq_ctx *ctx);
+/* hipStream_t of the context (as void*), for callers that sequence their own work. */
+void *staq_ctx_stream(staq_ctx *ctx);
+
+/* ---- frames: a reference/deformed pair resident in HBM as FP32 --------- */
+/* f = fNormalized (already times its mask, main_ALDIC_Quadtree_v2.m:57-59),
+ * g = gNormalized, mask = ImgRefMask (NULL = all ones); H x W column-major.
+ * Computes Df (funImgGradient.m:53-78) on the device.  slot >= 0 selects the
+ * frame pair (several pairs may be resident for batched launches). */
+int staq_set_frame(staq_ctx *ctx, int slot, const double *f, const double *g, const double *mask,
+                   long H, long W);
+int staq_set_frame_f32(staq_ctx *ctx, int slot, const float *f, const float *g, const float *mask,
+                       long H, long W);
+/* Device-resident FP32 images (copied into the context's buffers). */
+int staq_set_frame_dev(staq_ctx *ctx, int slot, const float *f, const float *g, const float *mask,
+                       long H, long W, void *stream);
+/* Copy the device gradient of a slot back: (H-6) x (W-6) each. */
+int staq_get_gradient(staq_ctx *ctx, int slot, double *dfdx, double *dfdy);
+
+/* ---- drop-ins of the reference's functions (host pointers) ------------- */
+
+/* Z = ba_interp2(F, X, Y, 'cubic')  (ba_interp2.cpp:318-335).  F is M x N x P,
+ * X (dim-2 coordinate) and Y (dim-1 coordinate) hold nd points; Z is nd x P. */
+int staq_ba_interp2(staq_ctx *ctx, const double *F, long M, long N, long P, const double *X,
+                    const double *Y, long nd, double *Z);
+int staq_ba_interp2_f32(staq_ctx *ctx, const float *F, long M, long N, long P, const float *X,
+                        const float *Y, long nd, float *Z);
+/* Device pointers, FP32 or FP64 (single = 1 for FP32). */
+int staq_ba_interp2_dev(staq_ctx *ctx, const void *F, long M, long N, long P, const void *X,
+                        const void *Y, long nd, void *Z, int single, void *stream);
+
+/* Df = funImgGradient(img, img, mask) (func/funImgGradient.m:53-78):
+ * dfdx, dfdy are (H-6) x (W-6). */
+int staq_img_gradient(staq_ctx *ctx, const double *img, const double *mask, long H, long W,
+                      double *dfdx, double *dfdy);
+
+/* [x,y,u,v,cc] = funIntegerSearch(f, g, [r1 r2], gridx, gridy, winsize, winstepsize, 0)
+ * (func/funIntegerSearch.m:3-35, :82-203) on frame `slot` (f = its reference,
+ * g = its deformed image).  Outputs are ny x nx column-major (rows = y) and
+ * must hold max_pts entries; *nx, *ny receive the grid size. */
+int staq_integer_search(staq_ctx *ctx, int slot, long r1, long r2, const long gridx[2],
+                        const long gridy[2], long winsize, long winstepsize, long max_pts,
+                        long *nx, long *ny, double *x0, double *y0, double *u, double *v,
+                        double *ccmax);
+
+/* Raw subset outputs of LocalICGNQuadtree (func_quadtree/LocalICGNQuadtree.m:63-147)
+ * or, with params->plain, LocalICGN (func/LocalICGN.m:60-115), before the
+ * bad-subset classification: U (2N), F (4N), Hpar (N x 21), status (N).
+ * winsize is per node (N entries); slot may be NULL (all frame 0). */
+int staq_local_icgn(staq_ctx *ctx, const staq_params *p, long N, const double *U0,
+                    const double *coords, const long *winsize, const int *slot, double *U,
+                    double *F, double *Hpar, int *status);
+
+/* Raw outputs of Subpb1Quadtree (func_quadtree/Subpb1Quadtree.m:70-169) or,
+ * with params->plain, Subpb1 (func/Subpb1.m): U (2N) and status (N; an error
+ * the reference catches is -1 here).  udual is accepted for signature parity
+ * and unused, as in funICGN_Subpb1. */
+int staq_subpb1(staq_ctx *ctx, const staq_params *p, long N, const double *UOld,
+                const double *FOld, const double *udual, const double *vdual,
+                const double *coords, const long *wx, const long *wy, const int *slot,
+                double *U, int *status);
+
+/* ---- ZNSSD mismatch and its adjoint (no reference counterpart) ---------- */
+/* cost_i = sum_p (fhat - ghat(W(P_i)))^2 over the (w+1)^2 window of node i
+ * (whole-window statistics); status 1 = window or warp outside the image.
+ * P is 6N [u_x v_x u_y v_y u v] per subset. */
+int staq_znssd_fwd(staq_ctx *ctx, long N, const double *P, const double *coords,
+                   const long *winsize, const int *slot, double *cost, int *status);
+/* Vector-Jacobian product: gradP (6N) = gcost_i * dcost_i/dP_i (gcost NULL =
+ * ones); gradU (2N, may be NULL) = the translation part, i.e. the gradient
+ * with respect to the nodal displacements the subsets are centred on. */
+int staq_znssd_bwd(staq_ctx *ctx, long N, const double *P, const double *coords,
+                   const long *winsize, const int *slot, const double *gcost, double *gradP,
+                   double *gradU);
+/* Device-pointer forms (winsize/slot int32; slot may be NULL). */
+int staq_znssd_fwd_dev(staq_ctx *ctx, long N, int wmax, const double *P, const double *coords,
+                       const int *winsize, const int *slot, double *cost, int *status,
+                       void *stream);
+int staq_znssd_bwd_dev(staq_ctx *ctx, long N, int wmax, const double *P, const double *coords,
+                       const int *winsize, const int *slot, const double *gcost, double *gradP,
+                       void *stream);
+
+/* ---- device-resident batches (one launch per ADMM local step) ---------- */
+/* A batch holds the subset list (coords N x 2, wx, wy (NULL = wx), slot (NULL
+ * = 0)) in HBM together with its inputs and outputs, sorted into window-size
+ * bins; launches are asynchronous on the context stream and may be captured
+ * into a hipGraph. */
+int staq_batch_create(staq_ctx *ctx, staq_batch **b, long N, const double *coords, const long *wx,
+                      const long *wy, const int *slot);
+void staq_batch_destroy(staq_batch *b);
+enum { STAQ_IN_U0 = 0, STAQ_IN_UOLD = 1, STAQ_IN_FOLD = 2, STAQ_IN_VDUAL = 3 };
+/* Upload an input vector (U0/UOld/vdual: 2N, FOld: 4N) from host memory. */
+int staq_batch_set_input(staq_batch *b, int which, const double *host);
+/* 6-DOF local step (LocalICGNQuadtree) over the whole batch. */
+int staq_batch_icgn6(staq_batch *b, const staq_params *p);
+/* 2-DOF AL local step (Subpb1Quadtree) over the whole batch. */
+int staq_batch_icgn2(staq_batch *b, const staq_params *p);
+/* The one-workgroup-per-launch form: N launches of one subset each.  With
+ * kernel_ms_sum non-NULL each launch is bracketed by events and the summed
+ * device time (the overhead-free floor of this form) is returned. */
+int staq_batch_icgn6_per_subset(staq_batch *b, const staq_params *p, double *kernel_ms_sum);
+/* Copy outputs to host (any pointer may be NULL). */
+int staq_batch_get_output(staq_batch *b, double *U, double *F, double *Hpar, int *status);
+/* Feed the 6-DOF outputs (U, F) of this batch to its 2-DOF inputs (UOld, FOld)
+ * on the device, vdual = 0 — the first AL step with an identity global step. */
+int staq_batch_chain_outputs(staq_batch *b);
+/* Time each candidate workgroup size per window bin on the resident inputs
+ * and keep the fastest (mode 6 or 2). */
+int staq_batch_autotune(staq_batch *b, const staq_params *p, int mode);
+/* Capture one local step (mode 6 or 2) into a hipGraph, then replay it. */
+int staq_batch_graph_capture(staq_batch *b, const staq_params *p, int mode);
+int staq_batch_graph_launch(staq_batch *b);
+/* Device pointers of the batch arrays (for callers that wrap them). */
+typedef struct staq_batch_ptrs {
+    double *coords, *U0, *UOld, *FOld, *vdual, *U, *F, *Hpar;
+    int *status, *wx, *wy, *slot;
+    long N;
+    int wmax;
+} staq_batch_ptrs;
+int staq_batch_get_ptrs(staq_batch *b, staq_batch_ptrs *out);
+/* Number of window bins and, for bin k, its window size cap, item count and
+ * workgroup size chosen for mode 6 / mode 2. */
+int staq_batch_bins(staq_batch *b, int *nbins, int *wcap, long *count, int *bs6, int *bs2,
+                    int maxbins);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

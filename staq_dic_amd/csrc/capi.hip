@@ -1,0 +1,1079 @@
+// Host side of the C ABI declared in include/staq_dic.h.
+//
+// A context owns one HIP stream (default priority) and the frames resident in
+// HBM.  A batch owns a device-resident subset list sorted into window-size
+// bins; a local step is: reset kernel (queue heads, defer flag), one launch
+// per bin with LDS sized for that bin, and a fallback launch that only runs
+// if some subset's window grew beyond its bin (masked subsets,
+// funICGNQuadtree.m:68-79).  Everything a step enqueues is a kernel, so the
+// step can be captured into a hipGraph.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/staq_dic.h"
+#include "kernels.h"
+
+
+
+using namespace staq;
+
+// ---------------------------------------------------------------------------
+// small device helpers
+// ---------------------------------------------------------------------------
+__global__ void reset_ints_kernel(int *p, int n)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = 0;
+}
+
+__global__ void defer_flag_kernel(const int *status, const int *order, long item0, long n,
+                                  int *flag)
+{
+    const long i = blockIdx.x * (long)blockDim.x + threadIdx.x;
+    if (i < n && status[order[item0 + i]] == ST_DEFER) *flag = 1;
+}
+
+__global__ void chain_kernel(const double *U, const double *F, double *UOld, double *FOld,
+                             double *vdual, long N)
+{
+    const long i = blockIdx.x * (long)blockDim.x + threadIdx.x;
+    if (i < 2 * N) {
+        UOld[i] = U[i];
+        vdual[i] = 0.0;
+    }
+    if (i < 4 * N) FOld[i] = F[i];
+}
+
+__global__ void f64_to_f32_kernel(const double *in, float *out, long n)
+{
+    const long i = blockIdx.x * (long)blockDim.x + threadIdx.x;
+    if (i < n) out[i] = (float)in[i];
+}
+
+__global__ void ones_kernel(float *p, long n)
+{
+    const long i = blockIdx.x * (long)blockDim.x + threadIdx.x;
+    if (i < n) p[i] = 1.f;
+}
+
+// ---------------------------------------------------------------------------
+// context
+// ---------------------------------------------------------------------------
+namespace {
+
+struct DevBuf {
+    void *p = nullptr;
+    size_t n = 0;
+    hipError_t ensure(size_t bytes)
+    {
+        if (bytes <= n) return hipSuccess;
+        if (p) hipFree(p);
+        p = nullptr;
+        n = 0;
+        hipError_t e = hipMalloc(&p, bytes ? bytes : 1);
+        if (e == hipSuccess) n = bytes;
+        return e;
+    }
+    void release()
+    {
+        if (p) hipFree(p);
+        p = nullptr;
+        n = 0;
+    }
+    template <typename T>
+    T *as() const { return reinterpret_cast<T *>(p); }
+};
+
+struct Frame {
+    DevBuf f, g, mask, dfdx, dfdy;
+    long H = 0, W = 0;
+    bool set = false;
+};
+
+}  // namespace
+
+struct staq_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    std::string err;
+    std::vector<Frame> frames;
+    DevBuf d_frames;
+    DevBuf s[12];  // scratch for host-pointer calls
+};
+
+struct Bin {
+    int wxcap = 0, wycap = 0;
+    long item0 = 0, n = 0;
+    int npix_cap = 0, tile_cap = 0;
+    int bs[2] = {0, 0};  // [mode 6, mode 2]
+};
+
+struct staq_batch {
+    staq_ctx *ctx = nullptr;
+    long N = 0;
+    DevBuf coords, U0, UOld, FOld, vdual, U, F, H, status, wx, wy, slot, order, ints;
+    std::vector<Bin> bins;
+    int wmax = 0;
+    int fb_npix_cap = 0, fb_tile_cap = 0;
+    bool has_slot = false;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t gexec = nullptr;
+    int graph_mode = 0;
+};
+
+static int fail(staq_ctx *ctx, int code, const std::string &msg)
+{
+    if (ctx) ctx->err = msg;
+    return code;
+}
+
+static int hipfail(staq_ctx *ctx, hipError_t e, const char *what)
+{
+    return fail(ctx, e == hipErrorOutOfMemory ? STAQ_ENOMEM : STAQ_EHIP,
+                std::string(what) + ": " + hipGetErrorString(e));
+}
+
+#define HIPCHK(ctx, x)                                  \
+    do {                                                \
+        hipError_t e_ = (x);                            \
+        if (e_ != hipSuccess) return hipfail(ctx, e_, #x); \
+    } while (0)
+
+static hipStream_t pick_stream(staq_ctx *ctx, void *s)
+{
+    return s ? reinterpret_cast<hipStream_t>(s) : ctx->stream;
+}
+
+extern "C" {
+
+void staq_params_default(staq_params *p)
+{
+    if (!p) return;
+    p->tol = 1e-2;
+    p->mu = 1e-3;
+    p->max_iter = 100;
+    p->plain = 0;
+    p->block_size = 0;
+    p->persistent = 0;
+}
+
+int staq_ctx_create(staq_ctx **out, int device)
+{
+    if (!out) return STAQ_EINVAL;
+    *out = nullptr;
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess || n <= 0) return STAQ_EHIP;
+    if (device < 0 || device >= n) return STAQ_EINVAL;
+    if (hipSetDevice(device) != hipSuccess) return STAQ_EHIP;
+    staq_ctx *c = new staq_ctx();
+    c->device = device;
+    if (hipStreamCreate(&c->stream) != hipSuccess) {
+        delete c;
+        return STAQ_EHIP;
+    }
+    *out = c;
+    return STAQ_OK;
+}
+
+void staq_ctx_destroy(staq_ctx *ctx)
+{
+    if (!ctx) return;
+    hipSetDevice(ctx->device);
+    hipStreamSynchronize(ctx->stream);
+    for (auto &f : ctx->frames) {
+        f.f.release();
+        f.g.release();
+        f.mask.release();
+        f.dfdx.release();
+        f.dfdy.release();
+    }
+    ctx->d_frames.release();
+    for (auto &b : ctx->s) b.release();
+    hipStreamDestroy(ctx->stream);
+    delete ctx;
+}
+
+const char *staq_last_error(const staq_ctx *ctx) { return ctx ? ctx->err.c_str() : "no context"; }
+
+void *staq_ctx_stream(staq_ctx *ctx) { return ctx ? (void *)ctx->stream : nullptr; }
+
+}  // extern "C"
+
+// Refresh the device array of frame descriptors.
+static int sync_frames(staq_ctx *ctx, hipStream_t s)
+{
+    std::vector<FrameDev> h(ctx->frames.size());
+    for (size_t i = 0; i < h.size(); ++i) {
+        const Frame &f = ctx->frames[i];
+        h[i].f = f.f.as<float>();
+        h[i].g = f.g.as<float>();
+        h[i].mask = f.mask.as<float>();
+        h[i].dfdx = f.dfdx.as<float>();
+        h[i].dfdy = f.dfdy.as<float>();
+        h[i].H = (int)f.H;
+        h[i].W = (int)f.W;
+    }
+    HIPCHK(ctx, ctx->d_frames.ensure(sizeof(FrameDev) * std::max<size_t>(h.size(), 1)));
+    HIPCHK(ctx, hipMemcpyAsync(ctx->d_frames.p, h.data(), sizeof(FrameDev) * h.size(),
+                               hipMemcpyHostToDevice, s));
+    HIPCHK(ctx, hipStreamSynchronize(s));
+    return STAQ_OK;
+}
+
+static int frame_alloc(staq_ctx *ctx, int slot, long H, long W)
+{
+    if (slot < 0 || slot > 4096) return fail(ctx, STAQ_EINVAL, "frame slot out of range");
+    if (H < 16 || W < 16 || H > (1L << 20) || W > (1L << 20))
+        return fail(ctx, STAQ_EINVAL, "image must be at least 16 x 16");
+    if ((size_t)slot >= ctx->frames.size()) ctx->frames.resize(slot + 1);
+    Frame &fr = ctx->frames[slot];
+    const size_t n = (size_t)H * W, nd = (size_t)(H - 6) * (W - 6);
+    HIPCHK(ctx, fr.f.ensure(n * 4));
+    HIPCHK(ctx, fr.g.ensure(n * 4));
+    HIPCHK(ctx, fr.mask.ensure(n * 4));
+    HIPCHK(ctx, fr.dfdx.ensure(nd * 4));
+    HIPCHK(ctx, fr.dfdy.ensure(nd * 4));
+    fr.H = H;
+    fr.W = W;
+    return STAQ_OK;
+}
+
+static int frame_finish(staq_ctx *ctx, int slot, hipStream_t s)
+{
+    Frame &fr = ctx->frames[slot];
+    HIPCHK(ctx, img_gradient_launch(fr.f.as<float>(), fr.mask.as<float>(), (int)fr.H, (int)fr.W,
+                                    fr.dfdx.as<float>(), fr.dfdy.as<float>(), s));
+    fr.set = true;
+    return sync_frames(ctx, s);
+}
+
+extern "C" {
+
+int staq_set_frame_f32(staq_ctx *ctx, int slot, const float *f, const float *g, const float *mask,
+                       long H, long W)
+{
+    if (!ctx || !f || !g) return fail(ctx, STAQ_EINVAL, "null argument");
+    hipSetDevice(ctx->device);
+    int rc = frame_alloc(ctx, slot, H, W);
+    if (rc) return rc;
+    Frame &fr = ctx->frames[slot];
+    const size_t n = (size_t)H * W;
+    HIPCHK(ctx, hipMemcpyAsync(fr.f.p, f, n * 4, hipMemcpyHostToDevice, ctx->stream));
+    HIPCHK(ctx, hipMemcpyAsync(fr.g.p, g, n * 4, hipMemcpyHostToDevice, ctx->stream));
+    if (mask)
+        HIPCHK(ctx, hipMemcpyAsync(fr.mask.p, mask, n * 4, hipMemcpyHostToDevice, ctx->stream));
+    else
+        hipLaunchKernelGGL(ones_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, ctx->stream,
+                           fr.mask.as<float>(), (long)n);
+    HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+    return frame_finish(ctx, slot, ctx->stream);
+}
+
+int staq_set_frame(staq_ctx *ctx, int slot, const double *f, const double *g, const double *mask,
+                   long H, long W)
+{
+    if (!ctx || !f || !g) return fail(ctx, STAQ_EINVAL, "null argument");
+    if (H < 16 || W < 16) return fail(ctx, STAQ_EINVAL, "image must be at least 16 x 16");
+    const size_t n = (size_t)H * W;
+    std::vector<float> ff(n), gg(n), mm;
+    for (size_t i = 0; i < n; ++i) {
+        ff[i] = (float)f[i];
+        gg[i] = (float)g[i];
+    }
+    if (mask) {
+        mm.resize(n);
+        for (size_t i = 0; i < n; ++i) mm[i] = (float)mask[i];
+    }
+    return staq_set_frame_f32(ctx, slot, ff.data(), gg.data(), mask ? mm.data() : nullptr, H, W);
+}
+
+int staq_set_frame_dev(staq_ctx *ctx, int slot, const float *f, const float *g, const float *mask,
+                       long H, long W, void *stream)
+{
+    if (!ctx || !f || !g) return fail(ctx, STAQ_EINVAL, "null argument");
+    hipSetDevice(ctx->device);
+    hipStream_t s = pick_stream(ctx, stream);
+    int rc = frame_alloc(ctx, slot, H, W);
+    if (rc) return rc;
+    Frame &fr = ctx->frames[slot];
+    const size_t n = (size_t)H * W;
+    HIPCHK(ctx, hipMemcpyAsync(fr.f.p, f, n * 4, hipMemcpyDeviceToDevice, s));
+    HIPCHK(ctx, hipMemcpyAsync(fr.g.p, g, n * 4, hipMemcpyDeviceToDevice, s));
+    if (mask)
+        HIPCHK(ctx, hipMemcpyAsync(fr.mask.p, mask, n * 4, hipMemcpyDeviceToDevice, s));
+    else
+        hipLaunchKernelGGL(ones_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s,
+                           fr.mask.as<float>(), (long)n);
+    return frame_finish(ctx, slot, s);
+}
+
+int staq_get_gradient(staq_ctx *ctx, int slot, double *dfdx, double *dfdy)
+{
+    if (!ctx || slot < 0 || (size_t)slot >= ctx->frames.size() || !ctx->frames[slot].set)
+        return fail(ctx, STAQ_ENOFRAME, "frame slot not set");
+    hipSetDevice(ctx->device);
+    Frame &fr = ctx->frames[slot];
+    const size_t nd = (size_t)(fr.H - 6) * (fr.W - 6);
+    std::vector<float> t(nd);
+    HIPCHK(ctx, hipMemcpy(t.data(), fr.dfdx.p, nd * 4, hipMemcpyDeviceToHost));
+    if (dfdx)
+        for (size_t i = 0; i < nd; ++i) dfdx[i] = t[i];
+    HIPCHK(ctx, hipMemcpy(t.data(), fr.dfdy.p, nd * 4, hipMemcpyDeviceToHost));
+    if (dfdy)
+        for (size_t i = 0; i < nd; ++i) dfdy[i] = t[i];
+    return STAQ_OK;
+}
+
+// ---------------------------------------------------------------------------
+// ba_interp2 / gradient drop-ins
+// ---------------------------------------------------------------------------
+int staq_ba_interp2_dev(staq_ctx *ctx, const void *F, long M, long N, long P, const void *X,
+                        const void *Y, long nd, void *Z, int single, void *stream)
+{
+    if (!ctx || !F || !X || !Y || !Z || M <= 0 || N <= 0 || P <= 0 || nd < 0)
+        return fail(ctx, STAQ_EINVAL, "bad ba_interp2 arguments");
+    hipSetDevice(ctx->device);
+    HIPCHK(ctx, ba_interp2_launch(F, M, N, P, X, Y, nd, Z, single != 0, pick_stream(ctx, stream)));
+    return STAQ_OK;
+}
+
+static int interp_host(staq_ctx *ctx, const void *F, long M, long N, long P, const void *X,
+                       const void *Y, long nd, void *Z, size_t es)
+{
+    if (!ctx || !F || !X || !Y || !Z || M <= 0 || N <= 0 || P <= 0 || nd < 0)
+        return fail(ctx, STAQ_EINVAL, "bad ba_interp2 arguments");
+    hipSetDevice(ctx->device);
+    const size_t nF = (size_t)M * N * P * es, nX = (size_t)nd * es, nZ = (size_t)nd * P * es;
+    HIPCHK(ctx, ctx->s[0].ensure(nF));
+    HIPCHK(ctx, ctx->s[1].ensure(nX));
+    HIPCHK(ctx, ctx->s[2].ensure(nX));
+    HIPCHK(ctx, ctx->s[3].ensure(nZ));
+    hipStream_t s = ctx->stream;
+    HIPCHK(ctx, hipMemcpyAsync(ctx->s[0].p, F, nF, hipMemcpyHostToDevice, s));
+    HIPCHK(ctx, hipMemcpyAsync(ctx->s[1].p, X, nX, hipMemcpyHostToDevice, s));
+    HIPCHK(ctx, hipMemcpyAsync(ctx->s[2].p, Y, nX, hipMemcpyHostToDevice, s));
+    HIPCHK(ctx, ba_interp2_launch(ctx->s[0].p, M, N, P, ctx->s[1].p, ctx->s[2].p, nd, ctx->s[3].p,
+                                  es == 4, s));
+    HIPCHK(ctx, hipMemcpyAsync(Z, ctx->s[3].p, nZ, hipMemcpyDeviceToHost, s));
+    HIPCHK(ctx, hipStreamSynchronize(s));
+    return STAQ_OK;
+}
+
+int staq_ba_interp2(staq_ctx *ctx, const double *F, long M, long N, long P, const double *X,
+                    const double *Y, long nd, double *Z)
+{
+    return interp_host(ctx, F, M, N, P, X, Y, nd, Z, 8);
+}
+
+int staq_ba_interp2_f32(staq_ctx *ctx, const float *F, long M, long N, long P, const float *X,
+                        const float *Y, long nd, float *Z)
+{
+    return interp_host(ctx, F, M, N, P, X, Y, nd, Z, 4);
+}
+
+int staq_img_gradient(staq_ctx *ctx, const double *img, const double *mask, long H, long W,
+                      double *dfdx, double *dfdy)
+{
+    if (!ctx || !img || !dfdx || !dfdy || H < 7 || W < 7)
+        return fail(ctx, STAQ_EINVAL, "bad gradient arguments");
+    hipSetDevice(ctx->device);
+    const size_t n = (size_t)H * W, nd = (size_t)(H - 6) * (W - 6);
+    std::vector<float> f(n), m(n, 1.f), ox(nd), oy(nd);
+    for (size_t i = 0; i < n; ++i) f[i] = (float)img[i];
+    if (mask)
+        for (size_t i = 0; i < n; ++i) m[i] = (float)mask[i];
+    HIPCHK(ctx, ctx->s[0].ensure(n * 4));
+    HIPCHK(ctx, ctx->s[1].ensure(n * 4));
+    HIPCHK(ctx, ctx->s[2].ensure(nd * 4));
+    HIPCHK(ctx, ctx->s[3].ensure(nd * 4));
+    hipStream_t s = ctx->stream;
+    HIPCHK(ctx, hipMemcpyAsync(ctx->s[0].p, f.data(), n * 4, hipMemcpyHostToDevice, s));
+    HIPCHK(ctx, hipMemcpyAsync(ctx->s[1].p, m.data(), n * 4, hipMemcpyHostToDevice, s));
+    HIPCHK(ctx, img_gradient_launch(ctx->s[0].as<float>(), ctx->s[1].as<float>(), (int)H, (int)W,
+                                    ctx->s[2].as<float>(), ctx->s[3].as<float>(), s));
+    HIPCHK(ctx, hipMemcpyAsync(ox.data(), ctx->s[2].p, nd * 4, hipMemcpyDeviceToHost, s));
+    HIPCHK(ctx, hipMemcpyAsync(oy.data(), ctx->s[3].p, nd * 4, hipMemcpyDeviceToHost, s));
+    HIPCHK(ctx, hipStreamSynchronize(s));
+    for (size_t i = 0; i < nd; ++i) {
+        dfdx[i] = ox[i];
+        dfdy[i] = oy[i];
+    }
+    return STAQ_OK;
+}
+
+// ---------------------------------------------------------------------------
+// integer search
+// ---------------------------------------------------------------------------
+int staq_integer_search(staq_ctx *ctx, int slot, long r1, long r2, const long gridx[2],
+                        const long gridy[2], long w, long step, long max_pts, long *nx_out,
+                        long *ny_out, double *x0, double *y0, double *u, double *v,
+                        double *ccmax)
+{
+    if (!ctx || !gridx || !gridy || !nx_out || !ny_out || !x0 || !y0 || !u || !v || !ccmax)
+        return fail(ctx, STAQ_EINVAL, "null argument");
+    if (slot < 0 || (size_t)slot >= ctx->frames.size() || !ctx->frames[slot].set)
+        return fail(ctx, STAQ_ENOFRAME, "frame slot not set");
+    if (r1 < 0 || r2 < 0 || w < 2 || step < 1) return fail(ctx, STAQ_EINVAL, "bad search geometry");
+    if (search_lds_bytes((int)w, (int)r1, (int)r2) > 150 * 1024)
+        return fail(ctx, STAQ_EINVAL, "template plus search block exceed LDS");
+    hipSetDevice(ctx->device);
+    const Frame &fr = ctx->frames[slot];
+    const long H = fr.H, W = fr.W;
+    // funIntegerSearch.m:14-25 ROI shrink, :90-91 grid
+    long gx0 = gridx[0], gx1 = gridx[1], gy0 = gridy[0], gy1 = gridy[1];
+    while (gx0 - r1 - 0.5 * w < 1) ++gx0;
+    while (gy0 - r2 - 0.5 * w < 1) ++gy0;
+    while (gx1 + 1.5 * w + r1 > H) --gx1;
+    while (gy1 + 1.5 * w + r2 > W) --gy1;
+    const long nx = (gx1 - step >= gx0) ? (gx1 - step - gx0) / step + 1 : 0;
+    const long ny = (gy1 - step >= gy0) ? (gy1 - step - gy0) / step + 1 : 0;
+    *nx_out = nx;
+    *ny_out = ny;
+    if (nx * ny > max_pts) return fail(ctx, STAQ_EINVAL, "output arrays too small for the grid");
+    const long np = nx * ny;
+    if (np == 0) return STAQ_OK;
+    const long kk = (2 * r1 + 3) * (2 * r2 + 3);
+    const long chunk = std::max(1L, std::min(np, (256L << 20) / (kk * 16)));
+    hipStream_t s = ctx->stream;
+    HIPCHK(ctx, ctx->s[0].ensure((size_t)chunk * kk * 16));
+    for (int k = 0; k < 5; ++k) HIPCHK(ctx, ctx->s[1 + k].ensure((size_t)np * 8));
+    SearchArgs a;
+    a.f = fr.f.as<float>();
+    a.g = fr.g.as<float>();
+    a.H = (int)H;
+    a.W = (int)W;
+    a.r1 = (int)r1;
+    a.r2 = (int)r2;
+    a.w = (int)w;
+    a.gx0 = (int)gx0;
+    a.gy0 = (int)gy0;
+    a.step = (int)step;
+    a.nx = (int)nx;
+    a.ny = (int)ny;
+    a.scratch = ctx->s[0].as<double2>();
+    a.x0 = ctx->s[1].as<double>();
+    a.y0 = ctx->s[2].as<double>();
+    a.u = ctx->s[3].as<double>();
+    a.v = ctx->s[4].as<double>();
+    a.ccmax = ctx->s[5].as<double>();
+    for (long p0 = 0; p0 < np; p0 += chunk) {
+        a.point0 = p0;
+        HIPCHK(ctx, search_launch(a, std::min(chunk, np - p0), s));
+    }
+    double *outs[5] = {x0, y0, u, v, ccmax};
+    for (int k = 0; k < 5; ++k)
+        HIPCHK(ctx, hipMemcpyAsync(outs[k], ctx->s[1 + k].p, (size_t)np * 8, hipMemcpyDeviceToHost, s));
+    HIPCHK(ctx, hipStreamSynchronize(s));
+    return STAQ_OK;
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------------------
+// batches
+// ---------------------------------------------------------------------------
+static int grown_window(int w)
+{
+    // upper bound of the window after the mask-driven growth:
+    // 2*ceil(sqrt(n/(n-count))*w/2) with count < 0.4 n (funICGNQuadtree.m:70,
+    // funICGN_Subpb1.m:78-79), or round(sqrt(2) w) (funICGN.m:58)
+    const int a = 2 * (int)std::ceil(std::sqrt(1.0 / 0.6) * w / 2.0);
+    const int b = (int)std::lround(std::sqrt(2.0) * w);
+    return std::max(a, b) + 2;
+}
+
+static int tile_side(int n) { return (int)std::ceil(n * 1.12) + 8; }
+
+static int default_bs(int npix)
+{
+    if (npix <= 400) return 64;
+    if (npix <= 1300) return 128;
+    return 256;
+}
+
+extern "C" {
+
+int staq_batch_create(staq_ctx *ctx, staq_batch **out, long N, const double *coords,
+                      const long *wx, const long *wy, const int *slot)
+{
+    if (!ctx || !out || N <= 0 || !coords || !wx) return fail(ctx, STAQ_EINVAL, "bad batch arguments");
+    *out = nullptr;
+    hipSetDevice(ctx->device);
+    std::vector<int> hx(N), hy(N), hs(N, 0), order(N);
+    int wmax = 0;
+    for (long i = 0; i < N; ++i) {
+        hx[i] = (int)wx[i];
+        hy[i] = (int)(wy ? wy[i] : wx[i]);
+        if (slot) hs[i] = slot[i];
+        if (hx[i] < 2 || hy[i] < 2 || hx[i] > 2048 || hy[i] > 2048)
+            return fail(ctx, STAQ_EINVAL, "window size out of range");
+        if (hs[i] < 0 || (size_t)hs[i] >= ctx->frames.size() || !ctx->frames[hs[i]].set)
+            return fail(ctx, STAQ_ENOFRAME, "subset refers to a frame slot that is not set");
+        wmax = std::max(wmax, std::max(hx[i], hy[i]));
+        order[i] = (int)i;
+    }
+    // largest windows first: the hardware dispatcher then fills the tail of
+    // each bin with short subsets
+    auto key = [&](int i) { return (std::max(hx[i], hy[i]) + 7) / 8; };
+    std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return key(a) > key(b); });
+    staq_batch *b = new staq_batch();
+    b->ctx = ctx;
+    b->N = N;
+    b->wmax = wmax;
+    b->has_slot = slot != nullptr;
+    long i0 = 0;
+    int fbx = 0, fby = 0;
+    while (i0 < N) {
+        const int k = key(order[i0]);
+        long i1 = i0;
+        Bin bin;
+        while (i1 < N && key(order[i1]) == k) {
+            bin.wxcap = std::max(bin.wxcap, hx[order[i1]]);
+            bin.wycap = std::max(bin.wycap, hy[order[i1]]);
+            ++i1;
+        }
+        bin.item0 = i0;
+        bin.n = i1 - i0;
+        bin.npix_cap = (bin.wxcap + 1) * (bin.wycap + 1);
+        bin.tile_cap = tile_side(bin.wxcap + 1) * tile_side(bin.wycap + 1);
+        bin.bs[0] = bin.bs[1] = default_bs(bin.npix_cap);
+        b->bins.push_back(bin);
+        fbx = std::max(fbx, grown_window(bin.wxcap));
+        fby = std::max(fby, grown_window(bin.wycap));
+        i0 = i1;
+    }
+    b->fb_npix_cap = (fbx + 1) * (fby + 1);
+    b->fb_tile_cap = tile_side(fbx + 1) * tile_side(fby + 1);
+    const size_t nb = b->bins.size();
+    int rc = STAQ_OK;
+    auto chk = [&](hipError_t e, const char *w) {
+        if (e != hipSuccess && rc == STAQ_OK) rc = hipfail(ctx, e, w);
+    };
+    chk(b->coords.ensure(N * 16), "coords");
+    chk(b->U0.ensure(N * 16), "U0");
+    chk(b->UOld.ensure(N * 16), "UOld");
+    chk(b->FOld.ensure(N * 32), "FOld");
+    chk(b->vdual.ensure(N * 16), "vdual");
+    chk(b->U.ensure(N * 16), "U");
+    chk(b->F.ensure(N * 32), "F");
+    chk(b->H.ensure(N * 21 * 8), "H");
+    chk(b->status.ensure(N * 4), "status");
+    chk(b->wx.ensure(N * 4), "wx");
+    chk(b->wy.ensure(N * 4), "wy");
+    chk(b->slot.ensure(N * 4), "slot");
+    chk(b->order.ensure(N * 4), "order");
+    chk(b->ints.ensure((nb + 2) * 4), "counters");
+    if (rc == STAQ_OK) {
+        hipStream_t s = ctx->stream;
+        chk(hipMemcpyAsync(b->coords.p, coords, N * 16, hipMemcpyHostToDevice, s), "coords");
+        chk(hipMemcpyAsync(b->wx.p, hx.data(), N * 4, hipMemcpyHostToDevice, s), "wx");
+        chk(hipMemcpyAsync(b->wy.p, hy.data(), N * 4, hipMemcpyHostToDevice, s), "wy");
+        chk(hipMemcpyAsync(b->slot.p, hs.data(), N * 4, hipMemcpyHostToDevice, s), "slot");
+        chk(hipMemcpyAsync(b->order.p, order.data(), N * 4, hipMemcpyHostToDevice, s), "order");
+        chk(hipMemsetAsync(b->U0.p, 0, N * 16, s), "zero");
+        chk(hipMemsetAsync(b->UOld.p, 0, N * 16, s), "zero");
+        chk(hipMemsetAsync(b->FOld.p, 0, N * 32, s), "zero");
+        chk(hipMemsetAsync(b->vdual.p, 0, N * 16, s), "zero");
+        chk(hipStreamSynchronize(s), "sync");
+    }
+    if (rc != STAQ_OK) {
+        staq_batch_destroy(b);
+        return rc;
+    }
+    *out = b;
+    return STAQ_OK;
+}
+
+void staq_batch_destroy(staq_batch *b)
+{
+    if (!b) return;
+    hipSetDevice(b->ctx->device);
+    hipStreamSynchronize(b->ctx->stream);
+    if (b->gexec) hipGraphExecDestroy(b->gexec);
+    if (b->graph) hipGraphDestroy(b->graph);
+    DevBuf *all[] = {&b->coords, &b->U0, &b->UOld, &b->FOld, &b->vdual, &b->U, &b->F,
+                     &b->H, &b->status, &b->wx, &b->wy, &b->slot, &b->order, &b->ints};
+    for (DevBuf *d : all) d->release();
+    delete b;
+}
+
+int staq_batch_set_input(staq_batch *b, int which, const double *host)
+{
+    if (!b || !host) return STAQ_EINVAL;
+    staq_ctx *ctx = b->ctx;
+    hipSetDevice(ctx->device);
+    DevBuf *d = which == STAQ_IN_U0 ? &b->U0 : which == STAQ_IN_UOLD ? &b->UOld
+              : which == STAQ_IN_FOLD ? &b->FOld : which == STAQ_IN_VDUAL ? &b->vdual : nullptr;
+    if (!d) return fail(ctx, STAQ_EINVAL, "unknown input");
+    const size_t n = (which == STAQ_IN_FOLD ? 4 : 2) * (size_t)b->N * 8;
+    HIPCHK(ctx, hipMemcpyAsync(d->p, host, n, hipMemcpyHostToDevice, ctx->stream));
+    HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+    return STAQ_OK;
+}
+
+int staq_batch_get_output(staq_batch *b, double *U, double *F, double *Hpar, int *status)
+{
+    if (!b) return STAQ_EINVAL;
+    staq_ctx *ctx = b->ctx;
+    hipSetDevice(ctx->device);
+    hipStream_t s = ctx->stream;
+    const size_t N = b->N;
+    if (U) HIPCHK(ctx, hipMemcpyAsync(U, b->U.p, N * 16, hipMemcpyDeviceToHost, s));
+    if (F) HIPCHK(ctx, hipMemcpyAsync(F, b->F.p, N * 32, hipMemcpyDeviceToHost, s));
+    if (Hpar) HIPCHK(ctx, hipMemcpyAsync(Hpar, b->H.p, N * 21 * 8, hipMemcpyDeviceToHost, s));
+    if (status) HIPCHK(ctx, hipMemcpyAsync(status, b->status.p, N * 4, hipMemcpyDeviceToHost, s));
+    HIPCHK(ctx, hipStreamSynchronize(s));
+    return STAQ_OK;
+}
+
+int staq_batch_get_ptrs(staq_batch *b, staq_batch_ptrs *o)
+{
+    if (!b || !o) return STAQ_EINVAL;
+    o->coords = b->coords.as<double>();
+    o->U0 = b->U0.as<double>();
+    o->UOld = b->UOld.as<double>();
+    o->FOld = b->FOld.as<double>();
+    o->vdual = b->vdual.as<double>();
+    o->U = b->U.as<double>();
+    o->F = b->F.as<double>();
+    o->Hpar = b->H.as<double>();
+    o->status = b->status.as<int>();
+    o->wx = b->wx.as<int>();
+    o->wy = b->wy.as<int>();
+    o->slot = b->slot.as<int>();
+    o->N = b->N;
+    o->wmax = b->wmax;
+    return STAQ_OK;
+}
+
+int staq_batch_bins(staq_batch *b, int *nbins, int *wcap, long *count, int *bs6, int *bs2,
+                    int maxbins)
+{
+    if (!b || !nbins) return STAQ_EINVAL;
+    *nbins = (int)b->bins.size();
+    for (int k = 0; k < (int)b->bins.size() && k < maxbins; ++k) {
+        if (wcap) wcap[k] = std::max(b->bins[k].wxcap, b->bins[k].wycap);
+        if (count) count[k] = b->bins[k].n;
+        if (bs6) bs6[k] = b->bins[k].bs[0];
+        if (bs2) bs2[k] = b->bins[k].bs[1];
+    }
+    return STAQ_OK;
+}
+
+}  // extern "C"
+
+static IcgnArgs base_args(staq_batch *b, const staq_params *p)
+{
+    IcgnArgs a;
+    std::memset(&a, 0, sizeof(a));
+    a.frames = b->ctx->d_frames.as<FrameDev>();
+    a.slot = b->has_slot ? b->slot.as<int>() : nullptr;
+    a.coords = b->coords.as<double>();
+    a.wx = b->wx.as<int>();
+    a.wy = b->wy.as<int>();
+    a.order = b->order.as<int>();
+    a.U0 = b->U0.as<double>();
+    a.UOld = b->UOld.as<double>();
+    a.FOld = b->FOld.as<double>();
+    a.vdual = b->vdual.as<double>();
+    a.U = b->U.as<double>();
+    a.F = b->F.as<double>();
+    a.H21 = b->H.as<double>();
+    a.status = b->status.as<int>();
+    a.N = b->N;
+    a.tol = p->tol;
+    a.mu = p->mu;
+    a.plain = p->plain;
+    a.max_it = p->max_iter > 0 ? p->max_iter : 100;
+    return a;
+}
+
+static int occupancy_grid(int bs, size_t lds, long n)
+{
+    // persistent grid: enough workgroups to fill 256 CUs at the LDS-limited occupancy
+    const size_t per_cu_lds = 160 * 1024;
+    int per_cu = (int)std::max<size_t>(1, per_cu_lds / std::max<size_t>(lds + 4096, 1));
+    per_cu = std::min(per_cu, std::max(1, 2048 / bs));
+    const long g = (long)per_cu * 256;
+    return (int)std::min<long>(n, g);
+}
+
+// Enqueue one local step (mode 6 or 2) on stream s; bin_only >= 0 launches a
+// single bin (autotuning) without reset or fallback.
+static int enqueue_step(staq_batch *b, const staq_params *p, int mode, hipStream_t s,
+                        int bin_only = -1, int bs_override = 0)
+{
+    staq_ctx *ctx = b->ctx;
+    const int nb = (int)b->bins.size();
+    int *ints = b->ints.as<int>();  // [0, nb) bin queue heads, nb fallback head, nb+1 defer flag
+    if (bin_only < 0)
+        hipLaunchKernelGGL(reset_ints_kernel, dim3(1), dim3(256), 0, s, ints, nb + 2);
+    else
+        hipLaunchKernelGGL(reset_ints_kernel, dim3(1), dim3(256), 0, s, ints + bin_only, 1);
+    HIPCHK(ctx, hipGetLastError());
+    const int mi = mode == 6 ? 0 : 1;
+    for (int k = 0; k < nb; ++k) {
+        if (bin_only >= 0 && k != bin_only) continue;
+        const Bin &bin = b->bins[k];
+        IcgnArgs a = base_args(b, p);
+        a.item0 = bin.item0;
+        a.n_items = bin.n;
+        a.npix_cap = bin.npix_cap;
+        a.tile_cap = bin.tile_cap;
+        a.defer = 1;
+        const int bs = bs_override ? bs_override : (p->block_size ? p->block_size : bin.bs[mi]);
+        int grid = (int)bin.n;
+        if (p->persistent) {
+            a.work_counter = ints + k;
+            grid = occupancy_grid(bs, icgn_lds_bytes(a.npix_cap, a.tile_cap), bin.n);
+        }
+        HIPCHK(ctx, icgn_launch(a, mode, bs, grid, s));
+        if (bin_only < 0) {
+            const unsigned g2 = (unsigned)((bin.n + 255) / 256);
+            hipLaunchKernelGGL(defer_flag_kernel, dim3(g2), dim3(256), 0, s, b->status.as<int>(),
+                               b->order.as<int>(), bin.item0, bin.n, ints + nb + 1);
+            HIPCHK(ctx, hipGetLastError());
+        }
+    }
+    if (bin_only < 0) {
+        // fallback over the whole batch for windows that outgrew their bin
+        IcgnArgs a = base_args(b, p);
+        a.item0 = 0;
+        a.n_items = b->N;
+        a.npix_cap = b->fb_npix_cap;
+        a.tile_cap = b->fb_tile_cap;
+        a.defer = 0;
+        a.only_deferred = 1;
+        a.work_counter = ints + nb;
+        const int bs = 256;
+        const int grid = occupancy_grid(bs, icgn_lds_bytes(a.npix_cap, a.tile_cap), b->N);
+        // The fallback grid is launched unconditionally (graph-capturable);
+        // blocks exit at once when the defer flag is clear.
+        a.n_items = b->N;
+        a.defer_flag = ints + nb + 1;
+        HIPCHK(ctx, icgn_launch(a, mode, bs, grid, s));
+    }
+    return STAQ_OK;
+}
+
+extern "C" {
+
+int staq_batch_icgn6(staq_batch *b, const staq_params *p)
+{
+    if (!b || !p) return STAQ_EINVAL;
+    hipSetDevice(b->ctx->device);
+    return enqueue_step(b, p, 6, b->ctx->stream);
+}
+
+int staq_batch_icgn2(staq_batch *b, const staq_params *p)
+{
+    if (!b || !p) return STAQ_EINVAL;
+    hipSetDevice(b->ctx->device);
+    return enqueue_step(b, p, 2, b->ctx->stream);
+}
+
+int staq_batch_icgn6_per_subset(staq_batch *b, const staq_params *p, double *kernel_ms_sum)
+{
+    if (!b || !p) return STAQ_EINVAL;
+    staq_ctx *ctx = b->ctx;
+    hipSetDevice(ctx->device);
+    hipStream_t s = ctx->stream;
+    // Every subset gets the largest caps so that no subset defers: this is the
+    // reference's one-subset-per-call structure with one workgroup per call.
+    const int npix_cap = b->fb_npix_cap, tile_cap = b->fb_tile_cap;
+    const int bs = p->block_size ? p->block_size : 256;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    if (kernel_ms_sum) {
+        *kernel_ms_sum = 0.0;
+        HIPCHK(ctx, hipEventCreate(&e0));
+        HIPCHK(ctx, hipEventCreate(&e1));
+    }
+    for (long k = 0; k < b->N; ++k) {
+        IcgnArgs a = base_args(b, p);
+        a.item0 = k;
+        a.n_items = 1;
+        a.npix_cap = npix_cap;
+        a.tile_cap = tile_cap;
+        if (kernel_ms_sum) HIPCHK(ctx, hipEventRecord(e0, s));
+        HIPCHK(ctx, icgn_launch(a, 6, bs, 1, s));
+        if (kernel_ms_sum) {
+            HIPCHK(ctx, hipEventRecord(e1, s));
+            HIPCHK(ctx, hipEventSynchronize(e1));
+            float ms = 0.f;
+            HIPCHK(ctx, hipEventElapsedTime(&ms, e0, e1));
+            *kernel_ms_sum += ms;
+        }
+    }
+    if (e0) hipEventDestroy(e0);
+    if (e1) hipEventDestroy(e1);
+    return STAQ_OK;
+}
+
+int staq_batch_chain_outputs(staq_batch *b)
+{
+    if (!b) return STAQ_EINVAL;
+    hipSetDevice(b->ctx->device);
+    const unsigned g = (unsigned)((4 * b->N + 255) / 256);
+    hipLaunchKernelGGL(chain_kernel, dim3(g), dim3(256), 0, b->ctx->stream, b->U.as<double>(),
+                       b->F.as<double>(), b->UOld.as<double>(), b->FOld.as<double>(),
+                       b->vdual.as<double>(), b->N);
+    HIPCHK(b->ctx, hipGetLastError());
+    return STAQ_OK;
+}
+
+int staq_batch_autotune(staq_batch *b, const staq_params *p, int mode)
+{
+    if (!b || !p || (mode != 6 && mode != 2)) return STAQ_EINVAL;
+    staq_ctx *ctx = b->ctx;
+    hipSetDevice(ctx->device);
+    hipStream_t s = ctx->stream;
+    hipEvent_t e0, e1;
+    HIPCHK(ctx, hipEventCreate(&e0));
+    HIPCHK(ctx, hipEventCreate(&e1));
+    const int cand[4] = {64, 128, 256, 512};
+    const int mi = mode == 6 ? 0 : 1;
+    int rc = STAQ_OK;
+    for (int k = 0; k < (int)b->bins.size() && rc == STAQ_OK; ++k) {
+        Bin &bin = b->bins[k];
+        float best = 1e30f;
+        int best_bs = bin.bs[mi];
+        for (int c : cand) {
+            if (c == 512 && bin.npix_cap < 1500) continue;
+            if (c == 64 && bin.npix_cap > 4500) continue;
+            float t[3];
+            for (int r = 0; r < 3 && rc == STAQ_OK; ++r) {
+                hipEventRecord(e0, s);
+                rc = enqueue_step(b, p, mode, s, k, c);
+                hipEventRecord(e1, s);
+                hipEventSynchronize(e1);
+                hipEventElapsedTime(&t[r], e0, e1);
+            }
+            if (rc != STAQ_OK) break;
+            std::sort(t, t + 3);
+            if (t[1] < best) {
+                best = t[1];
+                best_bs = c;
+            }
+        }
+        bin.bs[mi] = best_bs;
+    }
+    hipEventDestroy(e0);
+    hipEventDestroy(e1);
+    if (b->gexec) {  // a captured graph holds the old schedule
+        hipGraphExecDestroy(b->gexec);
+        hipGraphDestroy(b->graph);
+        b->gexec = nullptr;
+        b->graph = nullptr;
+    }
+    return rc;
+}
+
+int staq_batch_graph_capture(staq_batch *b, const staq_params *p, int mode)
+{
+    if (!b || !p || (mode != 6 && mode != 2)) return STAQ_EINVAL;
+    staq_ctx *ctx = b->ctx;
+    hipSetDevice(ctx->device);
+    if (b->gexec) {
+        hipGraphExecDestroy(b->gexec);
+        hipGraphDestroy(b->graph);
+        b->gexec = nullptr;
+        b->graph = nullptr;
+    }
+    hipStream_t s = ctx->stream;
+    HIPCHK(ctx, hipStreamSynchronize(s));
+    HIPCHK(ctx, hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+    const int rc = enqueue_step(b, p, mode, s);
+    hipGraph_t g = nullptr;
+    const hipError_t e = hipStreamEndCapture(s, &g);
+    if (rc != STAQ_OK) {
+        if (g) hipGraphDestroy(g);
+        return rc;
+    }
+    if (e != hipSuccess) return hipfail(ctx, e, "hipStreamEndCapture");
+    hipGraphExec_t ge = nullptr;
+    const hipError_t e2 = hipGraphInstantiate(&ge, g, nullptr, nullptr, 0);
+    if (e2 != hipSuccess) {
+        hipGraphDestroy(g);
+        return hipfail(ctx, e2, "hipGraphInstantiate");
+    }
+    b->graph = g;
+    b->gexec = ge;
+    b->graph_mode = mode;
+    return STAQ_OK;
+}
+
+int staq_batch_graph_launch(staq_batch *b)
+{
+    if (!b || !b->gexec) return fail(b ? b->ctx : nullptr, STAQ_EINVAL, "no captured graph");
+    hipSetDevice(b->ctx->device);
+    HIPCHK(b->ctx, hipGraphLaunch(b->gexec, b->ctx->stream));
+    return STAQ_OK;
+}
+
+// ---------------------------------------------------------------------------
+// host-pointer drop-ins built on a temporary batch
+// ---------------------------------------------------------------------------
+int staq_local_icgn(staq_ctx *ctx, const staq_params *p, long N, const double *U0,
+                    const double *coords, const long *winsize, const int *slot, double *U,
+                    double *F, double *Hpar, int *status)
+{
+    if (!ctx || !p || N <= 0 || !U0 || !coords || !winsize)
+        return fail(ctx, STAQ_EINVAL, "null argument");
+    staq_batch *b = nullptr;
+    int rc = staq_batch_create(ctx, &b, N, coords, winsize, nullptr, slot);
+    if (rc) return rc;
+    rc = staq_batch_set_input(b, STAQ_IN_U0, U0);
+    if (!rc) rc = staq_batch_icgn6(b, p);
+    if (!rc) rc = staq_batch_get_output(b, U, F, Hpar, status);
+    staq_batch_destroy(b);
+    return rc;
+}
+
+int staq_subpb1(staq_ctx *ctx, const staq_params *p, long N, const double *UOld,
+                const double *FOld, const double *udual, const double *vdual,
+                const double *coords, const long *wx, const long *wy, const int *slot,
+                double *U, int *status)
+{
+    (void)udual;
+    if (!ctx || !p || N <= 0 || !UOld || !FOld || !vdual || !coords || !wx)
+        return fail(ctx, STAQ_EINVAL, "null argument");
+    staq_batch *b = nullptr;
+    int rc = staq_batch_create(ctx, &b, N, coords, wx, wy, slot);
+    if (rc) return rc;
+    rc = staq_batch_set_input(b, STAQ_IN_UOLD, UOld);
+    if (!rc) rc = staq_batch_set_input(b, STAQ_IN_FOLD, FOld);
+    if (!rc) rc = staq_batch_set_input(b, STAQ_IN_VDUAL, vdual);
+    if (!rc) rc = staq_batch_icgn2(b, p);
+    if (!rc) rc = staq_batch_get_output(b, U, nullptr, nullptr, status);
+    staq_batch_destroy(b);
+    return rc;
+}
+
+// ---------------------------------------------------------------------------
+// ZNSSD forward / backward
+// ---------------------------------------------------------------------------
+static int znssd_dev(staq_ctx *ctx, long N, int wmax, const double *P, const double *coords,
+                     const int *winsize, const int *slot, const double *gcost, double *cost,
+                     double *gradP, int *status, hipStream_t s)
+{
+    if (!ctx || N <= 0 || wmax < 2 || !P || !coords || !winsize)
+        return fail(ctx, STAQ_EINVAL, "bad znssd arguments");
+    if (ctx->frames.empty()) return fail(ctx, STAQ_ENOFRAME, "no frame set");
+    const long np = (long)(wmax + 1) * (wmax + 1);
+    if (np * 12 > 150 * 1024) return fail(ctx, STAQ_EINVAL, "window too large for the adjoint kernel");
+    ZnssdArgs a;
+    a.frames = ctx->d_frames.as<FrameDev>();
+    a.slot = slot;
+    a.coords = coords;
+    a.winsize = winsize;
+    a.P = P;
+    a.gcost = gcost;
+    a.cost = cost;
+    a.gradP = gradP;
+    a.status = status;
+    a.N = N;
+    a.npix_cap = (int)np;
+    HIPCHK(ctx, znssd_launch(a, s));
+    return STAQ_OK;
+}
+
+int staq_znssd_fwd_dev(staq_ctx *ctx, long N, int wmax, const double *P, const double *coords,
+                       const int *winsize, const int *slot, double *cost, int *status,
+                       void *stream)
+{
+    if (!ctx) return STAQ_EINVAL;
+    hipSetDevice(ctx->device);
+    return znssd_dev(ctx, N, wmax, P, coords, winsize, slot, nullptr, cost, nullptr, status,
+                     pick_stream(ctx, stream));
+}
+
+int staq_znssd_bwd_dev(staq_ctx *ctx, long N, int wmax, const double *P, const double *coords,
+                       const int *winsize, const int *slot, const double *gcost, double *gradP,
+                       void *stream)
+{
+    if (!ctx) return STAQ_EINVAL;
+    hipSetDevice(ctx->device);
+    return znssd_dev(ctx, N, wmax, P, coords, winsize, slot, gcost, nullptr, gradP, nullptr,
+                     pick_stream(ctx, stream));
+}
+
+static int znssd_host(staq_ctx *ctx, long N, const double *P, const double *coords,
+                      const long *winsize, const int *slot, const double *gcost, double *cost,
+                      int *status, double *gradP, double *gradU)
+{
+    if (!ctx || N <= 0 || !P || !coords || !winsize) return fail(ctx, STAQ_EINVAL, "null argument");
+    hipSetDevice(ctx->device);
+    std::vector<int> w(N), sl(slot ? N : 0);
+    int wmax = 2;
+    for (long i = 0; i < N; ++i) {
+        w[i] = (int)winsize[i];
+        wmax = std::max(wmax, w[i]);
+        if (slot) {
+            sl[i] = slot[i];
+            if (sl[i] < 0 || (size_t)sl[i] >= ctx->frames.size() || !ctx->frames[sl[i]].set)
+                return fail(ctx, STAQ_ENOFRAME, "subset refers to a frame slot that is not set");
+        }
+    }
+    hipStream_t s = ctx->stream;
+    HIPCHK(ctx, ctx->s[0].ensure(N * 48));
+    HIPCHK(ctx, ctx->s[1].ensure(N * 16));
+    HIPCHK(ctx, ctx->s[2].ensure(N * 4));
+    HIPCHK(ctx, ctx->s[3].ensure(N * 4));
+    HIPCHK(ctx, ctx->s[4].ensure(N * 8));
+    HIPCHK(ctx, ctx->s[5].ensure(N * 8));
+    HIPCHK(ctx, ctx->s[6].ensure(N * 48));
+    HIPCHK(ctx, ctx->s[7].ensure(N * 4));
+    HIPCHK(ctx, hipMemcpyAsync(ctx->s[0].p, P, N * 48, hipMemcpyHostToDevice, s));
+    HIPCHK(ctx, hipMemcpyAsync(ctx->s[1].p, coords, N * 16, hipMemcpyHostToDevice, s));
+    HIPCHK(ctx, hipMemcpyAsync(ctx->s[2].p, w.data(), N * 4, hipMemcpyHostToDevice, s));
+    if (slot) HIPCHK(ctx, hipMemcpyAsync(ctx->s[3].p, sl.data(), N * 4, hipMemcpyHostToDevice, s));
+    if (gcost) HIPCHK(ctx, hipMemcpyAsync(ctx->s[4].p, gcost, N * 8, hipMemcpyHostToDevice, s));
+    const int *dsl = slot ? ctx->s[3].as<int>() : nullptr;
+    int rc;
+    if (cost || status) {
+        rc = znssd_dev(ctx, N, wmax, ctx->s[0].as<double>(), ctx->s[1].as<double>(),
+                       ctx->s[2].as<int>(), dsl, nullptr, ctx->s[5].as<double>(), nullptr,
+                       ctx->s[7].as<int>(), s);
+        if (rc) return rc;
+        if (cost) HIPCHK(ctx, hipMemcpyAsync(cost, ctx->s[5].p, N * 8, hipMemcpyDeviceToHost, s));
+        if (status) HIPCHK(ctx, hipMemcpyAsync(status, ctx->s[7].p, N * 4, hipMemcpyDeviceToHost, s));
+    }
+    if (gradP || gradU) {
+        rc = znssd_dev(ctx, N, wmax, ctx->s[0].as<double>(), ctx->s[1].as<double>(),
+                       ctx->s[2].as<int>(), dsl, gcost ? ctx->s[4].as<double>() : nullptr, nullptr,
+                       ctx->s[6].as<double>(), nullptr, s);
+        if (rc) return rc;
+        std::vector<double> gp(6 * N);
+        HIPCHK(ctx, hipMemcpyAsync(gp.data(), ctx->s[6].p, N * 48, hipMemcpyDeviceToHost, s));
+        HIPCHK(ctx, hipStreamSynchronize(s));
+        if (gradP) std::memcpy(gradP, gp.data(), N * 48);
+        if (gradU)
+            for (long i = 0; i < N; ++i) {
+                gradU[2 * i] = gp[6 * i + 4];
+                gradU[2 * i + 1] = gp[6 * i + 5];
+            }
+    }
+    HIPCHK(ctx, hipStreamSynchronize(s));
+    return STAQ_OK;
+}
+
+int staq_znssd_fwd(staq_ctx *ctx, long N, const double *P, const double *coords,
+                   const long *winsize, const int *slot, double *cost, int *status)
+{
+    return znssd_host(ctx, N, P, coords, winsize, slot, nullptr, cost, status, nullptr, nullptr);
+}
+
+int staq_znssd_bwd(staq_ctx *ctx, long N, const double *P, const double *coords,
+                   const long *winsize, const int *slot, const double *gcost, double *gradP,
+                   double *gradU)
+{
+    return znssd_host(ctx, N, P, coords, winsize, slot, gcost, nullptr, nullptr, gradP, gradU);
+}
+
+}  // extern "C"

@@ -1,0 +1,283 @@
+// Device-side types and helpers shared by the STAQ-DIC HIP kernels (gfx950).
+//
+// Layout conventions (identical to the reference after its transpose on load,
+// func_quadtree/ReadImageQuadtree.m:78): images are column-major with dim 1 = x
+// (H entries, contiguous) and dim 2 = y (W entries); coordinates are 1-based.
+// Gradient arrays are (H-6) x (W-6), element (i, j) belongs to pixel (i+3, j+3)
+// (func/funImgGradient.m:53-78).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace staq {
+
+// One reference/deformed frame pair resident in HBM (FP32).
+struct FrameDev {
+    const float *f;     // reference image (normalised), H x W
+    const float *g;     // deformed image, H x W
+    const float *mask;  // reference mask (0/1), H x W
+    const float *dfdx;  // (H-6) x (W-6), masked
+    const float *dfdy;  // (H-6) x (W-6), masked
+    int H, W;
+};
+
+// Status codes of the reference (funICGNQuadtree.m:422-442, LocalICGNQuadtree.m:92-97).
+enum : int { ST_DEFER = -2, ST_NOTCONV = 101, ST_MASKED = 102, ST_FEATURELESS = 103, ST_ERROR = -1 };
+
+constexpr int kWave = 64;
+
+__device__ __forceinline__ float fidx(const float *a, int H, int x, int y)
+{
+    // 1-based (x, y) in an H-row column-major array
+    return a[(x - 1) + (size_t)(y - 1) * (size_t)H];
+}
+
+// Keys cubic-convolution weights, a = -0.5 (ba_interp2.cpp:204-213).
+template <typename T>
+__device__ __forceinline__ void keys_w(T t, T w[4])
+{
+    const T t2 = t * t, t3 = t2 * t;
+    w[0] = T(0.5) * (-t + T(2) * t2 - t3);
+    w[1] = T(0.5) * (T(2) - T(5) * t2 + T(3) * t3);
+    w[2] = T(0.5) * (t + T(4) * t2 - T(3) * t3);
+    w[3] = T(0.5) * (-t2 + t3);
+}
+
+template <typename T>
+__device__ __forceinline__ void keys_dw(T t, T w[4])
+{
+    const T t2 = t * t;
+    w[0] = T(0.5) * (T(-1) + T(4) * t - T(3) * t2);
+    w[1] = T(0.5) * (T(-10) * t + T(9) * t2);
+    w[2] = T(0.5) * (T(1) + T(8) * t - T(9) * t2);
+    w[3] = T(0.5) * (T(-2) * t + T(3) * t2);
+}
+
+template <typename T>
+__device__ __forceinline__ T wave_sum(T v)
+{
+#pragma unroll
+    for (int o = kWave / 2; o > 0; o >>= 1) v += __shfl_xor(v, o, kWave);
+    return v;
+}
+
+// Sum K values over the block.  Every thread passes its partials in v[];
+// the totals land in out[] of thread 0 only.  red must hold K * (BS/64).
+template <int BS, int K, typename T>
+__device__ __forceinline__ void block_sum(T (&v)[K], T *red, T (&out)[K])
+{
+    const int lane = threadIdx.x & (kWave - 1), wid = threadIdx.x / kWave;
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+        const T s = wave_sum(v[k]);
+        if (lane == 0) red[wid * K + k] = s;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+#pragma unroll
+        for (int k = 0; k < K; ++k) {
+            T s = red[k];
+#pragma unroll
+            for (int w = 1; w < BS / kWave; ++w) s += red[w * K + k];
+            out[k] = s;
+        }
+    }
+}
+
+// MATLAB mldivide for a small symmetric system: Cholesky when the diagonal is
+// positive and the factorisation succeeds, otherwise LU with partial pivoting.
+// A is n x n column-major; b and x have n entries.
+template <int N>
+__device__ inline void solve_small(const double *A_in, const double *b, double *x)
+{
+    double A[N * N], L[N * N];
+    for (int i = 0; i < N * N; ++i) A[i] = A_in[i];
+    bool spd = true;
+    for (int i = 0; i < N; ++i)
+        if (!(A[i + i * N] > 0)) spd = false;
+    if (spd) {
+        for (int i = 0; i < N * N; ++i) L[i] = 0.0;
+        for (int j = 0; j < N && spd; ++j) {
+            double d = A[j + j * N];
+            for (int k = 0; k < j; ++k) d -= L[j + k * N] * L[j + k * N];
+            if (!(d > 0)) {
+                spd = false;
+                break;
+            }
+            const double ljj = sqrt(d);
+            L[j + j * N] = ljj;
+            for (int i = j + 1; i < N; ++i) {
+                double s = A[i + j * N];
+                for (int k = 0; k < j; ++k) s -= L[i + k * N] * L[j + k * N];
+                L[i + j * N] = s / ljj;
+            }
+        }
+        if (spd) {
+            double y[N];
+            for (int i = 0; i < N; ++i) {
+                double s = b[i];
+                for (int k = 0; k < i; ++k) s -= L[i + k * N] * y[k];
+                y[i] = s / L[i + i * N];
+            }
+            for (int i = N - 1; i >= 0; --i) {
+                double s = y[i];
+                for (int k = i + 1; k < N; ++k) s -= L[k + i * N] * x[k];
+                x[i] = s / L[i + i * N];
+            }
+            return;
+        }
+    }
+    double bb[N];
+    for (int i = 0; i < N; ++i) bb[i] = b[i];
+    for (int k = 0; k < N; ++k) {
+        int p = k;
+        double best = fabs(A[k + k * N]);
+        for (int i = k + 1; i < N; ++i)
+            if (fabs(A[i + k * N]) > best) {
+                best = fabs(A[i + k * N]);
+                p = i;
+            }
+        if (p != k) {
+            for (int j = 0; j < N; ++j) {
+                const double t = A[k + j * N];
+                A[k + j * N] = A[p + j * N];
+                A[p + j * N] = t;
+            }
+            const double t = bb[k];
+            bb[k] = bb[p];
+            bb[p] = t;
+        }
+        for (int i = k + 1; i < N; ++i) {
+            const double l = A[i + k * N] / A[k + k * N];
+            A[i + k * N] = l;
+            for (int j = k + 1; j < N; ++j) A[i + j * N] -= l * A[k + j * N];
+            bb[i] -= l * bb[k];
+        }
+    }
+    for (int i = N - 1; i >= 0; --i) {
+        double s = bb[i];
+        for (int j = i + 1; j < N; ++j) s -= A[i + j * N] * x[j];
+        x[i] = s / A[i + i * N];
+    }
+}
+
+// The same factorisation kept in LDS: A (N x N, column-major) is overwritten
+// by its Cholesky factor (kind 0) or by LU factors with row pivots (kind 1).
+// Called by one thread; the system matrix is constant over the iterations, so
+// it is factored once per subset and each iteration only substitutes.
+template <int N>
+__device__ inline void factor_lds(double *A, double *work, int *piv, int *kind)
+{
+    // work (N x N, LDS) receives the trial Cholesky factor so that A stays
+    // intact for the LU fallback
+    bool spd = true;
+    for (int i = 0; i < N; ++i)
+        if (!(A[i + i * N] > 0)) spd = false;
+    if (spd) {
+        for (int j = 0; j < N && spd; ++j) {
+            double d = A[j + j * N];
+            for (int k = 0; k < j; ++k) d -= work[j + k * N] * work[j + k * N];
+            if (!(d > 0)) {
+                spd = false;
+                break;
+            }
+            const double ljj = sqrt(d);
+            work[j + j * N] = ljj;
+            for (int i = j + 1; i < N; ++i) {
+                double s = A[i + j * N];
+                for (int k = 0; k < j; ++k) s -= work[i + k * N] * work[j + k * N];
+                work[i + j * N] = s / ljj;
+            }
+        }
+        if (spd) {
+            for (int j = 0; j < N; ++j)
+                for (int i = j; i < N; ++i) A[i + j * N] = work[i + j * N];
+            *kind = 0;
+            return;
+        }
+    }
+    for (int i = 0; i < N; ++i) piv[i] = i;
+    for (int k = 0; k < N; ++k) {
+        int p = k;
+        double best = fabs(A[k + k * N]);
+        for (int i = k + 1; i < N; ++i)
+            if (fabs(A[i + k * N]) > best) {
+                best = fabs(A[i + k * N]);
+                p = i;
+            }
+        if (p != k) {
+            for (int j = 0; j < N; ++j) {
+                const double t = A[k + j * N];
+                A[k + j * N] = A[p + j * N];
+                A[p + j * N] = t;
+            }
+            const int t = piv[k];
+            piv[k] = piv[p];
+            piv[p] = t;
+        }
+        for (int i = k + 1; i < N; ++i) {
+            const double l = A[i + k * N] / A[k + k * N];
+            A[i + k * N] = l;
+            for (int j = k + 1; j < N; ++j) A[i + j * N] -= l * A[k + j * N];
+        }
+    }
+    *kind = 1;
+}
+
+template <int N>
+__device__ inline void solve_factored(const double *A, const int *piv, int kind, const double *b,
+                                      double *x)
+{
+    double y[N];
+    if (kind == 0) {
+        for (int i = 0; i < N; ++i) {
+            double s = b[i];
+            for (int k = 0; k < i; ++k) s -= A[i + k * N] * y[k];
+            y[i] = s / A[i + i * N];
+        }
+        for (int i = N - 1; i >= 0; --i) {
+            double s = y[i];
+            for (int k = i + 1; k < N; ++k) s -= A[k + i * N] * x[k];
+            x[i] = s / A[i + i * N];
+        }
+    } else {
+        for (int i = 0; i < N; ++i) {
+            double s = b[piv[i]];
+            for (int k = 0; k < i; ++k) s -= A[i + k * N] * y[k];
+            y[i] = s;
+        }
+        for (int i = N - 1; i >= 0; --i) {
+            double s = y[i];
+            for (int k = i + 1; k < N; ++k) s -= A[i + k * N] * x[k];
+            x[i] = s / A[i + i * N];
+        }
+    }
+}
+
+// Inverse-compositional update W(P) <- W(P) o W(dP)^-1 with the closed-form
+// inverse of funICGNQuadtree.m:393-411.  Returns false if det(dP) == 0.
+__device__ inline bool ic_compose(double P[6], const double d[6])
+{
+    const double det = (1 + d[0]) * (1 + d[3]) - d[1] * d[2];
+    if (!(det != 0)) return false;
+    const double q1 = (-d[0] - d[0] * d[3] + d[1] * d[2]) / det;
+    const double q2 = -d[1] / det;
+    const double q3 = -d[2] / det;
+    const double q4 = (-d[3] - d[0] * d[3] + d[1] * d[2]) / det;
+    const double q5 = (-d[4] - d[3] * d[4] + d[2] * d[5]) / det;
+    const double q6 = (-d[5] - d[0] * d[5] + d[1] * d[4]) / det;
+    const double a11 = 1 + P[0], a12 = P[2], a13 = P[4];
+    const double a21 = P[1], a22 = 1 + P[3], a23 = P[5];
+    const double b11 = 1 + q1, b12 = q3, b13 = q5;
+    const double b21 = q2, b22 = 1 + q4, b23 = q6;
+    P[0] = (a11 * b11 + a12 * b21) - 1;
+    P[1] = a21 * b11 + a22 * b21;
+    P[2] = a11 * b12 + a12 * b22;
+    P[3] = (a21 * b12 + a22 * b22) - 1;
+    P[4] = a11 * b13 + a12 * b23 + a13;
+    P[5] = a21 * b13 + a22 * b23 + a23;
+    return true;
+}
+
+}  // namespace staq

@@ -1,0 +1,263 @@
+// Bicubic gather (ba_interp2 drop-in), reference-image gradient, and the
+// ZNSSD forward/backward (adjoint) kernels for gfx950.
+#include "kernels.h"
+
+namespace staq {
+
+// ---------------------------------------------------------------------------
+// ba_interp2.cpp:184-234: Keys bicubic, taps clamped to the array (:60-130).
+// F is M x N (x P layers), X = column (dim-2) coordinate, Y = row (dim-1)
+// coordinate, 1-based; out is nd x P.  One thread per sample point; the
+// clamp-free interior case is the common one and reads 4 contiguous rows.
+// ---------------------------------------------------------------------------
+template <typename T>
+__global__ __launch_bounds__(256) void ba_interp2_kernel(const T *__restrict__ F, long M, long N,
+                                                         long P, const T *__restrict__ X,
+                                                         const T *__restrict__ Y, long nd,
+                                                         T *__restrict__ out)
+{
+    const long i = blockIdx.x * (long)blockDim.x + threadIdx.x;
+    if (i >= nd) return;
+    const T x = X[i], y = Y[i];
+    if (isnan(x) || isnan(y)) {
+        for (long p = 0; p < P; ++p) out[i + p * nd] = T(NAN);
+        return;
+    }
+    const T xf = floor(x), yf = floor(y);
+    T wx[4], wy[4];
+    keys_w(x - xf, wx);
+    keys_w(y - yf, wy);
+    // 0-based index of the floor pixel; out-of-range values are clamped, as
+    // the reference does, after conversion to integer
+    const double xfd = (double)xf - 1.0, yfd = (double)yf - 1.0;
+    const long cx = (long)fmin(fmax(xfd, -4.0), (double)N + 4.0);
+    const long cy = (long)fmin(fmax(yfd, -4.0), (double)M + 4.0);
+    long col[4], row[4];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        col[k] = min(max(cx - 1 + k, 0L), N - 1);
+        row[k] = min(max(cy - 1 + k, 0L), M - 1);
+    }
+    const long LO = M * N;
+    for (long p = 0; p < P; ++p) {
+        const T *Fp = F + p * LO;
+        T acc = 0;
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            T s = 0;
+#pragma unroll
+            for (int k = 0; k < 4; ++k) s += wx[k] * Fp[row[j] + col[k] * M];
+            acc += wy[j] * s;
+        }
+        out[i + p * nd] = acc;
+    }
+}
+
+hipError_t ba_interp2_launch(const void *F, long M, long N, long P, const void *X, const void *Y,
+                             long nd, void *out, bool single, hipStream_t s)
+{
+    if (nd <= 0) return hipSuccess;
+    const unsigned grid = (unsigned)((nd + 255) / 256);
+    if (single)
+        hipLaunchKernelGGL(ba_interp2_kernel<float>, dim3(grid), dim3(256), 0, s, (const float *)F, M,
+                           N, P, (const float *)X, (const float *)Y, nd, (float *)out);
+    else
+        hipLaunchKernelGGL(ba_interp2_kernel<double>, dim3(grid), dim3(256), 0, s, (const double *)F,
+                           M, N, P, (const double *)X, (const double *)Y, nd, (double *)out);
+    return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
+// func/funImgGradient.m:53-78: 7-tap central difference (imfilter correlation
+// with [-1/60 3/20 -3/4 0 3/4 -3/20 1/60]), cropped by 3, times the mask.
+// Thread (x, y) of the (H-6) x (W-6) output; consecutive threads walk dim 1 so
+// both the dim-1 taps and the output are coalesced.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void img_gradient_kernel(const float *__restrict__ f,
+                                                           const float *__restrict__ mask, int H,
+                                                           int W, float *__restrict__ dfdx,
+                                                           float *__restrict__ dfdy)
+{
+    const int Hd = H - 6, Wd = W - 6;
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;  // 0..Hd-1
+    const int j = blockIdx.y;                             // 0..Wd-1
+    if (i >= Hd || j >= Wd) return;
+    const int x = i + 3, y = j + 3;  // 0-based pixel
+    const float c1 = 3.0f / 4.0f, c2 = -3.0f / 20.0f, c3 = 1.0f / 60.0f;
+    const float *col = f + (size_t)y * H;
+    const float sx = c1 * (col[x + 1] - col[x - 1]) + c2 * (col[x + 2] - col[x - 2]) +
+                     c3 * (col[x + 3] - col[x - 3]);
+    const float sy = c1 * (f[x + (size_t)(y + 1) * H] - f[x + (size_t)(y - 1) * H]) +
+                     c2 * (f[x + (size_t)(y + 2) * H] - f[x + (size_t)(y - 2) * H]) +
+                     c3 * (f[x + (size_t)(y + 3) * H] - f[x + (size_t)(y - 3) * H]);
+    const float m = mask[x + (size_t)y * H];
+    dfdx[i + (size_t)j * Hd] = sx * m;
+    dfdy[i + (size_t)j * Hd] = sy * m;
+}
+
+hipError_t img_gradient_launch(const float *f, const float *mask, int H, int W, float *dfdx,
+                               float *dfdy, hipStream_t s)
+{
+    if (H < 7 || W < 7) return hipErrorInvalidValue;
+    dim3 grid((unsigned)((H - 6 + 255) / 256), (unsigned)(W - 6));
+    hipLaunchKernelGGL(img_gradient_kernel, grid, dim3(256), 0, s, f, mask, H, W, dfdx, dfdy);
+    return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
+// ZNSSD mismatch and its adjoint.
+//   C_i = sum_p (fhat_p - ghat_p)^2 = 2 - 2 ZNCC_i,   fhat = (f - fbar)/||f - fbar||,
+//   ghat likewise with g_p = bicubic(g)(u22_p, v22_p) and whole-window
+//   statistics.  The vector-Jacobian product w.r.t. the warp parameters is
+//   dC/dP = sum_p dC/dg_p (dg/du u_P + dg/dv v_P),
+//   dC/dg_p = -(2/||g - gbar||) (fhat_p - ZNCC ghat_p),
+//   u_P = [dx 0 dy 0 1 0], v_P = [0 dx 0 dy 0 1].
+// One workgroup per subset.  Pass 1 gathers g and its two derivatives (Keys
+// kernel and its derivative on the same 16 taps) into LDS and reduces the
+// five moments; pass 2 (backward only) forms the gradient.  Sums are FP32
+// per lane and FP64 across the block.
+// ---------------------------------------------------------------------------
+
+
+template <int BS>
+__global__ __launch_bounds__(BS) void znssd_kernel(ZnssdArgs a)
+{
+    constexpr int NW = BS / kWave;
+    extern __shared__ __align__(16) unsigned char smem[];
+    __shared__ double redd[NW * 6];
+    __shared__ double stat[8];
+    __shared__ int ok_sh;
+    float *gv = reinterpret_cast<float *>(smem);
+    float *gu = gv + a.npix_cap;
+    float *gw = gu + a.npix_cap;
+    const long idx = blockIdx.x;
+    const int tid = threadIdx.x;
+    const FrameDev fr = a.frames[a.slot ? a.slot[idx] : 0];
+    const int w = a.winsize[idx], nx = w + 1, npix = nx * nx;
+    const double x0 = round(a.coords[idx]), y0 = round(a.coords[idx + a.N]);
+    const double *P = a.P + 6 * idx;
+    if (tid == 0) {
+        // same window rule as the IC-GN kernels (gradient array bounds)
+        bool ok = npix <= a.npix_cap && x0 - w / 2 >= 4 && y0 - w / 2 >= 4 &&
+                  x0 + w / 2 <= fr.H - 3 && y0 + w / 2 <= fr.W - 3;
+        const double h = w * 0.5;
+        for (int q = 0; q < 4 && ok; ++q) {
+            const double dx = (q & 1) ? h : -h, dy = (q & 2) ? h : -h;
+            const double u = (1 + P[0]) * dx + P[2] * dy + (x0 + P[4]);
+            const double v = P[1] * dx + (1 + P[3]) * dy + (y0 + P[5]);
+            if (!(u >= 3 && u <= fr.H - 2 && v >= 3 && v <= fr.W - 2)) ok = false;
+        }
+        ok_sh = ok;
+    }
+    __syncthreads();
+    if (!ok_sh) {
+        if (tid == 0) {
+            if (a.cost) a.cost[idx] = 0.0;
+            if (a.gradP)
+                for (int k = 0; k < 6; ++k) a.gradP[6 * idx + k] = 0.0;
+            if (a.status) a.status[idx] = 1;
+        }
+        return;
+    }
+    const int xa = (int)x0 - w / 2, ya = (int)y0 - w / 2;
+    // local warp relative to an integer origin near the window keeps the FP32
+    // coordinates small
+    const int ou = (int)floor(x0 + P[4]), ov = (int)floor(y0 + P[5]);
+    const float a11 = (float)(1 + P[0]), a12 = (float)P[2], a21 = (float)P[1], a22 = (float)(1 + P[3]);
+    const float cu = (float)((x0 + P[4]) - ou), cv = (float)((y0 + P[5]) - ov);
+    const float rcp = 1.0f / nx, h = w * 0.5f;
+    double s[5] = {0, 0, 0, 0, 0};  // sum f, f^2, g, g^2, f g
+    for (int p = tid; p < npix; p += BS) {
+        const int j = (int)(((float)p + 0.5f) * rcp), i = p - j * nx;
+        const float dx = (float)i - h, dy = (float)j - h;
+        const float lu = fmaf(a11, dx, fmaf(a12, dy, cu));
+        const float lv = fmaf(a21, dx, fmaf(a22, dy, cv));
+        const float fu = floorf(lu), fv = floorf(lv);
+        float wu[4], wv[4], du[4], dv[4];
+        keys_w(lu - fu, wu);
+        keys_w(lv - fv, wv);
+        keys_dw(lu - fu, du);
+        keys_dw(lv - fv, dv);
+        const int r0 = ou + (int)fu - 1, c0 = ov + (int)fv - 1;  // 1-based first tap
+        float g = 0.f, gdu = 0.f, gdv = 0.f;
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            const int y = min(max(c0 + q, 1), fr.W);
+            const float *col = fr.g + (size_t)(y - 1) * fr.H;
+            float sv = 0.f, sd = 0.f;
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const float t = col[min(max(r0 + r, 1), fr.H) - 1];
+                sv = fmaf(wu[r], t, sv);
+                sd = fmaf(du[r], t, sd);
+            }
+            g = fmaf(wv[q], sv, g);
+            gdu = fmaf(wv[q], sd, gdu);
+            gdv = fmaf(dv[q], sv, gdv);
+        }
+        gv[p] = g;
+        gu[p] = gdu;
+        gw[p] = gdv;
+        const double f = fidx(fr.f, fr.H, xa + i, ya + j);
+        s[0] += f;
+        s[1] += f * f;
+        s[2] += g;
+        s[3] += (double)g * g;
+        s[4] += f * g;
+    }
+    double o[5];
+    block_sum<BS, 5>(s, redd, o);
+    if (tid == 0) {
+        const double n = npix;
+        const double mf = o[0] / n, mg = o[2] / n;
+        const double bf = sqrt(fmax(o[1] - o[0] * o[0] / n, 0.0));
+        const double bg = sqrt(fmax(o[3] - o[2] * o[2] / n, 0.0));
+        const double zncc = (o[4] - o[0] * o[2] / n) / (bf * bg);
+        stat[0] = mf;
+        stat[1] = bf;
+        stat[2] = mg;
+        stat[3] = bg;
+        stat[4] = zncc;
+        if (a.cost) a.cost[idx] = 2.0 - 2.0 * zncc;
+        if (a.status) a.status[idx] = 0;
+    }
+    __syncthreads();
+    if (!a.gradP) return;
+    const double mf = stat[0], bf = stat[1], mg = stat[2], bg = stat[3], zncc = stat[4];
+    const double up = a.gcost ? a.gcost[idx] : 1.0;
+    double gacc[6] = {0, 0, 0, 0, 0, 0};
+    for (int p = tid; p < npix; p += BS) {
+        const int j = (int)(((float)p + 0.5f) * rcp), i = p - j * nx;
+        const double fh = ((double)fidx(fr.f, fr.H, xa + i, ya + j) - mf) / bf;
+        const double gh = ((double)gv[p] - mg) / bg;
+        const double dCdg = -(2.0 / bg) * (fh - zncc * gh);
+        const double au = dCdg * gu[p], av = dCdg * gw[p];
+        const double dx = i - (double)h, dy = j - (double)h;
+        gacc[0] += au * dx;
+        gacc[1] += av * dx;
+        gacc[2] += au * dy;
+        gacc[3] += av * dy;
+        gacc[4] += au;
+        gacc[5] += av;
+    }
+    double go[6];
+    block_sum<BS, 6>(gacc, redd, go);
+    if (tid == 0)
+        for (int k = 0; k < 6; ++k) a.gradP[6 * idx + k] = up * go[k];
+}
+
+hipError_t znssd_launch(const ZnssdArgs &a, hipStream_t s)
+{
+    if (a.N <= 0) return hipSuccess;
+    const size_t lds = 3 * sizeof(float) * (size_t)a.npix_cap;
+    static bool attr = false;
+    if (!attr) {
+        hipFuncSetAttribute((const void *)znssd_kernel<256>,
+                            hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024 - 8192);
+        attr = true;
+    }
+    hipLaunchKernelGGL(znssd_kernel<256>, dim3((unsigned)a.N), dim3(256), lds, s, a);
+    return hipGetLastError();
+}
+
+}  // namespace staq

@@ -1,0 +1,882 @@
+// Batched inverse-compositional Gauss-Newton (IC-GN) subset fitting for gfx950.
+//
+//   MODE 6: funICGNQuadtree.m:1-451 (6-DOF affine, ZNSSD) and, with plain = 1,
+//           the uniform-grid twin func/funICGN.m:1-323.
+//   MODE 2: funICGN_Subpb1.m:1-474 (translation-only step of the augmented-
+//           Lagrangian loop with penalty mu, affine part fixed at FOld).
+//
+// One workgroup fits one subset; a launch covers a whole batch (all quadtree
+// nodes of one ADMM iteration, possibly across frame pairs).  With a work
+// counter the grid is persistent and pulls subsets from a device queue.
+//
+// Per subset:
+//   setup   - load the (w+1)^2 window of f*mask, the masked-pixel count
+//             decides 102 / window growth; 4-connected region of the mask
+//             through the seed (bwselect); FP64 sums for H (21 entries), the
+//             f statistics and the constant parts of the steepest-descent
+//             vector.
+//   iterate - corner bounds check in FP64 (the warp is affine so the
+//             extremes of u22/v22 are at the window corners), stage the
+//             bounding tile of g in LDS with coalesced loads, Keys bicubic
+//             from LDS, and ONE fused FP32 pass giving count, sum(g-c),
+//             sum((g-c)^2) and sum(J_k (g-c)); thread 0 turns those into
+//             b = bottomf * sum(J r) in FP64, runs the norm test, solves
+//             H dP = -b (FP64 Cholesky, LU fallback) and composes the warp.
+//   masked  - subsets that touch the mask (or whose g has exact zeros) take
+//             the exact path: bwselect flood fills in LDS and two-pass FP64
+//             statistics, following the reference line by line.
+#include "kernels.h"
+
+namespace staq {
+
+
+
+struct Ctl {
+    double P[6];
+    double x0, y0;
+    double Hm[36];     // factor of H (6-DOF) / scratch (2-DOF)
+    double Lw[36];     // trial Cholesky factor
+    int piv[6], fkind;
+    double H21[21];
+    double meanf, bottomf, sabs;
+    double SJ[6], FJ[6];
+    double normNew, normAbs, normInit;
+    double tH[4];      // 2-DOF damped system
+    double bvec[6];    // steepest-descent vector of the current iteration
+    double gmean, bottomg, ng, nf;
+    float cf, a11, a12, a21, a22, cu, cv;
+    int tile_u0, tile_v0, tnx, tny, use_tile;
+    int xa, ya, wx, wy, nx, ny, npix, w0;
+    int count, it, state, status, sum_bw2;
+    long idx;
+    int slot;
+    int path;          // 0 fused, 1 masked (flood fill + exact), 2 exact without flood fill
+    int fsel;          // 2-DOF: f, grad f, H come from the last g region (funICGN_Subpb1.m:171-190)
+    int nanP;
+    int sel0_all;
+    int all_sel;
+    long item;
+};
+
+__device__ __forceinline__ bool win_ok(const FrameDev &fr, double x0, double y0, int wx, int wy)
+{
+    const double xa = x0 - wx * 0.5, xb = x0 + wx * 0.5, ya = y0 - wy * 0.5, yb = y0 + wy * 0.5;
+    if (xa != floor(xa) || ya != floor(ya) || xb != floor(xb) || yb != floor(yb)) return false;
+    return xa >= 4 && ya >= 4 && xb <= fr.H - 3 && yb <= fr.W - 3;
+}
+
+// Row/column of pixel p in a column-major window with nx rows.
+__device__ __forceinline__ void pix_ij(int p, int nx, float rcp, int &i, int &j)
+{
+    j = (int)(((float)p + 0.5f) * rcp);
+    i = p - j * nx;
+}
+
+// Connected component of bw through (seed_r, seed_c) (1-based, row = dim 1),
+// 4- or 8-connected, by iterated propagation in LDS.
+template <int BS>
+__device__ void flood_fill(const unsigned char *bw, unsigned char *out, int nr, int nc,
+                           int seed_r, int seed_c, int conn, int npix, float rcp)
+{
+    for (int p = threadIdx.x; p < npix; p += BS) out[p] = 0;
+    __syncthreads();
+    if (threadIdx.x == 0 && seed_r >= 1 && seed_r <= nr && seed_c >= 1 && seed_c <= nc) {
+        const int s = (seed_r - 1) + (seed_c - 1) * nr;
+        if (bw[s]) out[s] = 1;
+    }
+    __syncthreads();
+    for (;;) {
+        int changed = 0;
+        for (int p = threadIdx.x; p < npix; p += BS) {
+            if (!bw[p] || out[p]) continue;
+            int i, j;
+            pix_ij(p, nr, rcp, i, j);
+            bool hit = false;
+            for (int dj = -1; dj <= 1 && !hit; ++dj)
+                for (int di = -1; di <= 1; ++di) {
+                    if (!di && !dj) continue;
+                    if (conn == 4 && di && dj) continue;
+                    const int qi = i + di, qj = j + dj;
+                    if (qi < 0 || qi >= nr || qj < 0 || qj >= nc) continue;
+                    if (out[qi + qj * nr]) {
+                        hit = true;
+                        break;
+                    }
+                }
+            if (hit) {
+                out[p] = 1;
+                changed = 1;
+            }
+        }
+        if (!__syncthreads_or(changed)) break;
+    }
+}
+
+template <int BS, int MODE>
+__global__ __launch_bounds__(BS) void icgn_kernel(IcgnArgs a)
+{
+    constexpr int NW = BS / kWave;
+    constexpr int NJ = MODE;  // steepest-descent components
+    extern __shared__ __align__(16) unsigned char smem[];
+    __shared__ Ctl c;
+    __shared__ double redd[NW * 21];
+    __shared__ float redf[NW * 12];
+    __shared__ long item_sh;
+
+    float *gbuf = reinterpret_cast<float *>(smem);
+    float *tile = gbuf + a.npix_cap;
+    unsigned char *tm = reinterpret_cast<unsigned char *>(tile + a.tile_cap);
+    unsigned char *sel0 = tm + a.npix_cap;
+    unsigned char *sel = sel0 + a.npix_cap;
+    unsigned char *bwb = sel + a.npix_cap;
+    const int tid = threadIdx.x;
+
+    for (long pass = 0;; ++pass) {
+        // ---------------------------------------------------------- pick work
+        if (tid == 0) {
+            long it;
+            if (a.work_counter)
+                it = a.item0 + atomicAdd(a.work_counter, 1);
+            else
+                it = pass == 0 ? a.item0 + blockIdx.x : a.n_items + a.item0;
+            item_sh = it;
+        }
+        __syncthreads();
+        const long item = item_sh;
+        if (item >= a.item0 + a.n_items) break;
+        if (a.defer_flag && *a.defer_flag == 0) break;
+
+        // ------------------------------------------------------------ setup
+        if (tid == 0) {
+            const long idx = a.order ? a.order[item] : item;
+            c.idx = idx;
+            c.slot = a.slot ? a.slot[idx] : 0;
+            double x0 = a.coords[idx], y0 = a.coords[idx + a.N];
+            if (!a.plain) {
+                x0 = round(x0);  // LocalICGNQuadtree.m:69
+                y0 = round(y0);
+            }
+            c.x0 = x0;
+            c.y0 = y0;
+            const int wx = a.wx[idx];
+            const int wy = MODE == 2 ? a.wy[idx] : wx;
+            c.wx = wx;
+            c.wy = wy;
+            c.w0 = wx;
+            if (MODE == 6) {
+                c.P[0] = c.P[1] = c.P[2] = c.P[3] = 0.0;
+                c.P[4] = a.U0[2 * idx];
+                c.P[5] = a.U0[2 * idx + 1];
+            } else {
+                for (int k = 0; k < 4; ++k) c.P[k] = a.FOld[4 * idx + k];
+                c.P[4] = a.UOld[2 * idx];
+                c.P[5] = a.UOld[2 * idx + 1];
+            }
+            c.state = 0;
+            c.status = 0;
+            c.it = 0;
+            c.normNew = 1;
+            c.normAbs = 1;
+            c.normInit = 0;
+            c.sum_bw2 = 0;
+            c.sel0_all = 1;
+            c.fsel = 0;
+            const FrameDev fr = a.frames[c.slot];
+            if (a.only_deferred && a.status[idx] != ST_DEFER) {
+                c.state = 4;  // done by the main launch
+            } else if (!win_ok(fr, x0, y0, wx, wy)) {
+                c.status = ST_ERROR;
+                c.state = 2;
+            } else if ((wx + 1) * (wy + 1) > a.npix_cap) {
+                c.status = a.defer ? ST_DEFER : ST_ERROR;
+                c.state = a.defer ? 3 : 2;
+            } else {
+                c.nx = wx + 1;
+                c.ny = wy + 1;
+                c.npix = c.nx * c.ny;
+                c.xa = (int)(x0 - wx / 2);
+                c.ya = (int)(y0 - wy / 2);
+            }
+        }
+        __syncthreads();
+        const FrameDev fr = a.frames[c.slot];
+        const int Hd = fr.H - 6;
+
+        if (c.state == 0) {
+            // masked-pixel count: zeros of f*mask (funICGNQuadtree.m:61,
+            // funICGN_Subpb1.m:68) or of DfDx (funICGN.m:54)
+            {
+                const int nx = c.nx, npix = c.npix, xa = c.xa, ya = c.ya;
+                const float rcp = 1.0f / nx;
+                float cnt = 0.f;
+                for (int p = tid; p < npix; p += BS) {
+                    int i, j;
+                    pix_ij(p, nx, rcp, i, j);
+                    const int x = xa + i, y = ya + j;
+                    if (MODE == 6 && a.plain)
+                        cnt += fr.dfdx[(x - 4) + (size_t)(y - 4) * Hd] == 0.f;
+                    else
+                        cnt += fidx(fr.f, fr.H, x, y) * fidx(fr.mask, fr.H, x, y) == 0.f;
+                }
+                float v[1] = {cnt}, o[1];
+                block_sum<BS, 1>(v, redf, o);
+                if (tid == 0) {
+                    const int count = (int)o[0];
+                    c.count = count;
+                    const double n0 = (double)c.nx * c.ny;
+                    const double cap = (MODE == 6 && a.plain) ? 0.5 * n0 : 0.4 * n0;
+                    if (!(count < cap)) {
+                        c.status = ST_MASKED;
+                        c.state = 2;
+                    } else {
+                        int wx = c.wx, wy = c.wy;
+                        bool grow;
+                        if (MODE == 6 && a.plain) {
+                            grow = count > 0.1 * n0;
+                            if (grow) wx = wy = (int)round(sqrt(2.0) * wx);
+                        } else {
+                            grow = count > 0;
+                            if (grow) {
+                                wx = 2 * (int)ceil(sqrt(n0 / (n0 - count)) * wx / 2.0);
+                                if (MODE == 6) {
+                                    wy = wx;
+                                } else {
+                                    const double n1 = (double)(wx + 1) * (wy + 1);
+                                    wy = 2 * (int)ceil(sqrt(n1 / (n1 - count)) * wy / 2.0);
+                                }
+                            }
+                        }
+                        if (grow) {
+                            if (!win_ok(fr, c.x0, c.y0, wx, wy)) {
+                                c.status = ST_ERROR;
+                                c.state = 2;
+                            } else if ((wx + 1) * (wy + 1) > a.npix_cap) {
+                                c.status = a.defer ? ST_DEFER : ST_ERROR;
+                                c.state = a.defer ? 3 : 2;
+                            } else {
+                                c.wx = wx;
+                                c.wy = wy;
+                                c.nx = wx + 1;
+                                c.ny = wy + 1;
+                                c.npix = c.nx * c.ny;
+                                c.xa = (int)(c.x0 - wx / 2);
+                                c.ya = (int)(c.y0 - wy / 2);
+                            }
+                        }
+                        c.sel0_all = (a.plain && MODE == 6) || count == 0;
+                    }
+                }
+                __syncthreads();
+            }
+        }
+
+        const int nx = c.nx, ny = c.ny, npix = c.npix, xa = c.xa, ya = c.ya;
+        const float rcp = 1.0f / nx;
+        const float hx = c.wx * 0.5f, hy = c.wy * 0.5f;
+        // bwselect seed: (floor((wx+1)/2), floor((wy+1)/2)) passed as (c, r):
+        // the column index is floor((wx+1)/2) and the row floor((wy+1)/2).
+        const int seed_c = (c.wx + 1) / 2, seed_r = (c.wy + 1) / 2;
+
+        if (c.state == 0 && !c.sel0_all) {
+            // :83-88 — 4-connected region of the window mask through the seed
+            for (int p = tid; p < npix; p += BS) {
+                int i, j;
+                pix_ij(p, nx, rcp, i, j);
+                tm[p] = fidx(fr.mask, fr.H, xa + i, ya + j) != 0.f;
+            }
+            __syncthreads();
+            flood_fill<BS>(tm, sel0, nx, ny, seed_r, seed_c, 4, npix, rcp);
+        }
+
+        // f-side sums in FP64 (:91-103, :119-120): statistics and the constant
+        // parts of b first, the 21 Hessian entries in a second pass
+        if (c.state == 0) {
+            constexpr int K = (MODE == 6) ? 16 : 11;
+            double v[K];
+#pragma unroll
+            for (int k = 0; k < K; ++k) v[k] = 0.0;
+            const bool all = c.sel0_all;
+            for (int p = tid; p < npix; p += BS) {
+                int i, j;
+                pix_ij(p, nx, rcp, i, j);
+                const int x = xa + i, y = ya + j;
+                const bool s = all || sel0[p];
+                const size_t gi = (x - 4) + (size_t)(y - 4) * Hd;
+                const double fx = s ? (double)fr.dfdx[gi] : 0.0;
+                const double fy = s ? (double)fr.dfdy[gi] : 0.0;
+                double fv = fidx(fr.f, fr.H, x, y);
+                if (!(MODE == 6 && a.plain)) fv *= (double)fidx(fr.mask, fr.H, x, y);
+                if (!s) fv = 0.0;
+                const bool nz = (MODE == 6 && a.plain) || fabs(fv) > 1e-10;
+                v[0] += nz ? 1.0 : 0.0;
+                v[1] += nz ? fv : 0.0;
+                v[2] += nz ? fv * fv : 0.0;
+                v[3] += fabs(fv);
+                if (MODE == 6) {
+                    const double dx = i - (double)hx, dy = j - (double)hy;
+                    const double J[6] = {fx * dx, fy * dx, fx * dy, fy * dy, fx, fy};
+#pragma unroll
+                    for (int k = 0; k < 6; ++k) {
+                        v[4 + k] += J[k];
+                        v[10 + k] += J[k] * fv;
+                    }
+                } else {
+                    v[4] += fx * fx;
+                    v[5] += fx * fy;
+                    v[6] += fy * fy;
+                    v[7] += fx;
+                    v[8] += fy;
+                    v[9] += fx * fv;
+                    v[10] += fy * fv;
+                }
+            }
+            double o[K];
+            block_sum<BS, K>(v, redd, o);
+            if (tid == 0) {
+                const double n = o[0], sf = o[1], sf2 = o[2];
+                const double mf = n > 0 ? sf / n : NAN;
+                const double bf = n >= 2 ? sqrt(fmax(sf2 - sf * sf / n, 0.0)) : (n == 1 ? 0.0 : NAN);
+                c.meanf = mf;
+                c.bottomf = bf;
+                c.sabs = o[3];
+                c.cf = isfinite(mf) ? (float)mf : 0.f;
+                if (MODE == 6) {
+                    for (int k = 0; k < 6; ++k) {
+                        c.SJ[k] = o[4 + k];
+                        c.FJ[k] = o[10 + k] - mf * o[4 + k];
+                    }
+                } else {
+                    const double s2 = 2.0 / (bf * bf);
+                    const double H2a = o[4] * s2 + a.mu, H2b = o[5] * s2, H2d = o[6] * s2 + a.mu;
+                    const double dl = 1e-3 * (H2a > H2d ? H2a : H2d);
+                    c.tH[0] = H2a + dl;
+                    c.tH[1] = c.tH[2] = H2b;
+                    c.tH[3] = H2d + dl;
+                    c.SJ[0] = o[7];
+                    c.SJ[1] = o[8];
+                    c.FJ[0] = o[9] - mf * o[7];
+                    c.FJ[1] = o[10] - mf * o[8];
+                }
+            }
+            __syncthreads();
+        }
+        if (MODE == 6 && c.state == 0) {
+            double v[21];
+#pragma unroll
+            for (int k = 0; k < 21; ++k) v[k] = 0.0;
+            const bool all = c.sel0_all;
+            for (int p = tid; p < npix; p += BS) {
+                int i, j;
+                pix_ij(p, nx, rcp, i, j);
+                const int x = xa + i, y = ya + j;
+                const bool s = all || sel0[p];
+                const size_t gi = (x - 4) + (size_t)(y - 4) * Hd;
+                const double fx = s ? (double)fr.dfdx[gi] : 0.0;
+                const double fy = s ? (double)fr.dfdy[gi] : 0.0;
+                const double dx = i - (double)hx, dy = j - (double)hy;
+                const double J[6] = {fx * dx, fy * dx, fx * dy, fy * dy, fx, fy};
+                int t = 0;
+#pragma unroll
+                for (int cc = 0; cc < 6; ++cc)
+#pragma unroll
+                    for (int r = cc; r < 6; ++r) v[t++] += J[r] * J[cc];
+            }
+            double o[21];
+            block_sum<BS, 21>(v, redd, o);
+            if (tid == 0) {
+                int t = 0;
+                for (int cc = 0; cc < 6; ++cc)
+                    for (int r = cc; r < 6; ++r) {
+                        c.H21[t] = o[t];
+                        c.Hm[r + cc * 6] = c.Hm[cc + r * 6] = o[t];
+                        ++t;
+                    }
+                factor_lds<6>(c.Hm, c.Lw, c.piv, &c.fkind);
+            }
+            __syncthreads();
+        }
+
+        // ---------------------------------------------------------- iterate
+        while (c.state == 0) {
+            if (tid == 0) {
+                const double tol = a.tol;
+                if (!(c.it <= a.max_it && c.normNew > tol && c.normAbs > tol)) {
+                    c.state = 1;
+                } else {
+                    c.it += 1;
+                    double *P = c.P;
+                    if (MODE == 6 && !a.plain && c.it > 1 && c.count > 0) {
+                        // funICGNQuadtree.m:139-145: the window is re-sized
+                        // from the last g region without reloading f; a new
+                        // size throws at :363 (status -1) unless the new grid
+                        // leaves the image first (:154-157, status 101).
+                        const double wn =
+                            2.0 * ceil(sqrt((double)(c.w0 + 1) * (c.w0 + 1) / (double)c.sum_bw2) *
+                                       c.wx / 2.0);
+                        if (wn != (double)c.wx) {
+                            if (!isfinite(wn) || c.x0 - wn / 2 < 1 || c.x0 + wn / 2 > fr.H ||
+                                c.y0 - wn / 2 < 1 || c.y0 + wn / 2 > fr.W) {
+                                c.status = ST_ERROR;
+                                c.state = 2;
+                            } else {
+                                const double h = wn / 2;
+                                bool out = false;
+                                for (int q = 0; q < 4; ++q) {
+                                    const double dx = (q & 1) ? h : -h, dy = (q & 2) ? h : -h;
+                                    const double u = (1 + P[0]) * dx + P[2] * dy + (c.x0 + P[4]);
+                                    const double w = P[1] * dx + (1 + P[3]) * dy + (c.y0 + P[5]);
+                                    if (u < 3 || u > fr.H - 2 || w < 3 || w > fr.W - 2) out = true;
+                                }
+                                if (out) {
+                                    c.normNew = 1e6;
+                                    c.state = 1;
+                                } else {
+                                    c.status = ST_ERROR;
+                                    c.state = 2;
+                                }
+                            }
+                        }
+                    }
+                    if (c.state == 0) {
+                        bool nanP = false;
+                        for (int k = 0; k < 6; ++k) nanP |= isnan(P[k]);
+                        if (nanP) {
+                            // NaN coordinates pass the bounds test and give NaN
+                            // g: logical(NaN) throws (quadtree / Subpb1), the
+                            // plain solver's norm turns NaN.
+                            if (MODE == 2 || !a.plain) {
+                                c.status = ST_ERROR;
+                                c.state = 2;
+                            } else if (c.it == 1) {
+                                c.normNew = 0;  // NaN initial norm -> relative 0
+                                c.normAbs = NAN;
+                                c.state = 1;
+                            } else {
+                                c.normNew = NAN;
+                                c.normAbs = NAN;
+                                c.state = 1;
+                            }
+                        }
+                    }
+                    if (c.state == 0) {
+                        double umin = INFINITY, umax = -INFINITY, vmin = INFINITY, vmax = -INFINITY;
+                        bool out = false;
+                        for (int q = 0; q < 4; ++q) {
+                            const double dx = (q & 1) ? hx : -hx, dy = (q & 2) ? hy : -hy;
+                            const double u = (1 + P[0]) * dx + P[2] * dy + (c.x0 + P[4]);
+                            const double w = P[1] * dx + (1 + P[3]) * dy + (c.y0 + P[5]);
+                            if (u < 3 || u > fr.H - 2 || w < 3 || w > fr.W - 2) out = true;
+                            umin = fmin(umin, u);
+                            umax = fmax(umax, u);
+                            vmin = fmin(vmin, w);
+                            vmax = fmax(vmax, w);
+                        }
+                        if (out) {
+                            c.normNew = 1e6;
+                            c.state = 1;
+                        } else {
+                            const int tu0 = (int)floor(umin) - 2, tv0 = (int)floor(vmin) - 2;
+                            c.tile_u0 = tu0;
+                            c.tile_v0 = tv0;
+                            c.tnx = (int)floor(umax) - (int)floor(umin) + 6;
+                            c.tny = (int)floor(vmax) - (int)floor(vmin) + 6;
+                            c.use_tile = c.tnx * c.tny <= a.tile_cap;
+                            c.a11 = (float)(1 + P[0]);
+                            c.a12 = (float)P[2];
+                            c.a21 = (float)P[1];
+                            c.a22 = (float)(1 + P[3]);
+                            c.cu = (float)((c.x0 + P[4]) - tu0);
+                            c.cv = (float)((c.y0 + P[5]) - tv0);
+                        }
+                    }
+                }
+            }
+            __syncthreads();
+            if (c.state != 0) break;
+
+            const int tu0 = c.tile_u0, tv0 = c.tile_v0, tnx = c.tnx, tny = c.tny;
+            const bool use_tile = c.use_tile;
+            if (use_tile) {
+                const int lane = tid & (kWave - 1), wid = tid / kWave;
+                for (int s = wid; s < tny; s += NW) {
+                    const int y = min(max(tv0 + s, 1), fr.W);
+                    const float *col = fr.g + (size_t)(y - 1) * fr.H;
+                    for (int r = lane; r < tnx; r += kWave) {
+                        const int x = min(max(tu0 + r, 1), fr.H);
+                        tile[r + s * tnx] = col[x - 1];
+                    }
+                }
+            }
+            __syncthreads();
+
+            // gather + fused sums
+            {
+                const float a11 = c.a11, a12 = c.a12, a21 = c.a21, a22 = c.a22;
+                const float cu = c.cu, cv = c.cv, cf = c.cf;
+                const bool all = c.sel0_all, wholestats = (MODE == 6 && a.plain);
+                float acc[6 + NJ];
+#pragma unroll
+                for (int k = 0; k < 6 + NJ; ++k) acc[k] = 0.f;
+                for (int p = tid; p < npix; p += BS) {
+                    int i, j;
+                    pix_ij(p, nx, rcp, i, j);
+                    const float dx = (float)i - hx, dy = (float)j - hy;
+                    const float lu = fmaf(a11, dx, fmaf(a12, dy, cu));
+                    const float lv = fmaf(a21, dx, fmaf(a22, dy, cv));
+                    const float fu = floorf(lu), fv = floorf(lv);
+                    float wu[4], wv[4];
+                    keys_w(lu - fu, wu);
+                    keys_w(lv - fv, wv);
+                    const int iu = (int)fu - 1, iv = (int)fv - 1;
+                    float g = 0.f;
+                    if (use_tile) {
+                        const float *b = tile + iu + iv * tnx;
+#pragma unroll
+                        for (int q = 0; q < 4; ++q) {
+                            const float *cq = b + q * tnx;
+                            g = fmaf(wv[q], fmaf(wu[0], cq[0], fmaf(wu[1], cq[1], fmaf(wu[2], cq[2], wu[3] * cq[3]))), g);
+                        }
+                    } else {
+#pragma unroll
+                        for (int q = 0; q < 4; ++q) {
+                            const int y = min(max(tv0 + iv + q, 1), fr.W);
+                            const float *col = fr.g + (size_t)(y - 1) * fr.H;
+                            float s = 0.f;
+#pragma unroll
+                            for (int r = 0; r < 4; ++r) s = fmaf(wu[r], col[min(max(tu0 + iu + r, 1), fr.H) - 1], s);
+                            g = fmaf(wv[q], s, g);
+                        }
+                    }
+                    gbuf[p] = g;
+                    const float d = g - cf;
+                    const bool nz = wholestats || fabsf(g) > 1e-10f;
+                    acc[0] += nz ? 1.f : 0.f;
+                    acc[1] += nz ? d : 0.f;
+                    acc[2] += nz ? d * d : 0.f;
+                    acc[3] += g == 0.f ? 1.f : 0.f;
+                    acc[4] += isnan(g) ? 1.f : 0.f;
+                    const bool s = all || sel0[p];
+                    const size_t gi = (xa + i - 4) + (size_t)(ya + j - 4) * Hd;
+                    const float fxv = s ? fr.dfdx[gi] : 0.f, fyv = s ? fr.dfdy[gi] : 0.f;
+                    const float u = fxv * d, v = fyv * d;
+                    if (MODE == 6) {
+                        acc[6] = fmaf(dx, u, acc[6]);
+                        acc[7] = fmaf(dx, v, acc[7]);
+                        acc[8] = fmaf(dy, u, acc[8]);
+                        acc[9] = fmaf(dy, v, acc[9]);
+                        acc[10] += u;
+                        acc[11] += v;
+                    } else {
+                        acc[6] += u;
+                        acc[7] += v;
+                    }
+                }
+                float o[6 + NJ];
+                block_sum<BS, 6 + NJ>(acc, redf, o);
+                if (tid == 0) {
+                    c.path = 0;
+                    const bool quad = !(MODE == 6 && a.plain);
+                    if (quad && o[4] > 0) {
+                        c.status = ST_ERROR;  // logical(NaN) throws
+                        c.state = 2;
+                    } else if (quad && (c.count > 0 || o[3] > 0)) {
+                        c.path = 1;
+                    } else if (MODE == 2 && c.fsel) {
+                        c.path = 2;
+                    } else {
+                        const double n = o[0], S1 = o[1], S2 = o[2];
+                        const double cfd = (double)cf;
+                        const double gbar = cfd + S1 / n;
+                        const double bg = n >= 2 ? sqrt(fmax(S2 - S1 * S1 / n, 0.0)) : (n == 1 ? 0.0 : NAN);
+                        const double bf = c.bottomf;
+                        if (!(bf > 0) || !(bg > 0) || !isfinite(bf) || !isfinite(bg) || !isfinite(gbar)) {
+                            c.path = 2;  // literal per-pixel residual for degenerate statistics
+                        } else {
+                            double b[6];
+                            for (int k = 0; k < NJ; ++k) {
+                                const double G = (double)o[6 + k] - (gbar - cfd) * c.SJ[k];
+                                b[k] = c.FJ[k] - (bf / bg) * G;
+                            }
+                            for (int k = 0; k < NJ; ++k) c.bvec[k] = b[k];
+                        }
+                    }
+                }
+                __syncthreads();
+            }
+
+            if (c.state == 0 && c.path != 0) {
+                // ------------------------------------- exact (masked) path
+                const bool masked = c.path == 1;
+                const bool fsel = MODE == 2 && (masked || c.fsel);
+                if (masked) {
+                    for (int p = tid; p < npix; p += BS) bwb[p] = gbuf[p] != 0.f;
+                    __syncthreads();
+                    // 6-DOF: 8-connected (:171); 2-DOF: 4-connected (:160)
+                    flood_fill<BS>(bwb, sel, nx, ny, seed_r, seed_c, MODE == 6 ? 8 : 4, npix, rcp);
+                    int allsel = 1;
+                    for (int p = tid; p < npix; p += BS) allsel &= sel[p];
+                    allsel = __syncthreads_and(allsel);
+                    if (allsel) {
+                        // the region is the whole window: use the window mask
+                        // instead, 8-connected (:175-176, Subpb1 :164-165)
+                        for (int p = tid; p < npix; p += BS) {
+                            int i, j;
+                            pix_ij(p, nx, rcp, i, j);
+                            tm[p] = fidx(fr.mask, fr.H, xa + i, ya + j) != 0.f;
+                        }
+                        __syncthreads();
+                        flood_fill<BS>(tm, sel, nx, ny, seed_r, seed_c, 8, npix, rcp);
+                    }
+                    for (int p = tid; p < npix; p += BS)
+                        if (!sel[p]) gbuf[p] = 0.f;
+                    __syncthreads();
+                }
+                // pass A: counts and sums (g; 2-DOF masked also f and H)
+                {
+                    double v[9];
+#pragma unroll
+                    for (int k = 0; k < 9; ++k) v[k] = 0.0;
+                    const bool whole = MODE == 6 && a.plain;
+                    for (int p = tid; p < npix; p += BS) {
+                        const double g = gbuf[p];
+                        const bool nz = whole || fabs(g) > 1e-10;
+                        v[0] += nz ? 1.0 : 0.0;
+                        v[1] += nz ? g : 0.0;
+                        v[2] += masked ? (double)sel[p] : 0.0;
+                        if (fsel) {
+                            int i, j;
+                            pix_ij(p, nx, rcp, i, j);
+                            const int x = xa + i, y = ya + j;
+                            const bool s = sel[p];
+                            const size_t gi = (x - 4) + (size_t)(y - 4) * Hd;
+                            const double fx = s ? (double)fr.dfdx[gi] : 0.0;
+                            const double fy = s ? (double)fr.dfdy[gi] : 0.0;
+                            const double fv = s ? (double)fidx(fr.f, fr.H, x, y) * (double)fidx(fr.mask, fr.H, x, y) : 0.0;
+                            const bool fnz = fabs(fv) > 1e-10;
+                            v[3] += fnz ? 1.0 : 0.0;
+                            v[4] += fnz ? fv : 0.0;
+                            v[5] += fx * fx;
+                            v[6] += fx * fy;
+                            v[7] += fy * fy;
+                            v[8] += fabs(fv);
+                        }
+                    }
+                    double o[9];
+                    block_sum<BS, 9>(v, redd, o);
+                    if (tid == 0) {
+                        c.gmean = o[0] > 0 ? o[1] / o[0] : NAN;
+                        c.ng = o[0];
+                        c.nf = o[3];
+                        if (masked) c.sum_bw2 = (int)o[2];
+                        if (fsel) {
+                            c.meanf = o[3] > 0 ? o[4] / o[3] : NAN;
+                            c.sabs = o[8];
+                            c.Hm[0] = o[5];
+                            c.Hm[1] = o[6];
+                            c.Hm[2] = o[7];
+                        }
+                    }
+                    __syncthreads();
+                }
+                // pass A2: second moments about the means
+                {
+                    double v[2] = {0.0, 0.0};
+                    const bool whole = MODE == 6 && a.plain;
+                    const double mg = c.gmean, mf = c.meanf;
+                    for (int p = tid; p < npix; p += BS) {
+                        const double g = gbuf[p];
+                        if (whole || fabs(g) > 1e-10) v[0] += (g - mg) * (g - mg);
+                        if (fsel && sel[p]) {
+                            int i, j;
+                            pix_ij(p, nx, rcp, i, j);
+                            const double fv = (double)fidx(fr.f, fr.H, xa + i, ya + j) *
+                                              (double)fidx(fr.mask, fr.H, xa + i, ya + j);
+                            if (fabs(fv) > 1e-10) v[1] += (fv - mf) * (fv - mf);
+                        }
+                    }
+                    double o[2];
+                    block_sum<BS, 2>(v, redd, o);
+                    if (tid == 0) {
+                        const double ng = c.ng;
+                        c.bottomg = ng >= 2 ? sqrt(o[0]) : (ng == 1 ? 0.0 : NAN);
+                        if (masked && MODE == 2) c.fsel = 1;
+                        if (fsel) {
+                            const double nf = c.nf;
+                            const double bf = nf >= 2 ? sqrt(o[1]) : (nf == 1 ? 0.0 : NAN);
+                            c.bottomf = bf;
+                            const double s2 = 2.0 / (bf * bf);
+                            const double H2a = c.Hm[0] * s2 + a.mu, H2b = c.Hm[1] * s2, H2d = c.Hm[2] * s2 + a.mu;
+                            const double dl = 1e-3 * (H2a > H2d ? H2a : H2d);
+                            c.tH[0] = H2a + dl;
+                            c.tH[1] = c.tH[2] = H2b;
+                            c.tH[3] = H2d + dl;
+                        }
+                    }
+                    __syncthreads();
+                }
+                // pass B: b = bottomf * sum(J r), r evaluated on every pixel (:362-371)
+                {
+                    double v[NJ];
+#pragma unroll
+                    for (int k = 0; k < NJ; ++k) v[k] = 0.0;
+                    const double mf = c.meanf, bf = c.bottomf, mg = c.gmean, bg = c.bottomg;
+                    const bool all = c.sel0_all;
+                    for (int p = tid; p < npix; p += BS) {
+                        int i, j;
+                        pix_ij(p, nx, rcp, i, j);
+                        const int x = xa + i, y = ya + j;
+                        bool s;
+                        double fv;
+                        if (fsel) {
+                            s = sel[p];
+                            fv = s ? (double)fidx(fr.f, fr.H, x, y) * (double)fidx(fr.mask, fr.H, x, y) : 0.0;
+                        } else {
+                            s = all || sel0[p];
+                            fv = fidx(fr.f, fr.H, x, y);
+                            if (!(MODE == 6 && a.plain)) fv *= (double)fidx(fr.mask, fr.H, x, y);
+                            if (!s) fv = 0.0;
+                        }
+                        const size_t gi = (x - 4) + (size_t)(y - 4) * Hd;
+                        const double fx = s ? (double)fr.dfdx[gi] : 0.0;
+                        const double fy = s ? (double)fr.dfdy[gi] : 0.0;
+                        const double r = (fv - mf) / bf - ((double)gbuf[p] - mg) / bg;
+                        if (MODE == 6) {
+                            const double dx = i - (double)hx, dy = j - (double)hy;
+                            v[0] += dx * fx * r;
+                            v[1] += dx * fy * r;
+                            v[2] += dy * fx * r;
+                            v[3] += dy * fy * r;
+                            v[4] += fx * r;
+                            v[5] += fy * r;
+                        } else {
+                            v[0] += fx * r;
+                            v[1] += fy * r;
+                        }
+                    }
+                    double o[NJ];
+                    block_sum<BS, NJ>(v, redd, o);
+                    if (tid == 0)
+                        for (int k = 0; k < NJ; ++k) c.bvec[k] = c.bottomf * o[k];
+                    __syncthreads();
+                }
+            }
+
+            // ------------------------------- norm test, GN solve, IC update
+            if (tid == 0 && c.state == 0) {
+                const double tol = a.tol;
+                double b[6];
+                for (int k = 0; k < NJ; ++k) b[k] = c.bvec[k];
+                double d[6] = {0, 0, 0, 0, 0, 0};
+                double nb2 = 0.0;
+                if (MODE == 6) {
+                    for (int k = 0; k < 6; ++k) nb2 += b[k] * b[k];
+                } else {
+                    // :379 tempb = 2 b / bottomf^2 + mu (P - UOld - vdual)
+                    const double bf = c.bottomf, s2 = 2.0 / (bf * bf);
+                    const long idx = c.idx;
+                    b[0] = b[0] * s2 + a.mu * (c.P[4] - a.UOld[2 * idx] - a.vdual[2 * idx]);
+                    b[1] = b[1] * s2 + a.mu * (c.P[5] - a.UOld[2 * idx + 1] - a.vdual[2 * idx + 1]);
+                    nb2 = b[0] * b[0] + b[1] * b[1];
+                }
+                double nn = sqrt(nb2);
+                c.normAbs = nn;
+                if (c.it == 1) c.normInit = nn;
+                nn = c.normInit > tol ? nn / c.normInit : 0.0;
+                c.normNew = nn;
+                if (nn < tol || c.normAbs < tol) {
+                    c.state = 1;
+                } else {
+                    if (MODE == 6) {
+                        const double nbv[6] = {-b[0], -b[1], -b[2], -b[3], -b[4], -b[5]};
+                        solve_factored<6>(c.Hm, c.piv, c.fkind, nbv, d);
+                    } else {
+                        const double nbv[2] = {-b[0], -b[1]};
+                        double d2[2];
+                        solve_small<2>(c.tH, nbv, d2);
+                        d[4] = d2[0];
+                        d[5] = d2[1];
+                    }
+                    if (!ic_compose(c.P, d)) c.state = 1;
+                }
+            }
+            __syncthreads();
+        }
+
+        // ---------------------------------------------------------- outputs
+        if (tid == 0 && c.state == 3) a.status[c.idx] = ST_DEFER;
+        if (tid == 0 && c.state <= 2) {
+            int st = c.status;
+            if (c.state == 1) {
+                const double tol = a.tol;
+                int it = c.it;
+                if (!(c.normNew < tol || c.normAbs < tol)) it = a.max_it + 1;
+                if (isnan(c.normNew)) it = a.max_it + 1;
+                if (!(MODE == 6 && a.plain) && c.sabs < 1e-6) it = a.max_it + 3;
+                st = it;
+            }
+            const long idx = c.idx;
+            double P[6];
+            for (int k = 0; k < 6; ++k) P[k] = c.P[k];
+            if (st == ST_ERROR)
+                for (int k = 0; k < 6; ++k) P[k] = NAN;
+            if (st == ST_MASKED && MODE == 6) P[0] = P[1] = P[2] = P[3] = 0.0;
+            if (st == ST_MASKED && MODE == 6) {
+                P[4] = a.U0[2 * idx];
+                P[5] = a.U0[2 * idx + 1];
+            }
+            a.U[2 * idx] = P[4];
+            a.U[2 * idx + 1] = P[5];
+            if (MODE == 6) {
+                for (int k = 0; k < 4; ++k) a.F[4 * idx + k] = P[k];
+                const bool zeroH = st == ST_ERROR || st == ST_MASKED;
+                for (int k = 0; k < 21; ++k) a.H21[idx + k * a.N] = zeroH ? 0.0 : c.H21[k];
+            }
+            a.status[idx] = st;
+        }
+        __syncthreads();
+        if (!a.work_counter) break;
+    }
+}
+
+// Host-side launch helpers (instantiated for the block sizes the autotuner may pick).
+size_t icgn_lds_bytes(int npix_cap, int tile_cap)
+{
+    size_t b = sizeof(float) * (size_t)(npix_cap + tile_cap) + 4u * (size_t)npix_cap;
+    return (b + 15) & ~(size_t)15;
+}
+
+template <int BS, int MODE>
+static hipError_t launch_bs(const IcgnArgs &a, int grid, hipStream_t s)
+{
+    const size_t lds = icgn_lds_bytes(a.npix_cap, a.tile_cap);
+    static bool attr_set = false;
+    if (!attr_set) {
+        hipFuncSetAttribute((const void *)icgn_kernel<BS, MODE>,
+                            hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024 - 8192);
+        attr_set = true;
+    }
+    hipLaunchKernelGGL((icgn_kernel<BS, MODE>), dim3(grid), dim3(BS), lds, s, a);
+    return hipGetLastError();
+}
+
+hipError_t icgn_launch(const IcgnArgs &a, int mode, int bs, int grid, hipStream_t s)
+{
+    if (mode == 6) {
+        switch (bs) {
+        case 64: return launch_bs<64, 6>(a, grid, s);
+        case 128: return launch_bs<128, 6>(a, grid, s);
+        case 256: return launch_bs<256, 6>(a, grid, s);
+        case 512: return launch_bs<512, 6>(a, grid, s);
+        }
+    } else {
+        switch (bs) {
+        case 64: return launch_bs<64, 2>(a, grid, s);
+        case 128: return launch_bs<128, 2>(a, grid, s);
+        case 256: return launch_bs<256, 2>(a, grid, s);
+        case 512: return launch_bs<512, 2>(a, grid, s);
+        }
+    }
+    return hipErrorInvalidValue;
+}
+
+}  // namespace staq

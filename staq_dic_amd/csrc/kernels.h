@@ -1,0 +1,67 @@
+// Kernel argument blocks and launchers shared by the kernels and the C ABI.
+#pragma once
+
+#include "common.h"
+
+namespace staq {
+
+struct IcgnArgs {
+    const FrameDev *frames;
+    const int *slot;        // frame index per subset (nullptr -> 0)
+    const double *coords;   // N x 2 column-major (coordinatesFEM)
+    const int *wx, *wy;     // window size per subset (wy: 2-DOF only)
+    const int *order;       // processing order (nullptr -> identity)
+    const double *U0;       // 6-DOF: initial displacement, 2N interleaved
+    const double *UOld;     // 2-DOF: U of the global step, 2N
+    const double *FOld;     // 2-DOF: F of the global step, 4N
+    const double *vdual;    // 2-DOF: displacement dual, 2N
+    double *U, *F, *H21;    // outputs (F, H21: 6-DOF)
+    int *status;
+    int *work_counter;      // persistent mode: device queue head (nullptr -> 1 subset/block)
+    const int *defer_flag;  // fallback launch: exit at once when no subset was deferred
+    long N;                 // subset count (array extents)
+    long n_items;           // items of this launch
+    long item0;             // first item (one launch per subset uses this)
+    double tol, mu;
+    int plain;              // uniform-grid semantics (no node rounding, whole-window stats)
+    int max_it;             // 100 in the reference
+    int npix_cap, tile_cap; // dynamic LDS sizing (pixels, floats)
+    int defer;              // a window larger than npix_cap is left for a fallback launch
+    int only_deferred;      // fallback launch: process only subsets left by the main launch
+};
+
+struct ZnssdArgs {
+    const FrameDev *frames;
+    const int *slot;
+    const double *coords;  // N x 2 column-major
+    const int *winsize;
+    const double *P;       // 6N: [u_x v_x u_y v_y u v] per subset
+    const double *gcost;   // upstream dL/dC_i (backward), may be null (= 1)
+    double *cost;          // N (forward), may be null
+    double *gradP;         // 6N (backward), may be null
+    int *status;           // N: 0 ok, 1 window or warp outside the image
+    long N;
+    int npix_cap;
+};
+
+struct SearchArgs {
+    const float *f, *g;
+    int H, W;
+    int r1, r2, w;
+    int gx0, gy0, step, nx, ny;  // grid: ii = gx0 + ix*step, jj = gy0 + iy*step
+    double2 *scratch;            // (num, den) per kept shift, per point of this launch
+    long point0;                 // first point of this launch
+    double *x0, *y0, *u, *v, *ccmax;  // ny x nx column-major outputs
+};
+
+hipError_t icgn_launch(const IcgnArgs &a, int mode, int bs, int grid, hipStream_t s);
+size_t icgn_lds_bytes(int npix_cap, int tile_cap);
+hipError_t ba_interp2_launch(const void *F, long M, long N, long P, const void *X, const void *Y,
+                             long nd, void *out, bool single, hipStream_t s);
+hipError_t img_gradient_launch(const float *f, const float *mask, int H, int W, float *dfdx,
+                               float *dfdy, hipStream_t s);
+hipError_t znssd_launch(const ZnssdArgs &a, hipStream_t s);
+hipError_t search_launch(const SearchArgs &a, long npoints, hipStream_t s);
+size_t search_lds_bytes(int w, int r1, int r2);
+
+}  // namespace staq

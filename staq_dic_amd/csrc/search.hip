@@ -1,0 +1,233 @@
+// Whole-field integer search (funIntegerSearch.m:82-203) with findpeak
+// (findpeak.m:1-68), one workgroup per search point, for gfx950.
+//
+// For a grid point the template C = f(ii:ii+w, jj:jj+w) and the search block
+// D = g(ii-r1 : ii+w+r1, jj-r2 : jj+w+r2) are staged in LDS (FP32; the images
+// are FP32 in HBM).  Every kept template placement s1, s2 in [-1, 2r+1]
+// (funIntegerSearch.m:134) is scored with normxcorr2's definition on a
+// zero-padded D:
+//   num = sum(T.*Dw) - sum(Dw) sum(T)/mn,
+//   den = sqrt(sum((T - mean T)^2)) sqrt(max(sum(Dw.^2) - sum(Dw)^2/mn, 0)),
+//   C   = num/den if den > sqrt(eps(max den)) else 0.
+// FP32 x FP32 products are exact in FP64 and are accumulated in FP64, so the
+// map (and therefore the integer argmax) matches an FP64 evaluation up to
+// summation order.  The argmax takes the first maximum in column-major order
+// (max(f(:))); thread 0 then fits findpeak's 3x3 quadratic in closed form.
+#include "kernels.h"
+
+namespace staq {
+
+
+
+template <int BS>
+__global__ __launch_bounds__(BS) void search_kernel(SearchArgs a)
+{
+    constexpr int NW = BS / kWave;
+    extern __shared__ __align__(16) unsigned char smem[];
+    __shared__ double redd[NW * 3];
+    __shared__ double bestv[NW];
+    __shared__ int besti[NW];
+    __shared__ double sh_sT, sh_denT, sh_tol;
+    __shared__ int sh_best, sh_const;
+
+    const int tid = threadIdx.x;
+    const long t = a.point0 + blockIdx.x;
+    const int ix = (int)(t % a.nx), iy = (int)(t / a.nx);
+    const int ii = a.gx0 + ix * a.step, jj = a.gy0 + iy * a.step;  // 1-based corner
+    const int m = a.w + 1;
+    const int k1 = 2 * a.r1 + 3, k2 = 2 * a.r2 + 3;
+    const int d1 = m + 2 * a.r1, d2 = m + 2 * a.r2;
+    const int p1 = d1 + 2, p2 = d2 + 2;  // D with a one-pixel zero ring
+    float *T = reinterpret_cast<float *>(smem);
+    float *Dp = T + m * m;
+    double2 *map = a.scratch + (size_t)blockIdx.x * (size_t)(k1 * k2);
+
+    // stage template and padded search block
+    for (int q = tid; q < m * m; q += BS) {
+        const int i = q % m, j = q / m;
+        T[q] = a.f[(ii + i - 1) + (size_t)(jj + j - 1) * a.H];
+    }
+    for (int q = tid; q < p1 * p2; q += BS) {
+        const int i = q % p1, j = q / p1;
+        const int di = i - 1, dj = j - 1;  // 0-based in D
+        float v = 0.f;
+        if (di >= 0 && di < d1 && dj >= 0 && dj < d2)
+            v = a.g[(ii - a.r1 + di - 1) + (size_t)(jj - a.r2 + dj - 1) * a.H];
+        Dp[q] = v;
+    }
+    __syncthreads();
+
+    // template statistics (FP64)
+    {
+        double s[3] = {0, 0, 0};
+        const float t0 = T[0];
+        for (int q = tid; q < m * m; q += BS) {
+            s[0] += T[q];
+            s[2] += T[q] != t0 ? 1.0 : 0.0;
+        }
+        double o[3];
+        block_sum<BS, 3>(s, redd, o);
+        if (tid == 0) {
+            sh_sT = o[0];
+            sh_const = o[2] == 0.0;
+        }
+        __syncthreads();
+        double s2[1] = {0};
+        const double mT = sh_sT / (double)(m * m);
+        for (int q = tid; q < m * m; q += BS) s2[0] += ((double)T[q] - mT) * ((double)T[q] - mT);
+        double o2[1];
+        block_sum<BS, 1>(s2, redd, o2);
+        if (tid == 0) sh_denT = sqrt(o2[0]);
+        __syncthreads();
+    }
+    const long o_idx = iy + (long)ix * a.ny;
+    if (sh_const) {  // normxcorr2 rejects a constant template -> NaN (catch)
+        if (tid == 0) {
+            a.u[o_idx] = a.v[o_idx] = a.ccmax[o_idx] = NAN;
+            a.x0[o_idx] = (ii + ii + a.w) / 2.0;
+            a.y0[o_idx] = (jj + jj + a.w) / 2.0;
+        }
+        return;
+    }
+
+    // numerator / denominator for every kept placement
+    const double mn = (double)(m * m), sT = sh_sT, denT = sh_denT;
+    double dmax = 0.0;
+    for (int q = tid; q < k1 * k2; q += BS) {
+        const int ka = q % k1, kb = q / k1;  // placement s = k - 1, padded offset = k
+        double sTA = 0.0, sA = 0.0, sAA = 0.0;
+        for (int j = 0; j < m; ++j) {
+            const float *Tc = T + j * m;
+            const float *Dc = Dp + ka + (kb + j) * p1;
+            for (int i = 0; i < m; ++i) {
+                const double d = Dc[i];
+                sTA = fma((double)Tc[i], d, sTA);
+                sA += d;
+                sAA = fma(d, d, sAA);
+            }
+        }
+        const double num = sTA - sA * sT / mn;
+        const double dA = fmax(sAA - sA * sA / mn, 0.0);
+        const double den = denT * sqrt(dA);
+        map[q] = make_double2(num, den);
+        dmax = fmax(dmax, den);
+    }
+    // block max of den
+    {
+        double v = dmax;
+#pragma unroll
+        for (int o = kWave / 2; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o, kWave));
+        if ((tid & (kWave - 1)) == 0) redd[tid / kWave] = v;
+        __syncthreads();
+        if (tid == 0) {
+            double mx = redd[0];
+            for (int w = 1; w < NW; ++w) mx = fmax(mx, redd[w]);
+            sh_tol = sqrt(nextafter(mx, INFINITY) - mx);  // sqrt(eps(max den))
+        }
+        __syncthreads();
+    }
+    // first maximum in column-major order
+    {
+        const double tol = sh_tol;
+        double bv = -INFINITY;
+        int bi = 0x7fffffff;
+        for (int q = tid; q < k1 * k2; q += BS) {
+            const double2 nd = map[q];
+            const double cval = nd.y > tol ? nd.x / nd.y : 0.0;
+            if (cval > bv) {  // q increases, so ties keep the first index
+                bv = cval;
+                bi = q;
+            }
+        }
+#pragma unroll
+        for (int o = kWave / 2; o > 0; o >>= 1) {
+            const double ov = __shfl_xor(bv, o, kWave);
+            const int oi = __shfl_xor(bi, o, kWave);
+            if (ov > bv || (ov == bv && oi < bi)) {
+                bv = ov;
+                bi = oi;
+            }
+        }
+        if ((tid & (kWave - 1)) == 0) {
+            bestv[tid / kWave] = bv;
+            besti[tid / kWave] = bi;
+        }
+        __syncthreads();
+        if (tid == 0) {
+            double v = bestv[0];
+            int i = besti[0];
+            for (int w = 1; w < NW; ++w)
+                if (bestv[w] > v || (bestv[w] == v && besti[w] < i)) {
+                    v = bestv[w];
+                    i = besti[w];
+                }
+            sh_best = i;
+        }
+        __syncthreads();
+    }
+    if (tid != 0) return;
+
+    // findpeak.m:27-65 on the stored map
+    const double tol = sh_tol;
+    auto C = [&](int ka, int kb) {
+        const double2 nd = map[ka + kb * k1];
+        return nd.y > tol ? nd.x / nd.y : 0.0;
+    };
+    const int bi = sh_best;
+    const int yp = bi % k1 + 1, xp = bi / k1 + 1;  // row (dim 1), column (dim 2)
+    double xpeak = xp, ypeak = yp, maxf = C(yp - 1, xp - 1);
+    if (!(xp == 1 || xp == k2 || yp == 1 || yp == k1)) {
+        double su = 0, sxu = 0, syu = 0, sxyu = 0, sxxu = 0, syyu = 0;
+        for (int dx = -1; dx <= 1; ++dx)
+            for (int dy = -1; dy <= 1; ++dy) {
+                const double uu = C(yp - 1 + dy, xp - 1 + dx);
+                su += uu;
+                sxu += dx * uu;
+                syu += dy * uu;
+                sxyu += dx * dy * uu;
+                sxxu += dx * dx * uu;
+                syyu += dy * dy * uu;
+            }
+        // normal equations of the 3x3 quadratic fit in closed form
+        const double A1 = (5.0 * su - 3.0 * sxxu - 3.0 * syyu) / 9.0;
+        const double A2 = sxu / 6.0, A3 = syu / 6.0, A4 = sxyu / 4.0;
+        const double A5 = -su / 3.0 + sxxu / 2.0, A6 = -su / 3.0 + syyu / 2.0;
+        const double den = A4 * A4 - 4.0 * A5 * A6;
+        double xo = (-A3 * A4 + 2.0 * A6 * A2) / den;
+        double yo = -1.0 / den * (A4 * A2 - 2.0 * A5 * A3);
+        if (!(fabs(xo) > 1 || fabs(yo) > 1)) {
+            xo = round(1000.0 * xo) / 1000.0;
+            yo = round(1000.0 * yo) / 1000.0;
+            xpeak = xp + xo;
+            ypeak = yp + yo;
+            maxf = A1 + A2 * xo + A3 * yo + A4 * xo * yo + A5 * xo * xo + A6 * yo * yo;
+        }
+    }
+    a.u[o_idx] = ypeak - (double)(a.r1 + 2);  // zero_disp = ceil((2r+3)/2)
+    a.v[o_idx] = xpeak - (double)(a.r2 + 2);
+    a.ccmax[o_idx] = maxf;
+    a.x0[o_idx] = (ii + ii + a.w) / 2.0;
+    a.y0[o_idx] = (jj + jj + a.w) / 2.0;
+}
+
+size_t search_lds_bytes(int w, int r1, int r2)
+{
+    const size_t m = (size_t)w + 1;
+    return sizeof(float) * (m * m + (m + 2 * r1 + 2) * (m + 2 * r2 + 2));
+}
+
+hipError_t search_launch(const SearchArgs &a, long npoints, hipStream_t s)
+{
+    if (npoints <= 0) return hipSuccess;
+    const size_t lds = search_lds_bytes(a.w, a.r1, a.r2);
+    static bool attr = false;
+    if (!attr) {
+        hipFuncSetAttribute((const void *)search_kernel<256>,
+                            hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024 - 8192);
+        attr = true;
+    }
+    hipLaunchKernelGGL(search_kernel<256>, dim3((unsigned)npoints), dim3(256), lds, s, a);
+    return hipGetLastError();
+}
+
+}  // namespace staq

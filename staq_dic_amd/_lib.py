@@ -7,6 +7,7 @@ from __future__ import annotations
 
 import ctypes as C
 import os
+import weakref
 
 import numpy as np
 
@@ -134,8 +135,12 @@ class Context:
             raise StaqError(f"staq_ctx_create failed ({rc}): no usable HIP device {device}")
         self.h = h
         self.frames = {}
+        self._batches = weakref.WeakSet()
 
     def close(self):
+        # batches hold a pointer to the context: destroy them first
+        for b in list(getattr(self, "_batches", ())):
+            b.close()
         if getattr(self, "h", None):
             self.L.staq_ctx_destroy(self.h)
             self.h = None
@@ -192,11 +197,12 @@ class Batch:
                                            None if self.slot is None else ip(self.slot)),
                   "staq_batch_create")
         self.h = h
+        ctx._batches.add(self)
 
     def close(self):
-        if getattr(self, "h", None):
+        if getattr(self, "h", None) and getattr(self.ctx, "h", None):
             self.L.staq_batch_destroy(self.h)
-            self.h = None
+        self.h = None
 
     def __del__(self):
         try:

@@ -111,7 +111,9 @@ struct Bin {
     int wxcap = 0, wycap = 0;
     long item0 = 0, n = 0;
     int npix_cap = 0, tile_cap = 0;
-    int bs[2] = {0, 0};  // [mode 6, mode 2]
+    int bs[2] = {0, 0};  // [mode 6, mode 2]: 0 = wave kernel, else workgroup size
+    int ppt = 0;         // pixels per lane of the wave kernel (0: not eligible)
+    int wave_tile_cap = 0;
 };
 
 struct staq_batch {
@@ -542,7 +544,16 @@ int staq_batch_create(staq_ctx *ctx, staq_batch **out, long N, const double *coo
         bin.n = i1 - i0;
         bin.npix_cap = (bin.wxcap + 1) * (bin.wycap + 1);
         bin.tile_cap = tile_side(bin.wxcap + 1) * tile_side(bin.wycap + 1);
-        bin.bs[0] = bin.bs[1] = default_bs(bin.npix_cap);
+        // the wave kernel covers the bin's windows plus the mask-driven growth
+        // of its smallest class; larger grown windows defer to the fallback
+        const int wg = std::max(grown_window(bin.wxcap), grown_window(bin.wycap)) - 2;
+        bin.ppt = icgn_wave_ppt(std::max(bin.npix_cap, (wg + 1) * (wg + 1)));
+        if (!bin.ppt) bin.ppt = icgn_wave_ppt(bin.npix_cap);
+        if (bin.ppt) {
+            const int side = tile_side((int)std::sqrt(64.0 * bin.ppt));
+            bin.wave_tile_cap = side * side;
+        }
+        bin.bs[0] = bin.bs[1] = bin.ppt ? 0 : default_bs(bin.npix_cap);
         b->bins.push_back(bin);
         fbx = std::max(fbx, grown_window(bin.wxcap));
         fby = std::max(fby, grown_window(bin.wycap));
@@ -707,7 +718,7 @@ static int occupancy_grid(int bs, size_t lds, long n)
 // Enqueue one local step (mode 6 or 2) on stream s; bin_only >= 0 launches a
 // single bin (autotuning) without reset or fallback.
 static int enqueue_step(staq_batch *b, const staq_params *p, int mode, hipStream_t s,
-                        int bin_only = -1, int bs_override = 0)
+                        int bin_only = -1, int bs_override = -1)
 {
     staq_ctx *ctx = b->ctx;
     const int nb = (int)b->bins.size();
@@ -727,13 +738,24 @@ static int enqueue_step(staq_batch *b, const staq_params *p, int mode, hipStream
         a.npix_cap = bin.npix_cap;
         a.tile_cap = bin.tile_cap;
         a.defer = 1;
-        const int bs = bs_override ? bs_override : (p->block_size ? p->block_size : bin.bs[mi]);
+        int bs = bs_override >= 0 ? bs_override : (p->block_size ? p->block_size : bin.bs[mi]);
+        if (bs == 0 && !bin.ppt) bs = default_bs(bin.npix_cap);
         int grid = (int)bin.n;
-        if (p->persistent) {
-            a.work_counter = ints + k;
-            grid = occupancy_grid(bs, icgn_lds_bytes(a.npix_cap, a.tile_cap), bin.n);
+        if (bs == 0) {
+            // one wavefront per subset
+            a.tile_cap = bin.wave_tile_cap;
+            if (p->persistent) {
+                a.work_counter = ints + k;
+                grid = occupancy_grid(64, sizeof(float) * a.tile_cap + 2048, bin.n);
+            }
+            HIPCHK(ctx, icgn_wave_launch(a, mode, bin.ppt, grid, s));
+        } else {
+            if (p->persistent) {
+                a.work_counter = ints + k;
+                grid = occupancy_grid(bs, icgn_lds_bytes(a.npix_cap, a.tile_cap), bin.n);
+            }
+            HIPCHK(ctx, icgn_launch(a, mode, bs, grid, s));
         }
-        HIPCHK(ctx, icgn_launch(a, mode, bs, grid, s));
         if (bin_only < 0) {
             const unsigned g2 = (unsigned)((bin.n + 255) / 256);
             hipLaunchKernelGGL(defer_flag_kernel, dim3(g2), dim3(256), 0, s, b->status.as<int>(),
@@ -836,7 +858,7 @@ int staq_batch_autotune(staq_batch *b, const staq_params *p, int mode)
     hipEvent_t e0, e1;
     HIPCHK(ctx, hipEventCreate(&e0));
     HIPCHK(ctx, hipEventCreate(&e1));
-    const int cand[4] = {64, 128, 256, 512};
+    const int cand[5] = {0, 64, 128, 256, 512};
     const int mi = mode == 6 ? 0 : 1;
     int rc = STAQ_OK;
     for (int k = 0; k < (int)b->bins.size() && rc == STAQ_OK; ++k) {
@@ -844,6 +866,7 @@ int staq_batch_autotune(staq_batch *b, const staq_params *p, int mode)
         float best = 1e30f;
         int best_bs = bin.bs[mi];
         for (int c : cand) {
+            if (c == 0 && !bin.ppt) continue;
             if (c == 512 && bin.npix_cap < 1500) continue;
             if (c == 64 && bin.npix_cap > 4500) continue;
             float t[3];

@@ -27,10 +27,75 @@ enum : int { ST_DEFER = -2, ST_NOTCONV = 101, ST_MASKED = 102, ST_FEATURELESS = 
 
 constexpr int kWave = 64;
 
+// Loads through the global address space.  Pointers read from a struct in
+// memory are generic (flat) to the compiler, and flat loads share the LDS
+// wait counter, so every LDS wait would also wait for HBM.
+template <typename T>
+__device__ __forceinline__ T gld(const T *p, size_t i)
+{
+    return ((const __attribute__((address_space(1))) T *)p)[i];
+}
+
 __device__ __forceinline__ float fidx(const float *a, int H, int x, int y)
 {
     // 1-based (x, y) in an H-row column-major array
-    return a[(x - 1) + (size_t)(y - 1) * (size_t)H];
+    return gld(a, (size_t)((x - 1) + (y - 1) * H));  // images stay below 2^31 pixels
+}
+
+// One DPP move of a double (both 32-bit halves); lanes without a source get 0.
+template <int ctrl>
+__device__ __forceinline__ double dpp_f64(double v)
+{
+    const long long bits = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_update_dpp(0, (int)bits, ctrl, 0xf, 0xf, true);
+    const int hi = __builtin_amdgcn_update_dpp(0, (int)(bits >> 32), ctrl, 0xf, 0xf, true);
+    return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
+
+template <int ctrl>
+__device__ __forceinline__ unsigned long long dpp_u64(unsigned long long v)
+{
+    const int lo = __builtin_amdgcn_update_dpp(0, (int)v, ctrl, 0xf, 0xf, true);
+    const int hi = __builtin_amdgcn_update_dpp(0, (int)(v >> 32), ctrl, 0xf, 0xf, true);
+    return ((unsigned long long)(unsigned int)hi << 32) | (unsigned int)lo;
+}
+
+__device__ __forceinline__ double readlane_f64(double v, int l)
+{
+    const long long bits = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_readlane((int)bits, l);
+    const int hi = __builtin_amdgcn_readlane((int)(bits >> 32), l);
+    return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
+
+// FP64 wavefront sum with the same DPP scheme, result in every lane.
+__device__ __forceinline__ double wave_sum_dpp_f64(double v)
+{
+    v += dpp_f64<0x111>(v);
+    v += dpp_f64<0x112>(v);
+    v += dpp_f64<0x114>(v);
+    v += dpp_f64<0x118>(v);
+    return (readlane_f64(v, 15) + readlane_f64(v, 31)) + (readlane_f64(v, 47) + readlane_f64(v, 63));
+}
+
+// Sum over the 64 lanes of a wavefront, result in every lane: DPP row shifts
+// build each 16-lane row's sum in its last lane, four readlanes add the rows.
+__device__ __forceinline__ float wave_sum_dpp(float v)
+{
+    int x = __float_as_int(v);
+    v += __int_as_float(__builtin_amdgcn_update_dpp(0, x, 0x111, 0xf, 0xf, true));  // row_shr:1
+    x = __float_as_int(v);
+    v += __int_as_float(__builtin_amdgcn_update_dpp(0, x, 0x112, 0xf, 0xf, true));  // row_shr:2
+    x = __float_as_int(v);
+    v += __int_as_float(__builtin_amdgcn_update_dpp(0, x, 0x114, 0xf, 0xf, true));  // row_shr:4
+    x = __float_as_int(v);
+    v += __int_as_float(__builtin_amdgcn_update_dpp(0, x, 0x118, 0xf, 0xf, true));  // row_shr:8
+    x = __float_as_int(v);
+    const float r0 = __int_as_float(__builtin_amdgcn_readlane(x, 15));
+    const float r1 = __int_as_float(__builtin_amdgcn_readlane(x, 31));
+    const float r2 = __int_as_float(__builtin_amdgcn_readlane(x, 47));
+    const float r3 = __int_as_float(__builtin_amdgcn_readlane(x, 63));
+    return (r0 + r1) + (r2 + r3);
 }
 
 // Keys cubic-convolution weights, a = -0.5 (ba_interp2.cpp:204-213).
@@ -60,29 +125,6 @@ __device__ __forceinline__ T wave_sum(T v)
 #pragma unroll
     for (int o = kWave / 2; o > 0; o >>= 1) v += __shfl_xor(v, o, kWave);
     return v;
-}
-
-// Sum K values over the block.  Every thread passes its partials in v[];
-// the totals land in out[] of thread 0 only.  red must hold K * (BS/64).
-template <int BS, int K, typename T>
-__device__ __forceinline__ void block_sum(T (&v)[K], T *red, T (&out)[K])
-{
-    const int lane = threadIdx.x & (kWave - 1), wid = threadIdx.x / kWave;
-#pragma unroll
-    for (int k = 0; k < K; ++k) {
-        const T s = wave_sum(v[k]);
-        if (lane == 0) red[wid * K + k] = s;
-    }
-    __syncthreads();
-    if (threadIdx.x == 0) {
-#pragma unroll
-        for (int k = 0; k < K; ++k) {
-            T s = red[k];
-#pragma unroll
-            for (int w = 1; w < BS / kWave; ++w) s += red[w * K + k];
-            out[k] = s;
-        }
-    }
 }
 
 // MATLAB mldivide for a small symmetric system: Cholesky when the diagonal is
@@ -261,12 +303,13 @@ __device__ inline bool ic_compose(double P[6], const double d[6])
 {
     const double det = (1 + d[0]) * (1 + d[3]) - d[1] * d[2];
     if (!(det != 0)) return false;
-    const double q1 = (-d[0] - d[0] * d[3] + d[1] * d[2]) / det;
-    const double q2 = -d[1] / det;
-    const double q3 = -d[2] / det;
-    const double q4 = (-d[3] - d[0] * d[3] + d[1] * d[2]) / det;
-    const double q5 = (-d[4] - d[3] * d[4] + d[2] * d[5]) / det;
-    const double q6 = (-d[5] - d[0] * d[5] + d[1] * d[4]) / det;
+    const double id = 1.0 / det;
+    const double q1 = (-d[0] - d[0] * d[3] + d[1] * d[2]) * id;
+    const double q2 = -d[1] * id;
+    const double q3 = -d[2] * id;
+    const double q4 = (-d[3] - d[0] * d[3] + d[1] * d[2]) * id;
+    const double q5 = (-d[4] - d[3] * d[4] + d[2] * d[5]) * id;
+    const double q6 = (-d[5] - d[0] * d[5] + d[1] * d[4]) * id;
     const double a11 = 1 + P[0], a12 = P[2], a13 = P[4];
     const double a21 = P[1], a22 = 1 + P[3], a23 = P[5];
     const double b11 = 1 + q1, b12 = q3, b13 = q5;
@@ -278,6 +321,33 @@ __device__ inline bool ic_compose(double P[6], const double d[6])
     P[4] = a11 * b13 + a12 * b23 + a13;
     P[5] = a21 * b13 + a22 * b23 + a23;
     return true;
+}
+
+// Sum K values over the block.  Every thread passes its partials in v[];
+// the totals land in out[] of thread 0 only.  red must hold K * (BS/64).
+template <int BS, int K, typename T>
+__device__ __forceinline__ void block_sum(T (&v)[K], T *red, T (&out)[K])
+{
+    const int lane = threadIdx.x & (kWave - 1), wid = threadIdx.x / kWave;
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+        T s;
+        if constexpr (sizeof(T) == 4)
+            s = wave_sum_dpp(v[k]);
+        else
+            s = wave_sum(v[k]);
+        if (lane == 0) red[wid * K + k] = s;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+#pragma unroll
+        for (int k = 0; k < K; ++k) {
+            T s = red[k];
+#pragma unroll
+            for (int w = 1; w < BS / kWave; ++w) s += red[w * K + k];
+            out[k] = s;
+        }
+    }
 }
 
 }  // namespace staq

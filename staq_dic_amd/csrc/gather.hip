@@ -70,29 +70,34 @@ hipError_t ba_interp2_launch(const void *F, long M, long N, long P, const void *
 // ---------------------------------------------------------------------------
 // func/funImgGradient.m:53-78: 7-tap central difference (imfilter correlation
 // with [-1/60 3/20 -3/4 0 3/4 -3/20 1/60]), cropped by 3, times the mask.
-// Thread (x, y) of the (H-6) x (W-6) output; consecutive threads walk dim 1 so
-// both the dim-1 taps and the output are coalesced.
+// The taps are summed in FP64 in filter order with contraction off and the
+// result rounded to FP32: funICGN.m:54 tests DfDx == 0, so flat image regions
+// must give the same exact zeros (or the same rounding residue) as an FP64
+// evaluation of the filter, not whatever a reassociated FP32 sum gives.
+// Thread (x, y) of the (H-6) x (W-6) output; consecutive threads walk dim 1.
 // ---------------------------------------------------------------------------
 __global__ __launch_bounds__(256) void img_gradient_kernel(const float *__restrict__ f,
                                                            const float *__restrict__ mask, int H,
                                                            int W, float *__restrict__ dfdx,
                                                            float *__restrict__ dfdy)
 {
+#pragma clang fp contract(off)
+    const double h[7] = {-1.0 / 60, 3.0 / 20, -3.0 / 4, 0.0, 3.0 / 4, -3.0 / 20, 1.0 / 60};
     const int Hd = H - 6, Wd = W - 6;
     const int i = blockIdx.x * blockDim.x + threadIdx.x;  // 0..Hd-1
     const int j = blockIdx.y;                             // 0..Wd-1
     if (i >= Hd || j >= Wd) return;
     const int x = i + 3, y = j + 3;  // 0-based pixel
-    const float c1 = 3.0f / 4.0f, c2 = -3.0f / 20.0f, c3 = 1.0f / 60.0f;
     const float *col = f + (size_t)y * H;
-    const float sx = c1 * (col[x + 1] - col[x - 1]) + c2 * (col[x + 2] - col[x - 2]) +
-                     c3 * (col[x + 3] - col[x - 3]);
-    const float sy = c1 * (f[x + (size_t)(y + 1) * H] - f[x + (size_t)(y - 1) * H]) +
-                     c2 * (f[x + (size_t)(y + 2) * H] - f[x + (size_t)(y - 2) * H]) +
-                     c3 * (f[x + (size_t)(y + 3) * H] - f[x + (size_t)(y - 3) * H]);
-    const float m = mask[x + (size_t)y * H];
-    dfdx[i + (size_t)j * Hd] = sx * m;
-    dfdy[i + (size_t)j * Hd] = sy * m;
+    double sx = 0.0, sy = 0.0;
+#pragma unroll
+    for (int k = 0; k < 7; ++k) {
+        sx += h[k] * (double)col[x + k - 3];
+        sy += h[k] * (double)f[x + (size_t)(y + k - 3) * H];
+    }
+    const double m = mask[x + (size_t)y * H];
+    dfdx[i + (size_t)j * Hd] = (float)(sx * m);
+    dfdy[i + (size_t)j * Hd] = (float)(sy * m);
 }
 
 hipError_t img_gradient_launch(const float *f, const float *mask, int H, int W, float *dfdx,

@@ -214,7 +214,7 @@ __global__ __launch_bounds__(BS) void icgn_kernel(IcgnArgs a)
                     pix_ij(p, nx, rcp, i, j);
                     const int x = xa + i, y = ya + j;
                     if (MODE == 6 && a.plain)
-                        cnt += fr.dfdx[(x - 4) + (size_t)(y - 4) * Hd] == 0.f;
+                        cnt += gld(fr.dfdx, (size_t)((x - 4) + (y - 4) * Hd)) == 0.f;
                     else
                         cnt += fidx(fr.f, fr.H, x, y) * fidx(fr.mask, fr.H, x, y) == 0.f;
                 }
@@ -301,9 +301,9 @@ __global__ __launch_bounds__(BS) void icgn_kernel(IcgnArgs a)
                 pix_ij(p, nx, rcp, i, j);
                 const int x = xa + i, y = ya + j;
                 const bool s = all || sel0[p];
-                const size_t gi = (x - 4) + (size_t)(y - 4) * Hd;
-                const double fx = s ? (double)fr.dfdx[gi] : 0.0;
-                const double fy = s ? (double)fr.dfdy[gi] : 0.0;
+                const size_t gi = (size_t)((x - 4) + (y - 4) * Hd);
+                const double fx = s ? (double)gld(fr.dfdx, gi) : 0.0;
+                const double fy = s ? (double)gld(fr.dfdy, gi) : 0.0;
                 double fv = fidx(fr.f, fr.H, x, y);
                 if (!(MODE == 6 && a.plain)) fv *= (double)fidx(fr.mask, fr.H, x, y);
                 if (!s) fv = 0.0;
@@ -370,9 +370,9 @@ __global__ __launch_bounds__(BS) void icgn_kernel(IcgnArgs a)
                 pix_ij(p, nx, rcp, i, j);
                 const int x = xa + i, y = ya + j;
                 const bool s = all || sel0[p];
-                const size_t gi = (x - 4) + (size_t)(y - 4) * Hd;
-                const double fx = s ? (double)fr.dfdx[gi] : 0.0;
-                const double fy = s ? (double)fr.dfdy[gi] : 0.0;
+                const size_t gi = (size_t)((x - 4) + (y - 4) * Hd);
+                const double fx = s ? (double)gld(fr.dfdx, gi) : 0.0;
+                const double fy = s ? (double)gld(fr.dfdy, gi) : 0.0;
                 const double dx = i - (double)hx, dy = j - (double)hy;
                 const double J[6] = {fx * dx, fy * dx, fx * dy, fy * dy, fx, fy};
                 int t = 0;
@@ -497,13 +497,25 @@ __global__ __launch_bounds__(BS) void icgn_kernel(IcgnArgs a)
             const int tu0 = c.tile_u0, tv0 = c.tile_v0, tnx = c.tnx, tny = c.tny;
             const bool use_tile = c.use_tile;
             if (use_tile) {
-                const int lane = tid & (kWave - 1), wid = tid / kWave;
-                for (int s = wid; s < tny; s += NW) {
-                    const int y = min(max(tv0 + s, 1), fr.W);
-                    const float *col = fr.g + (size_t)(y - 1) * fr.H;
-                    for (int r = lane; r < tnx; r += kWave) {
-                        const int x = min(max(tu0 + r, 1), fr.H);
-                        tile[r + s * tnx] = col[x - 1];
+                // flat over the tile, 4 independent loads in flight per thread
+                const int total = tnx * tny;
+                const float rcpn = 1.0f / tnx;
+                for (int t0 = 0; t0 < total; t0 += BS * 4) {
+                    float v[4];
+#pragma unroll
+                    for (int q = 0; q < 4; ++q) {
+                        const int t = t0 + tid + BS * q;
+                        v[q] = 0.f;
+                        if (t < total) {
+                            const int sj = (int)(((float)t + 0.5f) * rcpn), ri = t - sj * tnx;
+                            const int y = min(max(tv0 + sj, 1), fr.W), x = min(max(tu0 + ri, 1), fr.H);
+                            v[q] = gld(fr.g, (size_t)((x - 1) + (y - 1) * fr.H));
+                        }
+                    }
+#pragma unroll
+                    for (int q = 0; q < 4; ++q) {
+                        const int t = t0 + tid + BS * q;
+                        if (t < total) tile[t] = v[q];
                     }
                 }
             }
@@ -543,7 +555,7 @@ __global__ __launch_bounds__(BS) void icgn_kernel(IcgnArgs a)
                             const float *col = fr.g + (size_t)(y - 1) * fr.H;
                             float s = 0.f;
 #pragma unroll
-                            for (int r = 0; r < 4; ++r) s = fmaf(wu[r], col[min(max(tu0 + iu + r, 1), fr.H) - 1], s);
+                            for (int r = 0; r < 4; ++r) s = fmaf(wu[r], gld(col, (size_t)(min(max(tu0 + iu + r, 1), fr.H) - 1)), s);
                             g = fmaf(wv[q], s, g);
                         }
                     }
@@ -556,8 +568,8 @@ __global__ __launch_bounds__(BS) void icgn_kernel(IcgnArgs a)
                     acc[3] += g == 0.f ? 1.f : 0.f;
                     acc[4] += isnan(g) ? 1.f : 0.f;
                     const bool s = all || sel0[p];
-                    const size_t gi = (xa + i - 4) + (size_t)(ya + j - 4) * Hd;
-                    const float fxv = s ? fr.dfdx[gi] : 0.f, fyv = s ? fr.dfdy[gi] : 0.f;
+                    const size_t gi = (size_t)((xa + i - 4) + (ya + j - 4) * Hd);
+                    const float fxv = s ? gld(fr.dfdx, gi) : 0.f, fyv = s ? gld(fr.dfdy, gi) : 0.f;
                     const float u = fxv * d, v = fyv * d;
                     if (MODE == 6) {
                         acc[6] = fmaf(dx, u, acc[6]);
@@ -648,9 +660,9 @@ __global__ __launch_bounds__(BS) void icgn_kernel(IcgnArgs a)
                             pix_ij(p, nx, rcp, i, j);
                             const int x = xa + i, y = ya + j;
                             const bool s = sel[p];
-                            const size_t gi = (x - 4) + (size_t)(y - 4) * Hd;
-                            const double fx = s ? (double)fr.dfdx[gi] : 0.0;
-                            const double fy = s ? (double)fr.dfdy[gi] : 0.0;
+                            const size_t gi = (size_t)((x - 4) + (y - 4) * Hd);
+                            const double fx = s ? (double)gld(fr.dfdx, gi) : 0.0;
+                            const double fy = s ? (double)gld(fr.dfdy, gi) : 0.0;
                             const double fv = s ? (double)fidx(fr.f, fr.H, x, y) * (double)fidx(fr.mask, fr.H, x, y) : 0.0;
                             const bool fnz = fabs(fv) > 1e-10;
                             v[3] += fnz ? 1.0 : 0.0;
@@ -736,9 +748,9 @@ __global__ __launch_bounds__(BS) void icgn_kernel(IcgnArgs a)
                             if (!(MODE == 6 && a.plain)) fv *= (double)fidx(fr.mask, fr.H, x, y);
                             if (!s) fv = 0.0;
                         }
-                        const size_t gi = (x - 4) + (size_t)(y - 4) * Hd;
-                        const double fx = s ? (double)fr.dfdx[gi] : 0.0;
-                        const double fy = s ? (double)fr.dfdy[gi] : 0.0;
+                        const size_t gi = (size_t)((x - 4) + (y - 4) * Hd);
+                        const double fx = s ? (double)gld(fr.dfdx, gi) : 0.0;
+                        const double fy = s ? (double)gld(fr.dfdy, gi) : 0.0;
                         const double r = (fv - mf) / bf - ((double)gbuf[p] - mg) / bg;
                         if (MODE == 6) {
                             const double dx = i - (double)hx, dy = j - (double)hy;

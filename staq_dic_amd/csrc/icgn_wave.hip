@@ -1,0 +1,1012 @@
+// Wave-per-subset IC-GN kernel for gfx950 (the fast path of the batched local
+// step).  Same semantics as icgn.hip (funICGNQuadtree.m / funICGN.m /
+// funICGN_Subpb1.m); differs only in how the work is laid out:
+//
+//   * one 64-lane wavefront fits one subset; a workgroup is one wavefront, so
+//     every barrier is a single-wave s_barrier;
+//   * each lane keeps its pixels' df/dx, df/dy and g in registers (PPT pixels
+//     per lane, pixel p = lane + 64 k of the column-major window);
+//   * the per-subset control state (warp P, norms, statistics) is computed
+//     redundantly by all 64 lanes in FP64 registers, so the Gauss-Newton step
+//     needs no serial single-thread section and no LDS round trip;
+//   * sums are FP32 per lane and reduced with a butterfly (every lane ends
+//     with the total); the inverse Hessian (FP64) sits in LDS and is read with
+//     broadcast loads;
+//   * bwselect regions are flood-filled bit-parallel: window column j is a
+//     64-bit word held by lane j, one propagation step is a few shifts and two
+//     cross-lane shuffles.
+//
+// Windows that do not fit 64 * PPT pixels after the mask-driven growth are
+// deferred (status -2) to the workgroup kernel of icgn.hip.
+#include "kernels.h"
+
+namespace staq {
+
+namespace {
+
+template <typename T>
+__device__ __forceinline__ T wave_allsum(T v)
+{
+#pragma unroll
+    for (int o = 1; o < kWave; o <<= 1) v += __shfl_xor(v, o, kWave);
+    return v;
+}
+
+__device__ __forceinline__ bool wave_any(bool p) { return __ballot(p) != 0ull; }
+
+// Connected component (4- or 8-connected) through the seed of the window
+// whose column j is the 64-bit word bw held by lane j (bit i = row i).
+__device__ unsigned long long wave_flood(unsigned long long bw, int ny, int seed_r, int seed_c,
+                                         int conn)
+{
+    const int lane = threadIdx.x;
+    unsigned long long sel = 0ull;
+    if (lane < ny && seed_c >= 1 && seed_c <= ny && seed_r >= 1 && seed_r <= 64 &&
+        lane == seed_c - 1)
+        sel = bw & (1ull << (seed_r - 1));
+    if (lane >= ny) bw = 0ull;
+    for (;;) {
+        unsigned long long l = dpp_u64<0x138>(sel);  // wave_shr:1 - column j-1
+        unsigned long long r = dpp_u64<0x130>(sel);  // wave_shl:1 - column j+1
+        if (conn == 8) {
+            l |= (l << 1) | (l >> 1);
+            r |= (r << 1) | (r >> 1);
+        }
+        // propagate along the column until stable, then across
+        unsigned long long n = bw & (sel | (sel << 1) | (sel >> 1) | l | r);
+        unsigned long long m = n;
+        for (int s = 0; s < 6; ++s) m = bw & (m | (m << 1) | (m >> 1));
+        const bool ch = m != sel;
+        sel = m;
+        if (!wave_any(ch)) break;
+    }
+    return sel;
+}
+
+// In-register Cholesky of a 6x6 SPD matrix given by its packed lower
+// triangle (column-major: (0,0),(1,0)..(5,0),(1,1)..); false if not SPD.
+__device__ __forceinline__ bool chol6(double (&h)[21])
+{
+#define PK(r, c) h[(c) * 6 - ((c) * ((c) - 1)) / 2 + ((r) - (c))]
+#pragma unroll
+    for (int j = 0; j < 6; ++j) {
+        double d = PK(j, j);
+#pragma unroll
+        for (int k = 0; k < j; ++k) d -= PK(j, k) * PK(j, k);
+        if (!(d > 0)) return false;
+        const double l = sqrt(d);
+        PK(j, j) = l;
+        const double il = 1.0 / l;
+#pragma unroll
+        for (int i = j + 1; i < 6; ++i) {
+            double s = PK(i, j);
+#pragma unroll
+            for (int k = 0; k < j; ++k) s -= PK(i, k) * PK(j, k);
+            PK(i, j) = s * il;
+        }
+    }
+#undef PK
+    return true;
+}
+
+struct WaveLds;
+
+struct WaveLds {
+    double Lp[21];     // packed Cholesky factor of H (column-major lower triangle)
+    double Ld[6];      // 1 / diag(Lp)
+    double A[36];      // LU factors when H is not positive definite
+    double work[36];
+    int piv[6], kind;  // kind 0: Cholesky in Lp, 1: LU in A
+    double H2i[4];     // 2-DOF: inverse of the damped 2x2 system
+    double FJ[6], SJ[6];
+    unsigned long long colw[64];  // scratch column words (atomic OR target)
+    unsigned long long tmw[64];   // window-mask columns
+    unsigned long long sel0w[64]; // setup region (4-connected mask region)
+};
+
+// Solve H d = -b with the factor in LDS (all lanes read it with broadcast loads).
+__device__ __forceinline__ void solve6_lds(const WaveLds &L, const double (&b)[6], double (&d)[6])
+{
+    double y[6];
+    if (L.kind == 0) {
+#define LP(r, c) L.Lp[(c) * 6 - ((c) * ((c) - 1)) / 2 + ((r) - (c))]
+#pragma unroll
+        for (int i = 0; i < 6; ++i) {
+            double s = -b[i];
+#pragma unroll
+            for (int k = 0; k < i; ++k) s -= LP(i, k) * y[k];
+            y[i] = s * L.Ld[i];
+        }
+#pragma unroll
+        for (int i = 5; i >= 0; --i) {
+            double s = y[i];
+#pragma unroll
+            for (int k = i + 1; k < 6; ++k) s -= LP(k, i) * d[k];
+            d[i] = s * L.Ld[i];
+        }
+#undef LP
+    } else {
+#pragma unroll
+        for (int i = 0; i < 6; ++i) {
+            double s = -b[L.piv[i]];
+            for (int k = 0; k < i; ++k) s -= L.A[i + k * 6] * y[k];
+            y[i] = s;
+        }
+        for (int i = 5; i >= 0; --i) {
+            double s = y[i];
+            for (int k = i + 1; k < 6; ++k) s -= L.A[i + k * 6] * d[k];
+            d[i] = s / L.A[i + i * 6];
+        }
+    }
+}
+
+// 2x2 damped system of funICGN_Subpb1.m:111,391-394, inverted by lane 0.
+__device__ __forceinline__ void set_h2(WaveLds &L, double h11, double h12, double h22, double bf,
+                                       double mu)
+{
+    const double s2 = 2.0 / (bf * bf);
+    const double H2a = h11 * s2 + mu, H2b = h12 * s2, H2d = h22 * s2 + mu;
+    const double dl = 1e-3 * (H2a > H2d ? H2a : H2d);
+    if (threadIdx.x == 0) {
+        const double A[4] = {H2a + dl, H2b, H2b, H2d + dl};
+        for (int c = 0; c < 2; ++c) {
+            double e[2] = {0, 0}, xs[2];
+            e[c] = 1.0;
+            solve_small<2>(A, e, xs);
+            L.H2i[0 + c * 2] = xs[0];
+            L.H2i[1 + c * 2] = xs[1];
+        }
+    }
+}
+
+template <int PPT, int MODE>
+__global__ __launch_bounds__(64) void icgn_wave_kernel(IcgnArgs a)
+{
+    constexpr int NJ = MODE;
+    constexpr int CAP = 64 * PPT;
+    extern __shared__ __align__(16) unsigned char smem[];
+    __shared__ WaveLds L;
+    __shared__ long item_sh;
+    float *tile = reinterpret_cast<float *>(smem);
+    const int lane = threadIdx.x;
+    const bool quad = !(MODE == 6 && a.plain);
+
+    for (long pass = 0;; ++pass) {
+        if (lane == 0)
+            item_sh = a.work_counter ? a.item0 + atomicAdd(a.work_counter, 1)
+                                     : (pass == 0 ? a.item0 + blockIdx.x : a.item0 + a.n_items);
+        __syncthreads();
+        const long item = item_sh;
+        if (item >= a.item0 + a.n_items) break;
+        const long idx = a.order ? a.order[item] : item;
+        const int slot = a.slot ? a.slot[idx] : 0;
+        const FrameDev fr = a.frames[slot];
+        const int Hd = fr.H - 6;
+
+        double x0 = a.coords[idx], y0 = a.coords[idx + a.N];
+        if (!a.plain) {
+            x0 = round(x0);  // LocalICGNQuadtree.m:69
+            y0 = round(y0);
+        }
+        int wx = a.wx[idx];
+        int wy = MODE == 2 ? a.wy[idx] : wx;
+        const int w0 = wx;
+        double P[6];
+        if (MODE == 6) {
+            P[0] = P[1] = P[2] = P[3] = 0.0;
+            P[4] = a.U0[2 * idx];
+            P[5] = a.U0[2 * idx + 1];
+        } else {
+            for (int k = 0; k < 4; ++k) P[k] = a.FOld[4 * idx + k];
+            P[4] = a.UOld[2 * idx];
+            P[5] = a.UOld[2 * idx + 1];
+        }
+        int state = 0, status = 0;  // 0 run, 1 loop done, 2 final status, 3 deferred
+        auto win_fits = [&](int wxx, int wyy) {
+            const double xa = x0 - wxx * 0.5, xb = x0 + wxx * 0.5, ya = y0 - wyy * 0.5,
+                         yb = y0 + wyy * 0.5;
+            if (xa != floor(xa) || ya != floor(ya) || xb != floor(xb) || yb != floor(yb)) return false;
+            return xa >= 4 && ya >= 4 && xb <= fr.H - 3 && yb <= fr.W - 3;
+        };
+        if (!win_fits(wx, wy)) {
+            state = 2;
+            status = ST_ERROR;
+        } else if ((wx + 1) * (wy + 1) > CAP || wx + 1 > 64 || wy + 1 > 64) {
+            state = 3;
+        }
+
+        // ------------------------------------------------------------ setup
+        int nx = wx + 1, ny = wy + 1, npix = nx * ny;
+        int xa = (int)x0 - wx / 2, ya = (int)y0 - wy / 2;
+        int count = 0;
+        if (state == 0) {
+            // masked-pixel count (funICGNQuadtree.m:61 / funICGN.m:54)
+            int cnt = 0;
+            const float rnx = 1.0f / (float)nx;
+#pragma unroll
+            for (int k = 0; k < PPT; ++k) {
+                const int p = lane + 64 * k;
+                if (p < npix) {
+                    const int j = (int)(((float)p + 0.5f) * rnx), i = p - j * nx;
+                    const int x = xa + i, y = ya + j;
+                    if (MODE == 6 && a.plain)
+                        cnt += gld(fr.dfdx, (size_t)((x - 4) + (y - 4) * Hd)) == 0.f;
+                    else
+                        cnt += fidx(fr.f, fr.H, x, y) * fidx(fr.mask, fr.H, x, y) == 0.f;
+                }
+            }
+            count = wave_allsum(cnt);
+            const double n0 = (double)nx * ny;
+            const double cap = (MODE == 6 && a.plain) ? 0.5 * n0 : 0.4 * n0;
+            if (!(count < cap)) {
+                state = 2;
+                status = ST_MASKED;
+            } else {
+                bool grow;
+                int gx = wx, gy = wy;
+                if (MODE == 6 && a.plain) {
+                    grow = count > 0.1 * n0;
+                    if (grow) gx = gy = (int)round(sqrt(2.0) * wx);
+                } else {
+                    grow = count > 0;
+                    if (grow) {
+                        gx = 2 * (int)ceil(sqrt(n0 / (n0 - count)) * wx / 2.0);
+                        if (MODE == 6) {
+                            gy = gx;
+                        } else {
+                            const double n1 = (double)(gx + 1) * (wy + 1);
+                            gy = 2 * (int)ceil(sqrt(n1 / (n1 - count)) * wy / 2.0);
+                        }
+                    }
+                }
+                if (grow) {
+                    if (!win_fits(gx, gy)) {
+                        state = 2;
+                        status = ST_ERROR;
+                    } else if ((gx + 1) * (gy + 1) > CAP || gx + 1 > 64 || gy + 1 > 64) {
+                        state = 3;
+                    } else {
+                        wx = gx;
+                        wy = gy;
+                        nx = wx + 1;
+                        ny = wy + 1;
+                        npix = nx * ny;
+                        xa = (int)x0 - wx / 2;
+                        ya = (int)y0 - wy / 2;
+                    }
+                }
+            }
+        }
+        const bool masked_f = quad && count > 0;
+        const float rnx = 1.0f / (float)nx;
+        const float hx = wx * 0.5f, hy = wy * 0.5f;
+        const int seed_c = (wx + 1) / 2, seed_r = (wy + 1) / 2;  // bwselect(BW, c, r): c col, r row
+
+        if (state == 0 && masked_f) {
+            // window mask columns and the 4-connected region through the seed (:83-88)
+            L.colw[lane] = 0ull;
+            __syncthreads();
+#pragma unroll
+            for (int k = 0; k < PPT; ++k) {
+                const int p = lane + 64 * k;
+                if (p < npix) {
+                    const int j = (int)(((float)p + 0.5f) * rnx), i = p - j * nx;
+                    if (fidx(fr.mask, fr.H, xa + i, ya + j) != 0.f) atomicOr(&L.colw[j], 1ull << i);
+                }
+            }
+            __syncthreads();
+            const unsigned long long tm = L.colw[lane];
+            L.tmw[lane] = tm;
+            L.sel0w[lane] = wave_flood(tm, ny, seed_r, seed_c, 4);
+            __syncthreads();
+        }
+
+        // per-pixel df/dx, df/dy (and f*mask for the 2-DOF region rebuild) in
+        // registers, s0bits = pixels inside the setup region; f statistics and
+        // steepest-descent sums in one pass about the shift c0 = f at the centre
+        constexpr bool FM = MODE == 2 && PPT <= 9;
+        float fxr[PPT], fyr[PPT], fmr[FM ? PPT : 1];
+        unsigned s0bits = ~0u;
+        double meanf = 0, bottomf = 0, sabs = 0, normNew = 1, normAbs = 1, normInit = 0;
+        if (state == 0) {
+            const float c0 = fidx(fr.f, fr.H, xa + nx / 2, ya + ny / 2) *
+                             (quad ? fidx(fr.mask, fr.H, xa + nx / 2, ya + ny / 2) : 1.f);
+            float s_n = 0.f, s_1 = 0.f, s_2 = 0.f, s_abs = 0.f;
+            float sj[NJ], sjf[NJ];
+            constexpr int NH = MODE == 6 ? 21 : 3;
+            float hs[NH];
+#pragma unroll
+            for (int q = 0; q < NJ; ++q) sj[q] = sjf[q] = 0.f;
+#pragma unroll
+            for (int q = 0; q < NH; ++q) hs[q] = 0.f;
+#pragma unroll
+            for (int k = 0; k < PPT; ++k) {
+                const int p = lane + 64 * k;
+                fxr[k] = fyr[k] = 0.f;
+                if (FM) fmr[FM ? k : 0] = 0.f;
+                if (p < npix) {
+                    const int j = (int)(((float)p + 0.5f) * rnx), i = p - j * nx;
+                    const int x = xa + i, y = ya + j;
+                    const size_t gi = (size_t)((x - 4) + (y - 4) * Hd);
+                    fxr[k] = gld(fr.dfdx, gi);
+                    fyr[k] = gld(fr.dfdy, gi);
+                    float f = fidx(fr.f, fr.H, x, y);
+                    if (quad) f *= fidx(fr.mask, fr.H, x, y);
+                    if (FM) fmr[FM ? k : 0] = f;
+                    const bool s = !masked_f || ((L.sel0w[j] >> i) & 1ull);
+                    if (!s) s0bits &= ~(1u << k);
+                    const float fx = s ? fxr[k] : 0.f, fy = s ? fyr[k] : 0.f;
+                    if (!s) f = 0.f;
+                    const bool nz = !quad || fabsf(f) > 1e-10f;
+                    const float df = f - c0;
+                    s_n += nz ? 1.f : 0.f;
+                    s_1 += nz ? df : 0.f;
+                    s_2 += nz ? df * df : 0.f;
+                    s_abs += fabsf(f);
+                    if (MODE == 6) {
+                        const float dx = (float)i - hx, dy = (float)j - hy;
+                        const float J[6] = {fx * dx, fy * dx, fx * dy, fy * dy, fx, fy};
+                        int t = 0;
+#pragma unroll
+                        for (int c = 0; c < 6; ++c) {
+                            sj[c] += J[c];
+                            sjf[c] += J[c] * df;
+#pragma unroll
+                            for (int r = c; r < 6; ++r) hs[t++] += J[r] * J[c];
+                        }
+                    } else {
+                        sj[0] += fx;
+                        sj[1] += fy;
+                        sjf[0] += fx * df;
+                        sjf[1] += fy * df;
+                        hs[0] += fx * fx;
+                        hs[1] += fx * fy;
+                        hs[2] += fy * fy;
+                    }
+                }
+            }
+            const double n = wave_sum_dpp_f64((double)s_n);
+            const double S1 = wave_sum_dpp_f64((double)s_1), S2 = wave_sum_dpp_f64((double)s_2);
+            sabs = wave_sum_dpp_f64((double)s_abs);
+            meanf = n > 0 ? (double)c0 + S1 / n : NAN;
+            bottomf = n >= 2 ? sqrt(fmax(S2 - S1 * S1 / n, 0.0)) : (n == 1 ? 0.0 : NAN);
+            const double dm = (double)c0 - meanf;
+#pragma unroll
+            for (int q = 0; q < NH; ++q) hs[q] = wave_sum_dpp(hs[q]);
+#pragma unroll
+            for (int q = 0; q < NJ; ++q) {
+                const double SJq = wave_sum_dpp_f64((double)sj[q]);
+                const double SJFq = wave_sum_dpp_f64((double)sjf[q]);
+                if (lane == 0) {
+                    L.SJ[q] = SJq;
+                    L.FJ[q] = SJFq + dm * SJq;  // sum J (f - meanf)
+                }
+            }
+            if (MODE == 6) {
+                double h[21];
+#pragma unroll
+                for (int q = 0; q < 21; ++q) h[q] = hs[q];
+                if (lane == 0)
+                    for (int q = 0; q < 21; ++q) a.H21[idx + (long)q * a.N] = h[q];
+                const bool spd = chol6(h);
+                if (lane == 0) {
+                    if (spd) {
+                        for (int q = 0; q < 21; ++q) L.Lp[q] = h[q];
+                        for (int c = 0; c < 6; ++c) L.Ld[c] = 1.0 / h[c * 6 - (c * (c - 1)) / 2];
+                        L.kind = 0;
+                    } else {
+                        int t = 0;
+                        for (int c = 0; c < 6; ++c)
+                            for (int r = c; r < 6; ++r) {
+                                L.A[r + c * 6] = L.A[c + r * 6] = (double)hs[t];
+                                ++t;
+                            }
+                        int kind;
+                        factor_lds<6>(L.A, L.work, L.piv, &kind);
+                        if (kind == 0) {  // the trial Cholesky in LDS succeeded after all
+                            for (int c = 0, t2 = 0; c < 6; ++c)
+                                for (int r = c; r < 6; ++r) L.Lp[t2++] = L.A[r + c * 6];
+                            for (int c = 0; c < 6; ++c) L.Ld[c] = 1.0 / L.A[c + c * 6];
+                        }
+                        L.kind = kind;
+                    }
+                }
+            } else {
+                set_h2(L, hs[0], hs[1], hs[2], bottomf, a.mu);
+            }
+            __syncthreads();
+        }
+
+        // ---------------------------------------------------------- iterate
+        int it = 0, sum_bw2 = 0;
+        bool fsel = false;  // 2-DOF: f side taken from the last g region
+        unsigned selbits = ~0u;  // pixels of the current g region (bit k = pixel lane + 64 k)
+        // g tile resident in LDS: [tu0, tu0+tnx) x [tv0, tv0+tny) (1-based)
+        int tu0 = 0, tv0 = 0, tnx = 0, tny = 0;
+        bool have_tile = false;
+        const float cf = isfinite(meanf) ? (float)meanf : 0.f;
+        double uo_u = 0.0, uo_v = 0.0, vd_u = 0.0, vd_v = 0.0;  // 2-DOF penalty targets
+        if (MODE == 2) {
+            uo_u = a.UOld[2 * idx];
+            uo_v = a.UOld[2 * idx + 1];
+            vd_u = a.vdual[2 * idx];
+            vd_v = a.vdual[2 * idx + 1];
+        }
+        while (state == 0) {
+            const double tol = a.tol;
+            if (!(it <= a.max_it && normNew > tol && normAbs > tol)) {
+                state = 1;
+                break;
+            }
+            ++it;
+            if (MODE == 6 && quad && it > 1 && count > 0) {
+                // funICGNQuadtree.m:139-145 (see icgn.hip)
+                const double wn =
+                    2.0 * ceil(sqrt((double)(w0 + 1) * (w0 + 1) / (double)sum_bw2) * wx / 2.0);
+                if (wn != (double)wx) {
+                    if (!isfinite(wn) || x0 - wn / 2 < 1 || x0 + wn / 2 > fr.H ||
+                        y0 - wn / 2 < 1 || y0 + wn / 2 > fr.W) {
+                        state = 2;
+                        status = ST_ERROR;
+                        break;
+                    }
+                    const double h = wn / 2;
+                    bool out = false;
+                    for (int q = 0; q < 4; ++q) {
+                        const double dx = (q & 1) ? h : -h, dy = (q & 2) ? h : -h;
+                        const double u = (1 + P[0]) * dx + P[2] * dy + (x0 + P[4]);
+                        const double v = P[1] * dx + (1 + P[3]) * dy + (y0 + P[5]);
+                        if (u < 3 || u > fr.H - 2 || v < 3 || v > fr.W - 2) out = true;
+                    }
+                    if (out) {
+                        normNew = 1e6;
+                        state = 1;
+                    } else {
+                        state = 2;
+                        status = ST_ERROR;
+                    }
+                    break;
+                }
+            }
+            bool nanP = false;
+            for (int k = 0; k < 6; ++k) nanP |= isnan(P[k]);
+            if (nanP) {
+                if (MODE == 2 || quad) {
+                    state = 2;
+                    status = ST_ERROR;
+                } else {
+                    normNew = it == 1 ? 0.0 : NAN;
+                    normAbs = NAN;
+                    state = 1;
+                }
+                break;
+            }
+            double umin = INFINITY, umax = -INFINITY, vmin = INFINITY, vmax = -INFINITY;
+            bool out = false;
+            for (int q = 0; q < 4; ++q) {
+                const double dx = (q & 1) ? hx : -hx, dy = (q & 2) ? hy : -hy;
+                const double u = (1 + P[0]) * dx + P[2] * dy + (x0 + P[4]);
+                const double v = P[1] * dx + (1 + P[3]) * dy + (y0 + P[5]);
+                if (u < 3 || u > fr.H - 2 || v < 3 || v > fr.W - 2) out = true;
+                umin = fmin(umin, u);
+                umax = fmax(umax, u);
+                vmin = fmin(vmin, v);
+                vmax = fmax(vmax, v);
+            }
+            if (out) {
+                normNew = 1e6;
+                state = 1;
+                break;
+            }
+            // support needed this iteration (taps floor-1..floor+2, one pixel
+            // of slack for the FP32 local coordinates)
+            const int nu0 = (int)floor(umin) - 2, nv0 = (int)floor(vmin) - 2;
+            const int nu1 = (int)floor(umax) + 3, nv1 = (int)floor(vmax) + 3;
+            bool use_tile = true;
+            bool reload = !have_tile || nu0 < tu0 || nv0 < tv0 || nu1 >= tu0 + tnx ||
+                          nv1 >= tv0 + tny;
+            if (reload) {
+                // reload with a margin so that the next iterations' small
+                // updates stay inside the resident tile
+                int m = 4;
+                while (m > 0 && (nu1 - nu0 + 1 + 2 * m) * (nv1 - nv0 + 1 + 2 * m) > a.tile_cap) --m;
+                tu0 = nu0 - m;
+                tv0 = nv0 - m;
+                tnx = nu1 - nu0 + 1 + 2 * m;
+                tny = nv1 - nv0 + 1 + 2 * m;
+                use_tile = tnx * tny <= a.tile_cap;
+                have_tile = use_tile;
+                if (!use_tile) {  // gather from HBM; origin as for a tile
+                    tu0 = nu0;
+                    tv0 = nv0;
+                    tnx = nu1 - nu0 + 1;
+                    tny = nv1 - nv0 + 1;
+                }
+            }
+            const float a11 = (float)(1 + P[0]), a12 = (float)P[2], a21 = (float)P[1],
+                        a22 = (float)(1 + P[3]);
+            const float cu = (float)((x0 + P[4]) - tu0), cv = (float)((y0 + P[5]) - tv0);
+            if (use_tile && reload) {
+                __syncthreads();  // last iteration's gathers are done with the old tile
+                // flat over the tile, 8 independent loads in flight per lane
+                const int total = tnx * tny;
+                const float rcpn = 1.0f / tnx;
+                for (int t0 = 0; t0 < total; t0 += 64 * 8) {
+                    float v[8];
+                    int o[8];
+#pragma unroll
+                    for (int q = 0; q < 8; ++q) {
+                        const int t = t0 + lane + 64 * q;
+                        o[q] = t;
+                        v[q] = 0.f;
+                        if (t < total) {
+                            const int sj = (int)(((float)t + 0.5f) * rcpn), ri = t - sj * tnx;
+                            const int y = min(max(tv0 + sj, 1), fr.W), x = min(max(tu0 + ri, 1), fr.H);
+                            v[q] = gld(fr.g, (size_t)((x - 1) + (y - 1) * fr.H));
+                        }
+                    }
+#pragma unroll
+                    for (int q = 0; q < 8; ++q)
+                        if (o[q] < total) tile[o[q]] = v[q];
+                }
+                __syncthreads();
+            }
+            // Keys bicubic of pixel (i, j) of the warped window, from the
+            // tile or, if the warped support outgrew it, from HBM
+            auto gather = [&](int i, int j) {
+                const float dx = (float)i - hx, dy = (float)j - hy;
+                const float lu = fmaf(a11, dx, fmaf(a12, dy, cu));
+                const float lv = fmaf(a21, dx, fmaf(a22, dy, cv));
+                const float fu = floorf(lu), fvv = floorf(lv);
+                float wu[4], wv[4];
+                keys_w(lu - fu, wu);
+                keys_w(lv - fvv, wv);
+                const int iu = (int)fu - 1, iv = (int)fvv - 1;
+                float g = 0.f;
+                if (use_tile) {
+                    const float *b = tile + iu + iv * tnx;
+#pragma unroll
+                    for (int q = 0; q < 4; ++q) {
+                        const float *c4 = b + q * tnx;
+                        g = fmaf(wv[q], fmaf(wu[0], c4[0], fmaf(wu[1], c4[1], fmaf(wu[2], c4[2], wu[3] * c4[3]))), g);
+                    }
+                } else {
+#pragma unroll
+                    for (int q = 0; q < 4; ++q) {
+                        const int y = min(max(tv0 + iv + q, 1), fr.W);
+                        const float *col = fr.g + (size_t)(y - 1) * fr.H;
+                        float s = 0.f;
+#pragma unroll
+                        for (int r = 0; r < 4; ++r) s = fmaf(wu[r], gld(col, (size_t)(min(max(tu0 + iu + r, 1), fr.H) - 1)), s);
+                        g = fmaf(wv[q], s, g);
+                    }
+                }
+                return g;
+            };
+            float gr[PPT];
+            float acc[6 + NJ];
+#pragma unroll
+            for (int q = 0; q < 6 + NJ; ++q) acc[q] = 0.f;
+#pragma unroll
+            for (int k = 0; k < PPT; ++k) {
+                const int p = lane + 64 * k;
+                gr[k] = 0.f;
+                if (p < npix) {
+                    const int j = (int)(((float)p + 0.5f) * rnx), i = p - j * nx;
+                    const float g = gather(i, j);
+                    gr[k] = g;
+                    const float d = g - cf;
+                    const bool nz = !quad || fabsf(g) > 1e-10f;
+                    acc[0] += nz ? 1.f : 0.f;
+                    acc[1] += nz ? d : 0.f;
+                    acc[2] += nz ? d * d : 0.f;
+                    acc[3] += g == 0.f ? 1.f : 0.f;
+                    acc[4] += isnan(g) ? 1.f : 0.f;
+                    const bool s0 = (s0bits >> k) & 1u;
+                    const float u = s0 ? fxr[k] * d : 0.f, v = s0 ? fyr[k] * d : 0.f;
+                    if (MODE == 6) {
+                        const float dx = (float)i - hx, dy = (float)j - hy;
+                        acc[6] = fmaf(dx, u, acc[6]);
+                        acc[7] = fmaf(dx, v, acc[7]);
+                        acc[8] = fmaf(dy, u, acc[8]);
+                        acc[9] = fmaf(dy, v, acc[9]);
+                        acc[10] += u;
+                        acc[11] += v;
+                    } else {
+                        acc[6] += u;
+                        acc[7] += v;
+                    }
+                }
+            }
+#pragma unroll
+            for (int q = 0; q < 6 + NJ; ++q) acc[q] = wave_sum_dpp(acc[q]);
+            if (quad && acc[4] > 0) {
+                state = 2;
+                status = ST_ERROR;  // logical(NaN) throws
+                break;
+            }
+            const bool masked_it = quad && (count > 0 || acc[3] > 0);
+            if (masked_it) {
+                // region of nonzero g through the seed: 8-connected for the
+                // 6-DOF solver (funICGNQuadtree.m:171), 4-connected for
+                // Subpb1 (funICGN_Subpb1.m:160); if that is the whole window,
+                // the window mask's 8-connected region (:175-176 / :164-165)
+                L.colw[lane] = 0ull;
+                __syncthreads();
+#pragma unroll
+                for (int k = 0; k < PPT; ++k) {
+                    const int p = lane + 64 * k;
+                    if (p < npix && gr[k] != 0.f) {
+                        const int j = (int)(((float)p + 0.5f) * rnx), i = p - j * nx;
+                        atomicOr(&L.colw[j], 1ull << i);
+                    }
+                }
+                __syncthreads();
+                const unsigned long long full = nx >= 64 ? ~0ull : ((1ull << nx) - 1ull);
+                unsigned long long sw = wave_flood(L.colw[lane], ny, seed_r, seed_c, MODE == 6 ? 8 : 4);
+                const bool all = !wave_any(lane < ny && sw != full);
+                if (all) {
+                    if (!masked_f) {
+                        L.colw[lane] = 0ull;
+                        __syncthreads();
+#pragma unroll
+                        for (int k = 0; k < PPT; ++k) {
+                            const int p = lane + 64 * k;
+                            if (p < npix) {
+                                const int j = (int)(((float)p + 0.5f) * rnx), i = p - j * nx;
+                                if (fidx(fr.mask, fr.H, xa + i, ya + j) != 0.f)
+                                    atomicOr(&L.colw[j], 1ull << i);
+                            }
+                        }
+                        __syncthreads();
+                        sw = wave_flood(L.colw[lane], ny, seed_r, seed_c, 8);
+                    } else {
+                        sw = wave_flood(L.tmw[lane], ny, seed_r, seed_c, 8);
+                    }
+                }
+                if (lane >= ny) sw = 0ull;
+                __syncthreads();
+                L.colw[lane] = sw;
+                __syncthreads();
+                sum_bw2 = wave_allsum((int)__popcll(sw));
+                selbits = 0u;
+#pragma unroll
+                for (int k = 0; k < PPT; ++k) {
+                    const int p = lane + 64 * k;
+                    if (p < npix) {
+                        const int j = (int)(((float)p + 0.5f) * rnx), i = p - j * nx;
+                        if ((L.colw[j] >> i) & 1ull) selbits |= 1u << k;
+                    }
+                }
+                if (MODE == 2) fsel = true;
+            }
+            const bool fs = MODE == 2 && fsel;
+            // the 2-DOF region rebuild needs f*mask per pixel; windows too large
+            // to keep it in registers take the literal path
+            bool literal = fs && !FM;
+            double b[6] = {0, 0, 0, 0, 0, 0};
+            if (!literal) {
+                double n = acc[0], S1 = acc[1], S2 = acc[2];
+                double T[6], SJv[6], FJv[6];
+#pragma unroll
+                for (int q = 0; q < NJ; ++q) {
+                    T[q] = acc[6 + q];
+                    SJv[q] = L.SJ[q];
+                    FJv[q] = L.FJ[q];
+                }
+                if (masked_it || fs) {
+                    // the same sums with g (and, for Subpb1, f and grad f) on the region
+                    constexpr int NR = MODE == 6 ? 9 : 16;
+                    float r[NR];
+#pragma unroll
+                    for (int q = 0; q < NR; ++q) r[q] = 0.f;
+#pragma unroll
+                    for (int k = 0; k < PPT; ++k) {
+                        const int p = lane + 64 * k;
+                        if (p < npix) {
+                            const bool sg = !masked_it || ((selbits >> k) & 1u);
+                            const float g = sg ? gr[k] : 0.f;
+                            const float d = g - cf;
+                            const bool nz = fabsf(g) > 1e-10f;
+                            r[0] += nz ? 1.f : 0.f;
+                            r[1] += nz ? d : 0.f;
+                            r[2] += nz ? d * d : 0.f;
+                            if (MODE == 6) {
+                                const int j = (int)(((float)p + 0.5f) * rnx), i = p - j * nx;
+                                const bool s0 = (s0bits >> k) & 1u;
+                                const float u = s0 ? fxr[k] * d : 0.f, v = s0 ? fyr[k] * d : 0.f;
+                                const float dx = (float)i - hx, dy = (float)j - hy;
+                                r[3] = fmaf(dx, u, r[3]);
+                                r[4] = fmaf(dx, v, r[4]);
+                                r[5] = fmaf(dy, u, r[5]);
+                                r[6] = fmaf(dy, v, r[6]);
+                                r[7] += u;
+                                r[8] += v;
+                            } else {
+                                const bool sf = (selbits >> k) & 1u;
+                                const float fx = sf ? fxr[k] : 0.f, fy = sf ? fyr[k] : 0.f;
+                                const float tf = sf ? fmr[FM ? k : 0] : 0.f;
+                                const float df = tf - cf;
+                                const bool fnz = fabsf(tf) > 1e-10f;
+                                r[3] += fx * d;
+                                r[4] += fy * d;
+                                r[5] += fnz ? 1.f : 0.f;
+                                r[6] += fnz ? df : 0.f;
+                                r[7] += fnz ? df * df : 0.f;
+                                r[8] += fabsf(tf);
+                                r[9] += fx * fx;
+                                r[10] += fx * fy;
+                                r[11] += fy * fy;
+                                r[12] += fx;
+                                r[13] += fy;
+                                r[14] += fx * df;
+                                r[15] += fy * df;
+                            }
+                        }
+                    }
+                    n = wave_sum_dpp_f64((double)r[0]);
+                    S1 = wave_sum_dpp_f64((double)r[1]);
+                    S2 = wave_sum_dpp_f64((double)r[2]);
+                    if (MODE == 6) {
+#pragma unroll
+                        for (int q = 0; q < 6; ++q) T[q] = wave_sum_dpp_f64((double)r[3 + q]);
+                    } else {
+                        T[0] = wave_sum_dpp_f64((double)r[3]);
+                        T[1] = wave_sum_dpp_f64((double)r[4]);
+                        // Subpb1 rebuilds f, grad f, H and the f statistics on
+                        // the region (funICGN_Subpb1.m:171-190); they persist
+                        const double nf = wave_sum_dpp_f64((double)r[5]);
+                        const double F1 = wave_sum_dpp_f64((double)r[6]);
+                        const double F2 = wave_sum_dpp_f64((double)r[7]);
+                        sabs = wave_sum_dpp_f64((double)r[8]);
+                        const double h11 = wave_sum_dpp(r[9]), h12 = wave_sum_dpp(r[10]),
+                                     h22 = wave_sum_dpp(r[11]);
+                        meanf = nf > 0 ? (double)cf + F1 / nf : NAN;
+                        bottomf = nf >= 2 ? sqrt(fmax(F2 - F1 * F1 / nf, 0.0)) : (nf == 1 ? 0.0 : NAN);
+                        const double dmf = (double)cf - meanf;
+                        for (int q = 0; q < 2; ++q) {
+                            SJv[q] = wave_sum_dpp_f64((double)r[12 + q]);
+                            FJv[q] = wave_sum_dpp_f64((double)r[14 + q]) + dmf * SJv[q];
+                        }
+                        __syncthreads();
+                        set_h2(L, h11, h12, h22, bottomf, a.mu);
+                        __syncthreads();
+                    }
+                }
+                const double gbar = (double)cf + S1 / n;
+                const double bg = n >= 2 ? sqrt(fmax(S2 - S1 * S1 / n, 0.0)) : (n == 1 ? 0.0 : NAN);
+                const double bf = bottomf;
+                if (!(bf > 0) || !(bg > 0) || !isfinite(bf) || !isfinite(bg) || !isfinite(gbar)) {
+                    literal = true;
+                } else {
+#pragma unroll
+                    for (int q = 0; q < NJ; ++q)
+                        b[q] = FJv[q] - (bf / bg) * (T[q] - (gbar - (double)cf) * SJv[q]);
+                }
+            }
+            if (literal) {
+                // ---- degenerate statistics: the reference's residual, FP64
+                const bool fsl = MODE == 2 && fsel;
+                // g of pixel k under the current region
+                auto gsel = [&](int i, int j) {
+                    const float g = gather(i, j);
+                    return (masked_it && !((L.colw[j] >> i) & 1ull)) ? 0.f : g;
+                };
+                // pass A: counts and first moments (2-DOF with the f side on
+                // the last region also rebuilds f, grad f and H, :171-190)
+                double v[7] = {0, 0, 0, 0, 0, 0, 0};
+#pragma unroll 3
+                for (int k = 0; k < PPT; ++k) {
+                    const int p = lane + 64 * k;
+                    if (p < npix) {
+                        const int j = (int)(((float)p + 0.5f) * rnx), i = p - j * nx;
+                        const double g = gsel(i, j);
+                        const bool nz = !quad || fabs(g) > 1e-10;
+                        v[0] += nz ? 1.0 : 0.0;
+                        v[1] += nz ? g : 0.0;
+                        if (fsl) {
+                            const bool s = (L.colw[j] >> i) & 1ull;
+                            const int x = xa + i, y = ya + j;
+                            const size_t gi = (size_t)((x - 4) + (y - 4) * Hd);
+                            const double fx = s ? (double)gld(fr.dfdx, gi) : 0.0;
+                            const double fy = s ? (double)gld(fr.dfdy, gi) : 0.0;
+                            const double fv = s ? (double)fidx(fr.f, fr.H, x, y) * (double)fidx(fr.mask, fr.H, x, y) : 0.0;
+                            v[2] += fabs(fv) > 1e-10 ? 1.0 : 0.0;
+                            v[3] += fabs(fv) > 1e-10 ? fv : 0.0;
+                            v[4] += fx * fx;
+                            v[5] += fx * fy;
+                            v[6] += fy * fy;
+                            // |f| for the featureless test
+                            v[2] += 0.0;
+                        }
+                    }
+                }
+                const double ng = wave_sum_dpp_f64(v[0]);
+                const double mg = ng > 0 ? wave_sum_dpp_f64(v[1]) / ng : NAN;
+                double mf = meanf, bfx = bottomf, nf = 0, h11 = 0, h12 = 0, h22 = 0;
+                if (fsl) {
+                    nf = wave_sum_dpp_f64(v[2]);
+                    mf = nf > 0 ? wave_sum_dpp_f64(v[3]) / nf : NAN;
+                    h11 = wave_sum_dpp_f64(v[4]);
+                    h12 = wave_sum_dpp_f64(v[5]);
+                    h22 = wave_sum_dpp_f64(v[6]);
+                }
+                // pass A2: second moments about the means
+                double s2g = 0.0, s2f = 0.0, sa = 0.0;
+#pragma unroll 3
+                for (int k = 0; k < PPT; ++k) {
+                    const int p = lane + 64 * k;
+                    if (p < npix) {
+                        const int j = (int)(((float)p + 0.5f) * rnx), i = p - j * nx;
+                        const double g = gsel(i, j);
+                        if (!quad || fabs(g) > 1e-10) s2g += (g - mg) * (g - mg);
+                        if (fsl && ((L.colw[j] >> i) & 1ull)) {
+                            const double fv = (double)fidx(fr.f, fr.H, xa + i, ya + j) *
+                                              (double)fidx(fr.mask, fr.H, xa + i, ya + j);
+                            if (fabs(fv) > 1e-10) s2f += (fv - mf) * (fv - mf);
+                            sa += fabs(fv);
+                        }
+                    }
+                }
+                s2g = wave_sum_dpp_f64(s2g);
+                const double bg = ng >= 2 ? sqrt(s2g) : (ng == 1 ? 0.0 : NAN);
+                if (fsl) {
+                    s2f = wave_sum_dpp_f64(s2f);
+                    sabs = wave_sum_dpp_f64(sa);
+                    bfx = nf >= 2 ? sqrt(s2f) : (nf == 1 ? 0.0 : NAN);
+                    meanf = mf;
+                    bottomf = bfx;
+                    __syncthreads();
+                    set_h2(L, h11, h12, h22, bfx, a.mu);
+                    __syncthreads();
+                }
+                // pass B: b = bottomf * sum(J r) over every pixel (:362-371)
+                double vb[6] = {0, 0, 0, 0, 0, 0};
+#pragma unroll 3
+                for (int k = 0; k < PPT; ++k) {
+                    const int p = lane + 64 * k;
+                    if (p < npix) {
+                        const int j = (int)(((float)p + 0.5f) * rnx), i = p - j * nx;
+                        const int x = xa + i, y = ya + j;
+                        double fv, fx, fy;
+                        if (fsl) {
+                            const bool s = (L.colw[j] >> i) & 1ull;
+                            const size_t gi = (size_t)((x - 4) + (y - 4) * Hd);
+                            fx = s ? (double)gld(fr.dfdx, gi) : 0.0;
+                            fy = s ? (double)gld(fr.dfdy, gi) : 0.0;
+                            fv = s ? (double)fidx(fr.f, fr.H, x, y) * (double)fidx(fr.mask, fr.H, x, y) : 0.0;
+                        } else {
+                            const bool s = !masked_f || ((L.sel0w[j] >> i) & 1ull);
+                            const size_t gi = (size_t)((x - 4) + (y - 4) * Hd);
+                            fx = s ? (double)gld(fr.dfdx, gi) : 0.0;
+                            fy = s ? (double)gld(fr.dfdy, gi) : 0.0;
+                            fv = fidx(fr.f, fr.H, x, y);
+                            if (quad) fv *= (double)fidx(fr.mask, fr.H, x, y);
+                            if (!s) fv = 0.0;
+                        }
+                        const double r = (fv - mf) / bfx - ((double)gsel(i, j) - mg) / bg;
+                        if (MODE == 6) {
+                            const double dx = i - (double)hx, dy = j - (double)hy;
+                            vb[0] += dx * fx * r;
+                            vb[1] += dx * fy * r;
+                            vb[2] += dy * fx * r;
+                            vb[3] += dy * fy * r;
+                            vb[4] += fx * r;
+                            vb[5] += fy * r;
+                        } else {
+                            vb[0] += fx * r;
+                            vb[1] += fy * r;
+                        }
+                    }
+                }
+#pragma unroll
+                for (int q = 0; q < NJ; ++q) b[q] = bfx * wave_sum_dpp_f64(vb[q]);
+            }
+            // norm test, Gauss-Newton step, inverse-compositional update
+            double nb2 = 0.0;
+            if (MODE == 6) {
+#pragma unroll
+                for (int q = 0; q < 6; ++q) nb2 += b[q] * b[q];
+            } else {
+                // :379 tempb = 2 b / bottomf^2 + mu (P - UOld - vdual)
+                const double s2 = 2.0 / (bottomf * bottomf);
+                b[0] = b[0] * s2 + a.mu * ((P[4] - uo_u) - vd_u);
+                b[1] = b[1] * s2 + a.mu * ((P[5] - uo_v) - vd_v);
+                nb2 = b[0] * b[0] + b[1] * b[1];
+            }
+            double nn = sqrt(nb2);
+            normAbs = nn;
+            if (it == 1) normInit = nn;
+            nn = normInit > tol ? nn / normInit : 0.0;
+            normNew = nn;
+            if (nn < tol || normAbs < tol) {
+                state = 1;
+                break;
+            }
+            double d[6] = {0, 0, 0, 0, 0, 0};
+            if (MODE == 6) {
+                solve6_lds(L, b, d);
+            } else {
+                d[4] = -(L.H2i[0] * b[0] + L.H2i[2] * b[1]);
+                d[5] = -(L.H2i[1] * b[0] + L.H2i[3] * b[1]);
+            }
+            if (!ic_compose(P, d)) {
+                state = 1;
+                break;
+            }
+        }
+
+        // ---------------------------------------------------------- outputs
+        if (state == 1) {
+            const double tol = a.tol;
+            int st = it;
+            if (!(normNew < tol || normAbs < tol)) st = a.max_it + 1;
+            if (isnan(normNew)) st = a.max_it + 1;
+            if (quad && sabs < 1e-6) st = a.max_it + 3;
+            status = st;
+        }
+        if (lane == 0) {
+            if (state == 3) {
+                a.status[idx] = ST_DEFER;
+            } else {
+                double Pout[6];
+                for (int k = 0; k < 6; ++k) Pout[k] = P[k];
+                if (status == ST_ERROR)
+                    for (int k = 0; k < 6; ++k) Pout[k] = NAN;
+                if (status == ST_MASKED && MODE == 6) {
+                    Pout[0] = Pout[1] = Pout[2] = Pout[3] = 0.0;
+                    Pout[4] = a.U0[2 * idx];
+                    Pout[5] = a.U0[2 * idx + 1];
+                }
+                a.U[2 * idx] = Pout[4];
+                a.U[2 * idx + 1] = Pout[5];
+                if (MODE == 6) {
+                    for (int k = 0; k < 4; ++k) a.F[4 * idx + k] = Pout[k];
+                    if (status == ST_ERROR || status == ST_MASKED)
+                        for (int k = 0; k < 21; ++k) a.H21[idx + (long)k * a.N] = 0.0;
+                }
+                a.status[idx] = status;
+            }
+        }
+        __syncthreads();
+        if (!a.work_counter) break;
+    }
+}
+
+template <int PPT, int MODE>
+hipError_t launch_wave(const IcgnArgs &a, int grid, hipStream_t s)
+{
+    const size_t lds = sizeof(float) * (size_t)a.tile_cap;
+    hipLaunchKernelGGL((icgn_wave_kernel<PPT, MODE>), dim3(grid), dim3(64), lds, s, a);
+    return hipGetLastError();
+}
+
+}  // namespace
+
+int icgn_wave_ppt(int npix_cap)
+{
+    if (npix_cap <= 64 * 5) return 5;
+    if (npix_cap <= 64 * 9) return 9;
+    if (npix_cap <= 64 * 18) return 18;
+    return 0;  // too large for the wave kernel
+}
+
+hipError_t icgn_wave_launch(const IcgnArgs &a, int mode, int ppt, int grid, hipStream_t s)
+{
+    if (mode == 6) {
+        switch (ppt) {
+        case 5: return launch_wave<5, 6>(a, grid, s);
+        case 9: return launch_wave<9, 6>(a, grid, s);
+        case 18: return launch_wave<18, 6>(a, grid, s);
+        }
+    } else {
+        switch (ppt) {
+        case 5: return launch_wave<5, 2>(a, grid, s);
+        case 9: return launch_wave<9, 2>(a, grid, s);
+        case 18: return launch_wave<18, 2>(a, grid, s);
+        }
+    }
+    return hipErrorInvalidValue;
+}
+
+}  // namespace staq

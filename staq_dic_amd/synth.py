@@ -275,7 +275,7 @@ def plain_grid_nodes(H, W, winsize, step, margin=None):
     return np.stack([X.ravel(order="F"), Y.ravel(order="F")], axis=1).astype(float)
 
 
-def quadtree_nodes(H, W, hole, leaf_max=32, leaf_min=8, band=(96, 40), margin=40):
+def quadtree_nodes(H, W, hole, leaf_max=32, leaf_min=8, band=(450, 150), margin=40):
     """Quadtree subset list refined at a circular hole edge.
 
     Cells of ``leaf_max`` px tile [margin, size - margin]; a cell within

@@ -163,6 +163,9 @@ int staq_batch_icgn2(staq_batch *b, const staq_params *p);
 int staq_batch_icgn6_per_subset(staq_batch *b, const staq_params *p, double *kernel_ms_sum);
 /* Copy outputs to host (any pointer may be NULL). */
 int staq_batch_get_output(staq_batch *b, double *U, double *F, double *Hpar, int *status);
+/* Reorder each window bin longest-first, predicted from the previous step's
+ * iteration counts and window sizes (scheduling only; results are unchanged). */
+int staq_batch_reorder(staq_batch *b);
 /* Feed the 6-DOF outputs (U, F) of this batch to its 2-DOF inputs (UOld, FOld)
  * on the device, vdual = 0 — the first AL step with an identity global step. */
 int staq_batch_chain_outputs(staq_batch *b);

@@ -67,6 +67,7 @@ SIGNATURES = {
     "staq_batch_icgn6_per_subset": (C.c_int, [_vp, C.POINTER(StaqParams), _dp]),
     "staq_batch_get_output": (C.c_int, [_vp, _dp, _dp, _dp, _ip]),
     "staq_batch_chain_outputs": (C.c_int, [_vp]),
+    "staq_batch_reorder": (C.c_int, [_vp]),
     "staq_batch_autotune": (C.c_int, [_vp, C.POINTER(StaqParams), C.c_int]),
     "staq_batch_graph_capture": (C.c_int, [_vp, C.POINTER(StaqParams), C.c_int]),
     "staq_batch_graph_launch": (C.c_int, [_vp]),
@@ -229,6 +230,9 @@ class Batch:
                                                           C.byref(ms) if timed else None),
                        "staq_batch_icgn6_per_subset")
         return ms.value
+
+    def reorder(self):
+        self.ctx.check(self.L.staq_batch_reorder(self.h), "staq_batch_reorder")
 
     def chain(self):
         self.ctx.check(self.L.staq_batch_chain_outputs(self.h), "staq_batch_chain_outputs")

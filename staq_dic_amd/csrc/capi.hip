@@ -32,13 +32,6 @@ __global__ void reset_ints_kernel(int *p, int n)
     if (i < n) p[i] = 0;
 }
 
-__global__ void defer_flag_kernel(const int *status, const int *order, long item0, long n,
-                                  int *flag)
-{
-    const long i = blockIdx.x * (long)blockDim.x + threadIdx.x;
-    if (i < n && status[order[item0 + i]] == ST_DEFER) *flag = 1;
-}
-
 __global__ void chain_kernel(const double *U, const double *F, double *UOld, double *FOld,
                              double *vdual, long N)
 {
@@ -127,6 +120,10 @@ struct staq_batch {
     hipGraph_t graph = nullptr;
     hipGraphExec_t gexec = nullptr;
     int graph_mode = 0;
+    // bins run concurrently: one side stream per extra bin, forked from and
+    // joined into the context stream with events
+    std::vector<hipStream_t> side;
+    std::vector<hipEvent_t> ev;  // [0] fork, [1..] joins
 };
 
 static int fail(staq_ctx *ctx, int code, const std::string &msg)
@@ -562,6 +559,14 @@ int staq_batch_create(staq_ctx *ctx, staq_batch **out, long N, const double *coo
     b->fb_npix_cap = (fbx + 1) * (fby + 1);
     b->fb_tile_cap = tile_side(fbx + 1) * tile_side(fby + 1);
     const size_t nb = b->bins.size();
+    for (size_t k = 1; k < nb; ++k) {
+        hipStream_t st = nullptr;
+        if (hipStreamCreateWithFlags(&st, hipStreamNonBlocking) == hipSuccess) b->side.push_back(st);
+    }
+    for (size_t k = 0; k < nb; ++k) {
+        hipEvent_t e = nullptr;
+        if (hipEventCreateWithFlags(&e, hipEventDisableTiming) == hipSuccess) b->ev.push_back(e);
+    }
     int rc = STAQ_OK;
     auto chk = [&](hipError_t e, const char *w) {
         if (e != hipSuccess && rc == STAQ_OK) rc = hipfail(ctx, e, w);
@@ -608,6 +613,11 @@ void staq_batch_destroy(staq_batch *b)
     hipStreamSynchronize(b->ctx->stream);
     if (b->gexec) hipGraphExecDestroy(b->gexec);
     if (b->graph) hipGraphDestroy(b->graph);
+    for (hipStream_t st : b->side) {
+        hipStreamSynchronize(st);
+        hipStreamDestroy(st);
+    }
+    for (hipEvent_t e : b->ev) hipEventDestroy(e);
     DevBuf *all[] = {&b->coords, &b->U0, &b->UOld, &b->FOld, &b->vdual, &b->U, &b->F,
                      &b->H, &b->status, &b->wx, &b->wy, &b->slot, &b->order, &b->ints};
     for (DevBuf *d : all) d->release();
@@ -715,22 +725,29 @@ static int occupancy_grid(int bs, size_t lds, long n)
     return (int)std::min<long>(n, g);
 }
 
-// Enqueue one local step (mode 6 or 2) on stream s; bin_only >= 0 launches a
-// single bin (autotuning) without reset or fallback.
+// Enqueue one local step (mode 6 or 2) on stream s: reset of the queue heads
+// and defer flag, the window bins (concurrently when side streams exist), and
+// the fallback for windows that outgrew their bin.  bin_only >= 0 launches a
+// single bin on s (autotuning), without fallback.
 static int enqueue_step(staq_batch *b, const staq_params *p, int mode, hipStream_t s,
                         int bin_only = -1, int bs_override = -1)
 {
     staq_ctx *ctx = b->ctx;
     const int nb = (int)b->bins.size();
     int *ints = b->ints.as<int>();  // [0, nb) bin queue heads, nb fallback head, nb+1 defer flag
-    if (bin_only < 0)
-        hipLaunchKernelGGL(reset_ints_kernel, dim3(1), dim3(256), 0, s, ints, nb + 2);
-    else
-        hipLaunchKernelGGL(reset_ints_kernel, dim3(1), dim3(256), 0, s, ints + bin_only, 1);
+    hipLaunchKernelGGL(reset_ints_kernel, dim3(1), dim3(256), 0, s, ints, nb + 2);
     HIPCHK(ctx, hipGetLastError());
     const int mi = mode == 6 ? 0 : 1;
+    const bool fork = bin_only < 0 && nb > 1 && (int)b->side.size() == nb - 1 &&
+                      (int)b->ev.size() == nb;
+    if (fork) HIPCHK(ctx, hipEventRecord(b->ev[0], s));
     for (int k = 0; k < nb; ++k) {
         if (bin_only >= 0 && k != bin_only) continue;
+        hipStream_t sk = s;
+        if (fork && k > 0) {
+            sk = b->side[k - 1];
+            HIPCHK(ctx, hipStreamWaitEvent(sk, b->ev[0], 0));
+        }
         const Bin &bin = b->bins[k];
         IcgnArgs a = base_args(b, p);
         a.item0 = bin.item0;
@@ -738,6 +755,7 @@ static int enqueue_step(staq_batch *b, const staq_params *p, int mode, hipStream
         a.npix_cap = bin.npix_cap;
         a.tile_cap = bin.tile_cap;
         a.defer = 1;
+        a.defer_out = ints + nb + 1;
         int bs = bs_override >= 0 ? bs_override : (p->block_size ? p->block_size : bin.bs[mi]);
         if (bs == 0 && !bin.ppt) bs = default_bs(bin.npix_cap);
         int grid = (int)bin.n;
@@ -748,23 +766,23 @@ static int enqueue_step(staq_batch *b, const staq_params *p, int mode, hipStream
                 a.work_counter = ints + k;
                 grid = occupancy_grid(64, sizeof(float) * a.tile_cap + 2048, bin.n);
             }
-            HIPCHK(ctx, icgn_wave_launch(a, mode, bin.ppt, grid, s));
+            HIPCHK(ctx, icgn_wave_launch(a, mode, bin.ppt, grid, sk));
         } else {
             if (p->persistent) {
                 a.work_counter = ints + k;
                 grid = occupancy_grid(bs, icgn_lds_bytes(a.npix_cap, a.tile_cap), bin.n);
             }
-            HIPCHK(ctx, icgn_launch(a, mode, bs, grid, s));
+            HIPCHK(ctx, icgn_launch(a, mode, bs, grid, sk));
         }
-        if (bin_only < 0) {
-            const unsigned g2 = (unsigned)((bin.n + 255) / 256);
-            hipLaunchKernelGGL(defer_flag_kernel, dim3(g2), dim3(256), 0, s, b->status.as<int>(),
-                               b->order.as<int>(), bin.item0, bin.n, ints + nb + 1);
-            HIPCHK(ctx, hipGetLastError());
+        if (fork && k > 0) {
+            HIPCHK(ctx, hipEventRecord(b->ev[k], sk));
+            HIPCHK(ctx, hipStreamWaitEvent(s, b->ev[k], 0));
         }
     }
     if (bin_only < 0) {
-        // fallback over the whole batch for windows that outgrew their bin
+        // fallback over the whole batch for windows that outgrew their bin;
+        // launched unconditionally (graph-capturable), its workgroups leave at
+        // once when the defer flag is clear
         IcgnArgs a = base_args(b, p);
         a.item0 = 0;
         a.n_items = b->N;
@@ -773,12 +791,9 @@ static int enqueue_step(staq_batch *b, const staq_params *p, int mode, hipStream
         a.defer = 0;
         a.only_deferred = 1;
         a.work_counter = ints + nb;
-        const int bs = 256;
-        const int grid = occupancy_grid(bs, icgn_lds_bytes(a.npix_cap, a.tile_cap), b->N);
-        // The fallback grid is launched unconditionally (graph-capturable);
-        // blocks exit at once when the defer flag is clear.
-        a.n_items = b->N;
         a.defer_flag = ints + nb + 1;
+        const int bs = 256;
+        const int grid = (int)std::min<long>(b->N, 256);
         HIPCHK(ctx, icgn_launch(a, mode, bs, grid, s));
     }
     return STAQ_OK;
@@ -837,6 +852,33 @@ int staq_batch_icgn6_per_subset(staq_batch *b, const staq_params *p, double *ker
     return STAQ_OK;
 }
 
+int staq_batch_reorder(staq_batch *b)
+{
+    // Longest-processing-time-first order inside each bin, predicted from the
+    // last step: iterations used (101 for a subset that ran out of
+    // iterations) times window pixels.  Results do not depend on the order.
+    if (!b) return STAQ_EINVAL;
+    staq_ctx *ctx = b->ctx;
+    hipSetDevice(ctx->device);
+    const long N = b->N;
+    std::vector<int> st(N), order(N), wx(N), wy(N);
+    HIPCHK(ctx, hipMemcpy(st.data(), b->status.p, N * 4, hipMemcpyDeviceToHost));
+    HIPCHK(ctx, hipMemcpy(order.data(), b->order.p, N * 4, hipMemcpyDeviceToHost));
+    HIPCHK(ctx, hipMemcpy(wx.data(), b->wx.p, N * 4, hipMemcpyDeviceToHost));
+    HIPCHK(ctx, hipMemcpy(wy.data(), b->wy.p, N * 4, hipMemcpyDeviceToHost));
+    auto cost = [&](int i) {
+        const int s = st[i];
+        const double it = (s >= 1 && s <= 101) ? s : (s == ST_DEFER ? 50 : 1);
+        return it * (wx[i] + 1.0) * (wy[i] + 1.0);
+    };
+    for (const Bin &bin : b->bins)
+        std::stable_sort(order.begin() + bin.item0, order.begin() + bin.item0 + bin.n,
+                         [&](int x, int y) { return cost(x) > cost(y); });
+    // a captured graph reads the order array when it runs, so it stays valid
+    HIPCHK(ctx, hipMemcpy(b->order.p, order.data(), N * 4, hipMemcpyHostToDevice));
+    return STAQ_OK;
+}
+
 int staq_batch_chain_outputs(staq_batch *b)
 {
     if (!b) return STAQ_EINVAL;
@@ -851,6 +893,10 @@ int staq_batch_chain_outputs(staq_batch *b)
 
 int staq_batch_autotune(staq_batch *b, const staq_params *p, int mode)
 {
+    // Coordinate descent over the window bins: for each bin try the wavefront
+    // kernel and each workgroup size, timing the whole step (all bins run
+    // concurrently and the fallback catches deferred windows, so a bin's
+    // choice is judged by its effect on the step), keep the fastest.
     if (!b || !p || (mode != 6 && mode != 2)) return STAQ_EINVAL;
     staq_ctx *ctx = b->ctx;
     hipSetDevice(ctx->device);
@@ -860,7 +906,21 @@ int staq_batch_autotune(staq_batch *b, const staq_params *p, int mode)
     HIPCHK(ctx, hipEventCreate(&e1));
     const int cand[5] = {0, 64, 128, 256, 512};
     const int mi = mode == 6 ? 0 : 1;
+    staq_params q = *p;
+    q.block_size = 0;
     int rc = STAQ_OK;
+    auto time_step = [&](float *ms) {
+        float t[3];
+        for (int r = 0; r < 3 && rc == STAQ_OK; ++r) {
+            hipEventRecord(e0, s);
+            rc = enqueue_step(b, &q, mode, s);
+            hipEventRecord(e1, s);
+            hipEventSynchronize(e1);
+            hipEventElapsedTime(&t[r], e0, e1);
+        }
+        std::sort(t, t + 3);
+        *ms = t[1];
+    };
     for (int k = 0; k < (int)b->bins.size() && rc == STAQ_OK; ++k) {
         Bin &bin = b->bins[k];
         float best = 1e30f;
@@ -869,18 +929,12 @@ int staq_batch_autotune(staq_batch *b, const staq_params *p, int mode)
             if (c == 0 && !bin.ppt) continue;
             if (c == 512 && bin.npix_cap < 1500) continue;
             if (c == 64 && bin.npix_cap > 4500) continue;
-            float t[3];
-            for (int r = 0; r < 3 && rc == STAQ_OK; ++r) {
-                hipEventRecord(e0, s);
-                rc = enqueue_step(b, p, mode, s, k, c);
-                hipEventRecord(e1, s);
-                hipEventSynchronize(e1);
-                hipEventElapsedTime(&t[r], e0, e1);
-            }
+            bin.bs[mi] = c;
+            float ms = 0.f;
+            time_step(&ms);
             if (rc != STAQ_OK) break;
-            std::sort(t, t + 3);
-            if (t[1] < best) {
-                best = t[1];
+            if (ms < best) {
+                best = ms;
                 best_bs = c;
             }
         }

@@ -303,7 +303,7 @@ __device__ inline bool ic_compose(double P[6], const double d[6])
 {
     const double det = (1 + d[0]) * (1 + d[3]) - d[1] * d[2];
     if (!(det != 0)) return false;
-    const double id = 1.0 / det;
+    const double id = __builtin_amdgcn_rcp(det);
     const double q1 = (-d[0] - d[0] * d[3] + d[1] * d[2]) * id;
     const double q2 = -d[1] * id;
     const double q3 = -d[2] * id;

@@ -130,6 +130,8 @@ __global__ __launch_bounds__(BS) void icgn_kernel(IcgnArgs a)
     unsigned char *sel = sel0 + a.npix_cap;
     unsigned char *bwb = sel + a.npix_cap;
     const int tid = threadIdx.x;
+    // fallback launch with nothing deferred: leave before touching the queue
+    if (a.defer_flag && *(volatile const int *)a.defer_flag == 0) return;
 
     for (long pass = 0;; ++pass) {
         // ---------------------------------------------------------- pick work
@@ -144,7 +146,6 @@ __global__ __launch_bounds__(BS) void icgn_kernel(IcgnArgs a)
         __syncthreads();
         const long item = item_sh;
         if (item >= a.item0 + a.n_items) break;
-        if (a.defer_flag && *a.defer_flag == 0) break;
 
         // ------------------------------------------------------------ setup
         if (tid == 0) {
@@ -815,7 +816,10 @@ __global__ __launch_bounds__(BS) void icgn_kernel(IcgnArgs a)
         }
 
         // ---------------------------------------------------------- outputs
-        if (tid == 0 && c.state == 3) a.status[c.idx] = ST_DEFER;
+        if (tid == 0 && c.state == 3) {
+            a.status[c.idx] = ST_DEFER;
+            if (a.defer_out) atomicOr(a.defer_out, 1);
+        }
         if (tid == 0 && c.state <= 2) {
             int st = c.status;
             if (c.state == 1) {

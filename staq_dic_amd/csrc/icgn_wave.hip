@@ -307,7 +307,12 @@ __global__ __launch_bounds__(64) void icgn_wave_kernel(IcgnArgs a)
         constexpr bool FM = MODE == 2 && PPT <= 9;
         float fxr[PPT], fyr[PPT], fmr[FM ? PPT : 1];
         unsigned s0bits = ~0u;
-        double meanf = 0, bottomf = 0, sabs = 0, normNew = 1, normAbs = 1, normInit = 0;
+        // convergence bookkeeping on squared norms (funICGNQuadtree.m:373-386):
+        // rel2 = (|b| / |b_1|)^2 or 0, abs2 = |b|^2; a square root is taken
+        // nowhere on the iteration's critical path
+        double meanf = 0, bottomf = 0, sabs = 0, rel2 = 1, abs2 = 1, inv_init2 = 0;
+        bool init_ok = false;
+        const double tol2 = a.tol * a.tol;
         if (state == 0) {
             const float c0 = fidx(fr.f, fr.H, xa + nx / 2, ya + ny / 2) *
                              (quad ? fidx(fr.mask, fr.H, xa + nx / 2, ya + ny / 2) : 1.f);
@@ -433,8 +438,7 @@ __global__ __launch_bounds__(64) void icgn_wave_kernel(IcgnArgs a)
             vd_v = a.vdual[2 * idx + 1];
         }
         while (state == 0) {
-            const double tol = a.tol;
-            if (!(it <= a.max_it && normNew > tol && normAbs > tol)) {
+            if (!(it <= a.max_it && rel2 > tol2 && abs2 > tol2)) {
                 state = 1;
                 break;
             }
@@ -459,7 +463,7 @@ __global__ __launch_bounds__(64) void icgn_wave_kernel(IcgnArgs a)
                         if (u < 3 || u > fr.H - 2 || v < 3 || v > fr.W - 2) out = true;
                     }
                     if (out) {
-                        normNew = 1e6;
+                        rel2 = 1e12;  // normOfWNew = 1e6 (:156)
                         state = 1;
                     } else {
                         state = 2;
@@ -475,8 +479,8 @@ __global__ __launch_bounds__(64) void icgn_wave_kernel(IcgnArgs a)
                     state = 2;
                     status = ST_ERROR;
                 } else {
-                    normNew = it == 1 ? 0.0 : NAN;
-                    normAbs = NAN;
+                    rel2 = it == 1 ? 0.0 : NAN;
+                    abs2 = NAN;
                     state = 1;
                 }
                 break;
@@ -494,7 +498,7 @@ __global__ __launch_bounds__(64) void icgn_wave_kernel(IcgnArgs a)
                 vmax = fmax(vmax, v);
             }
             if (out) {
-                normNew = 1e6;
+                rel2 = 1e12;
                 state = 1;
                 break;
             }
@@ -583,49 +587,55 @@ __global__ __launch_bounds__(64) void icgn_wave_kernel(IcgnArgs a)
                 }
                 return g;
             };
+            // per-iteration sums: S1 = sum(g - c), S2 = sum((g - c)^2) over the
+            // nonzero g, T_q = sum J_q (g - c) over the selected pixels; the
+            // counts and the zero / NaN tests are wave votes
             float gr[PPT];
-            float acc[6 + NJ];
+            float acc[2 + NJ];
 #pragma unroll
-            for (int q = 0; q < 6 + NJ; ++q) acc[q] = 0.f;
+            for (int q = 0; q < 2 + NJ; ++q) acc[q] = 0.f;
+            int n_nz = 0;
+            bool any_zero = false, any_nan = false;
 #pragma unroll
             for (int k = 0; k < PPT; ++k) {
                 const int p = lane + 64 * k;
                 gr[k] = 0.f;
+                bool nz = false;
                 if (p < npix) {
                     const int j = (int)(((float)p + 0.5f) * rnx), i = p - j * nx;
                     const float g = gather(i, j);
                     gr[k] = g;
                     const float d = g - cf;
-                    const bool nz = !quad || fabsf(g) > 1e-10f;
-                    acc[0] += nz ? 1.f : 0.f;
-                    acc[1] += nz ? d : 0.f;
-                    acc[2] += nz ? d * d : 0.f;
-                    acc[3] += g == 0.f ? 1.f : 0.f;
-                    acc[4] += isnan(g) ? 1.f : 0.f;
+                    nz = !quad || fabsf(g) > 1e-10f;
+                    acc[0] += nz ? d : 0.f;
+                    acc[1] += nz ? d * d : 0.f;
+                    any_zero |= g == 0.f;
+                    any_nan |= isnan(g);
                     const bool s0 = (s0bits >> k) & 1u;
                     const float u = s0 ? fxr[k] * d : 0.f, v = s0 ? fyr[k] * d : 0.f;
                     if (MODE == 6) {
                         const float dx = (float)i - hx, dy = (float)j - hy;
-                        acc[6] = fmaf(dx, u, acc[6]);
-                        acc[7] = fmaf(dx, v, acc[7]);
-                        acc[8] = fmaf(dy, u, acc[8]);
-                        acc[9] = fmaf(dy, v, acc[9]);
-                        acc[10] += u;
-                        acc[11] += v;
-                    } else {
+                        acc[2] = fmaf(dx, u, acc[2]);
+                        acc[3] = fmaf(dx, v, acc[3]);
+                        acc[4] = fmaf(dy, u, acc[4]);
+                        acc[5] = fmaf(dy, v, acc[5]);
                         acc[6] += u;
                         acc[7] += v;
+                    } else {
+                        acc[2] += u;
+                        acc[3] += v;
                     }
                 }
+                n_nz += (int)__popcll(__ballot(nz));
             }
 #pragma unroll
-            for (int q = 0; q < 6 + NJ; ++q) acc[q] = wave_sum_dpp(acc[q]);
-            if (quad && acc[4] > 0) {
+            for (int q = 0; q < 2 + NJ; ++q) acc[q] = wave_sum_dpp(acc[q]);
+            if (quad && wave_any(any_nan)) {
                 state = 2;
                 status = ST_ERROR;  // logical(NaN) throws
                 break;
             }
-            const bool masked_it = quad && (count > 0 || acc[3] > 0);
+            const bool masked_it = quad && (count > 0 || wave_any(any_zero));
             if (masked_it) {
                 // region of nonzero g through the seed: 8-connected for the
                 // 6-DOF solver (funICGNQuadtree.m:171), 4-connected for
@@ -686,11 +696,11 @@ __global__ __launch_bounds__(64) void icgn_wave_kernel(IcgnArgs a)
             bool literal = fs && !FM;
             double b[6] = {0, 0, 0, 0, 0, 0};
             if (!literal) {
-                double n = acc[0], S1 = acc[1], S2 = acc[2];
+                double n = n_nz, S1 = acc[0], S2 = acc[1];
                 double T[6], SJv[6], FJv[6];
 #pragma unroll
                 for (int q = 0; q < NJ; ++q) {
-                    T[q] = acc[6 + q];
+                    T[q] = acc[2 + q];
                     SJv[q] = L.SJ[q];
                     FJv[q] = L.FJ[q];
                 }
@@ -773,15 +783,18 @@ __global__ __launch_bounds__(64) void icgn_wave_kernel(IcgnArgs a)
                         __syncthreads();
                     }
                 }
-                const double gbar = (double)cf + S1 / n;
-                const double bg = n >= 2 ? sqrt(fmax(S2 - S1 * S1 / n, 0.0)) : (n == 1 ? 0.0 : NAN);
+                // gbar - c = S1/n, bottomg^2 = S2 - S1^2/n; b = FJ - (bf/bg) (T - (gbar-c) SJ)
+                const double inv_n = __builtin_amdgcn_rcp(n);
+                const double dg = S1 * inv_n;
+                const double var = S2 - S1 * dg;
                 const double bf = bottomf;
-                if (!(bf > 0) || !(bg > 0) || !isfinite(bf) || !isfinite(bg) || !isfinite(gbar)) {
+                if (!(n >= 2) || !(bf > 0) || !(var > 0) || !isfinite(bf) || !isfinite(var) ||
+                    !isfinite(dg)) {
                     literal = true;
                 } else {
+                    const double ratio = bf * __builtin_amdgcn_rsq(var);
 #pragma unroll
-                    for (int q = 0; q < NJ; ++q)
-                        b[q] = FJv[q] - (bf / bg) * (T[q] - (gbar - (double)cf) * SJv[q]);
+                    for (int q = 0; q < NJ; ++q) b[q] = FJv[q] - ratio * (T[q] - dg * SJv[q]);
                 }
             }
             if (literal) {
@@ -909,17 +922,18 @@ __global__ __launch_bounds__(64) void icgn_wave_kernel(IcgnArgs a)
                 for (int q = 0; q < 6; ++q) nb2 += b[q] * b[q];
             } else {
                 // :379 tempb = 2 b / bottomf^2 + mu (P - UOld - vdual)
-                const double s2 = 2.0 / (bottomf * bottomf);
+                const double s2 = 2.0 * __builtin_amdgcn_rcp(bottomf * bottomf);
                 b[0] = b[0] * s2 + a.mu * ((P[4] - uo_u) - vd_u);
                 b[1] = b[1] * s2 + a.mu * ((P[5] - uo_v) - vd_v);
                 nb2 = b[0] * b[0] + b[1] * b[1];
             }
-            double nn = sqrt(nb2);
-            normAbs = nn;
-            if (it == 1) normInit = nn;
-            nn = normInit > tol ? nn / normInit : 0.0;
-            normNew = nn;
-            if (nn < tol || normAbs < tol) {
+            abs2 = nb2;
+            if (it == 1) {
+                init_ok = nb2 > tol2;  // normOfWNewInit > tol
+                inv_init2 = __builtin_amdgcn_rcp(nb2);
+            }
+            rel2 = init_ok ? nb2 * inv_init2 : 0.0;
+            if (rel2 < tol2 || abs2 < tol2) {
                 state = 1;
                 break;
             }
@@ -938,16 +952,16 @@ __global__ __launch_bounds__(64) void icgn_wave_kernel(IcgnArgs a)
 
         // ---------------------------------------------------------- outputs
         if (state == 1) {
-            const double tol = a.tol;
             int st = it;
-            if (!(normNew < tol || normAbs < tol)) st = a.max_it + 1;
-            if (isnan(normNew)) st = a.max_it + 1;
+            if (!(rel2 < tol2 || abs2 < tol2)) st = a.max_it + 1;
+            if (isnan(rel2)) st = a.max_it + 1;
             if (quad && sabs < 1e-6) st = a.max_it + 3;
             status = st;
         }
         if (lane == 0) {
             if (state == 3) {
                 a.status[idx] = ST_DEFER;
+                if (a.defer_out) atomicOr(a.defer_out, 1);
             } else {
                 double Pout[6];
                 for (int k = 0; k < 6; ++k) Pout[k] = P[k];

@@ -19,6 +19,7 @@ struct IcgnArgs {
     int *status;
     int *work_counter;      // persistent mode: device queue head (nullptr -> 1 subset/block)
     const int *defer_flag;  // fallback launch: exit at once when no subset was deferred
+    int *defer_out;         // main launch: set to 1 when a subset is deferred
     long N;                 // subset count (array extents)
     long n_items;           // items of this launch
     long item0;             // first item (one launch per subset uses this)

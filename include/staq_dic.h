@@ -157,6 +157,12 @@ int staq_batch_set_input(staq_batch *b, int which, const double *host);
 int staq_batch_icgn6(staq_batch *b, const staq_params *p);
 /* 2-DOF AL local step (Subpb1Quadtree) over the whole batch. */
 int staq_batch_icgn2(staq_batch *b, const staq_params *p);
+/* One eager local step (mode 6 or 2) with every kernel launch bracketed by
+ * events on the stream it runs on; synchronizes and writes the device time of
+ * each bin's launch to kernel_ms[0 .. nbins-1] and of the fallback launch to
+ * kernel_ms[nbins] (nbins from staq_batch_bins).  A measurement aid with no
+ * counterpart in the reference. */
+int staq_batch_step_timed(staq_batch *b, const staq_params *p, int mode, float *kernel_ms);
 /* The one-workgroup-per-launch form: N launches of one subset each.  With
  * kernel_ms_sum non-NULL each launch is bracketed by events and the summed
  * device time (the overhead-free floor of this form) is returned. */
@@ -183,10 +189,11 @@ typedef struct staq_batch_ptrs {
     int wmax;
 } staq_batch_ptrs;
 int staq_batch_get_ptrs(staq_batch *b, staq_batch_ptrs *out);
-/* Number of window bins and, for bin k, its window size cap, item count and
- * workgroup size chosen for mode 6 / mode 2. */
+/* Number of window bins and, for bin k, its window size cap, item count,
+ * workgroup size chosen for mode 6 / mode 2 (0 = one wavefront per subset) and
+ * pixels per lane of the wavefront kernel (0 = the bin has none). */
 int staq_batch_bins(staq_batch *b, int *nbins, int *wcap, long *count, int *bs6, int *bs2,
-                    int maxbins);
+                    int *ppt, int maxbins);
 
 #ifdef __cplusplus
 }

@@ -65,6 +65,7 @@ SIGNATURES = {
     "staq_batch_icgn6": (C.c_int, [_vp, C.POINTER(StaqParams)]),
     "staq_batch_icgn2": (C.c_int, [_vp, C.POINTER(StaqParams)]),
     "staq_batch_icgn6_per_subset": (C.c_int, [_vp, C.POINTER(StaqParams), _dp]),
+    "staq_batch_step_timed": (C.c_int, [_vp, C.POINTER(StaqParams), C.c_int, _fp]),
     "staq_batch_get_output": (C.c_int, [_vp, _dp, _dp, _dp, _ip]),
     "staq_batch_chain_outputs": (C.c_int, [_vp]),
     "staq_batch_reorder": (C.c_int, [_vp]),
@@ -72,7 +73,7 @@ SIGNATURES = {
     "staq_batch_graph_capture": (C.c_int, [_vp, C.POINTER(StaqParams), C.c_int]),
     "staq_batch_graph_launch": (C.c_int, [_vp]),
     "staq_batch_get_ptrs": (C.c_int, [_vp, C.POINTER(BatchPtrs)]),
-    "staq_batch_bins": (C.c_int, [_vp, _ip, _ip, _lp, _ip, _ip, C.c_int]),
+    "staq_batch_bins": (C.c_int, [_vp, _ip, _ip, _lp, _ip, _ip, _ip, C.c_int]),
 }
 
 IN_U0, IN_UOLD, IN_FOLD, IN_VDUAL = 0, 1, 2, 3
@@ -231,6 +232,16 @@ class Batch:
                        "staq_batch_icgn6_per_subset")
         return ms.value
 
+    def step_timed(self, p, mode):
+        """One eager step; returns the device milliseconds of each bin's launch
+        followed by the fallback launch's."""
+        nb = len(self.bins())
+        ms = np.zeros(nb + 1, dtype=np.float32)
+        self.ctx.check(self.L.staq_batch_step_timed(self.h, C.byref(p), mode,
+                                                    ms.ctypes.data_as(C.POINTER(C.c_float))),
+                       "staq_batch_step_timed")
+        return ms.astype(np.float64)
+
     def reorder(self):
         self.ctx.check(self.L.staq_batch_reorder(self.h), "staq_batch_reorder")
 
@@ -268,8 +279,9 @@ class Batch:
         cnt = np.zeros(64, dtype=np.int64)
         b6 = np.zeros(64, dtype=np.int32)
         b2 = np.zeros(64, dtype=np.int32)
+        pp = np.zeros(64, dtype=np.int32)
         self.ctx.check(self.L.staq_batch_bins(self.h, C.byref(n), ip(wc), lp(cnt), ip(b6), ip(b2),
-                                              64), "staq_batch_bins")
+                                              ip(pp), 64), "staq_batch_bins")
         k = n.value
-        return [dict(wcap=int(wc[i]), count=int(cnt[i]), bs6=int(b6[i]), bs2=int(b2[i]))
-                for i in range(k)]
+        return [dict(wcap=int(wc[i]), count=int(cnt[i]), bs6=int(b6[i]), bs2=int(b2[i]),
+                     ppt=int(pp[i])) for i in range(k)]

@@ -124,6 +124,9 @@ struct staq_batch {
     // joined into the context stream with events
     std::vector<hipStream_t> side;
     std::vector<hipEvent_t> ev;  // [0] fork, [1..] joins
+    // timing events (created on first timed step): a start/stop pair per bin
+    // launch and one for the fallback launch, each recorded on that launch's stream
+    std::vector<hipEvent_t> tev;
 };
 
 static int fail(staq_ctx *ctx, int code, const std::string &msg)
@@ -618,6 +621,7 @@ void staq_batch_destroy(staq_batch *b)
         hipStreamDestroy(st);
     }
     for (hipEvent_t e : b->ev) hipEventDestroy(e);
+    for (hipEvent_t e : b->tev) hipEventDestroy(e);
     DevBuf *all[] = {&b->coords, &b->U0, &b->UOld, &b->FOld, &b->vdual, &b->U, &b->F,
                      &b->H, &b->status, &b->wx, &b->wy, &b->slot, &b->order, &b->ints};
     for (DevBuf *d : all) d->release();
@@ -674,7 +678,7 @@ int staq_batch_get_ptrs(staq_batch *b, staq_batch_ptrs *o)
 }
 
 int staq_batch_bins(staq_batch *b, int *nbins, int *wcap, long *count, int *bs6, int *bs2,
-                    int maxbins)
+                    int *ppt, int maxbins)
 {
     if (!b || !nbins) return STAQ_EINVAL;
     *nbins = (int)b->bins.size();
@@ -683,6 +687,7 @@ int staq_batch_bins(staq_batch *b, int *nbins, int *wcap, long *count, int *bs6,
         if (count) count[k] = b->bins[k].n;
         if (bs6) bs6[k] = b->bins[k].bs[0];
         if (bs2) bs2[k] = b->bins[k].bs[1];
+        if (ppt) ppt[k] = b->bins[k].ppt;
     }
     return STAQ_OK;
 }
@@ -730,7 +735,7 @@ static int occupancy_grid(int bs, size_t lds, long n)
 // the fallback for windows that outgrew their bin.  bin_only >= 0 launches a
 // single bin on s (autotuning), without fallback.
 static int enqueue_step(staq_batch *b, const staq_params *p, int mode, hipStream_t s,
-                        int bin_only = -1, int bs_override = -1)
+                        int bin_only = -1, int bs_override = -1, bool timed = false)
 {
     staq_ctx *ctx = b->ctx;
     const int nb = (int)b->bins.size();
@@ -759,6 +764,7 @@ static int enqueue_step(staq_batch *b, const staq_params *p, int mode, hipStream
         int bs = bs_override >= 0 ? bs_override : (p->block_size ? p->block_size : bin.bs[mi]);
         if (bs == 0 && !bin.ppt) bs = default_bs(bin.npix_cap);
         int grid = (int)bin.n;
+        if (timed) HIPCHK(ctx, hipEventRecord(b->tev[2 * k], sk));
         if (bs == 0) {
             // one wavefront per subset
             a.tile_cap = bin.wave_tile_cap;
@@ -774,6 +780,7 @@ static int enqueue_step(staq_batch *b, const staq_params *p, int mode, hipStream
             }
             HIPCHK(ctx, icgn_launch(a, mode, bs, grid, sk));
         }
+        if (timed) HIPCHK(ctx, hipEventRecord(b->tev[2 * k + 1], sk));
         if (fork && k > 0) {
             HIPCHK(ctx, hipEventRecord(b->ev[k], sk));
             HIPCHK(ctx, hipStreamWaitEvent(s, b->ev[k], 0));
@@ -794,7 +801,9 @@ static int enqueue_step(staq_batch *b, const staq_params *p, int mode, hipStream
         a.defer_flag = ints + nb + 1;
         const int bs = 256;
         const int grid = (int)std::min<long>(b->N, 256);
+        if (timed) HIPCHK(ctx, hipEventRecord(b->tev[2 * nb], s));
         HIPCHK(ctx, icgn_launch(a, mode, bs, grid, s));
+        if (timed) HIPCHK(ctx, hipEventRecord(b->tev[2 * nb + 1], s));
     }
     return STAQ_OK;
 }
@@ -813,6 +822,25 @@ int staq_batch_icgn2(staq_batch *b, const staq_params *p)
     if (!b || !p) return STAQ_EINVAL;
     hipSetDevice(b->ctx->device);
     return enqueue_step(b, p, 2, b->ctx->stream);
+}
+
+int staq_batch_step_timed(staq_batch *b, const staq_params *p, int mode, float *kernel_ms)
+{
+    if (!b || !p || !kernel_ms || (mode != 6 && mode != 2)) return STAQ_EINVAL;
+    staq_ctx *ctx = b->ctx;
+    hipSetDevice(ctx->device);
+    const int nb = (int)b->bins.size();
+    while ((int)b->tev.size() < 2 * nb + 2) {
+        hipEvent_t e;
+        HIPCHK(ctx, hipEventCreate(&e));
+        b->tev.push_back(e);
+    }
+    int rc = enqueue_step(b, p, mode, ctx->stream, -1, -1, true);
+    if (rc != STAQ_OK) return rc;
+    HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+    for (int k = 0; k <= nb; ++k)
+        HIPCHK(ctx, hipEventElapsedTime(&kernel_ms[k], b->tev[2 * k], b->tev[2 * k + 1]));
+    return STAQ_OK;
 }
 
 int staq_batch_icgn6_per_subset(staq_batch *b, const staq_params *p, double *kernel_ms_sum)

@@ -89,6 +89,15 @@ def load():
             raise RuntimeError(
                 f"{LIB_PATH} is missing: build it with `python -m staq_dic_amd.build` "
                 "(or __graft_entry__.build())")
+        # PyTorch-ROCm ships its own libamdhip64 (soname libamdhip64.so.7) and
+        # loads it under the name "libamdhip64.so"; loaded after this library
+        # it would bring a second HIP runtime into the process, which then
+        # finds no GPU.  Loading torch first makes this library bind to the
+        # runtime torch already holds, so tensors and streams are shared.
+        try:
+            import torch  # noqa: F401
+        except ImportError:
+            pass
         L = C.CDLL(LIB_PATH)
         for name, (res, args) in SIGNATURES.items():
             fn = getattr(L, name)

@@ -193,9 +193,8 @@ def classify_local(status, plain=False):
     Returns (bad mask, LocalICGNBadPtNum)."""
     st = np.asarray(status)
     bad = (st < 0) | (st > 99)
-    n_bad = int(bad.sum()) - (0 if plain else int((st == 102).sum()))
-    if plain:
-        n_bad = int(bad.sum()) - int((st == 102).sum())
+    # 102 (mostly masked) is bad but not counted (LocalICGNQuadtree.m:153-157, LocalICGN.m:119-122)
+    n_bad = int(bad.sum()) - int((st == 102).sum())
     good = ~bad
     if good.sum() >= 1:
         g = st[good].astype(float)

@@ -32,3 +32,26 @@ def oracle():
     from oracle import pyoracle
     pyoracle.build()
     return pyoracle
+
+
+@pytest.fixture(scope="session")
+def ref_frames():
+    """The reference's own frames 000 / 001 of the circular-hole sequence, its
+    1434-node quadtree mesh and its stored post-ADMM result for that pair
+    (tests/golden/circular_hole_000_001.npz), prepared as the reference's
+    driver prepares them: each image normalized over the ROI
+    (func/funNormalizeImg.m:34-38, std with N-1) and multiplied by its mask
+    (main_ALDIC_Quadtree_v2.m:57-64)."""
+    import numpy as np
+    z = np.load(os.path.join(ROOT, "tests", "golden", "circular_hole_000_001.npz"))
+    gx, gy = z["gridx"], z["gridy"]
+
+    def norm(img):
+        t = img[gx[0] - 1:gx[1], gy[0] - 1:gy[1]].astype(np.float64)
+        return (img.astype(np.float64) - t.mean()) / t.std(ddof=1)
+
+    m0 = z["mask0"].astype(np.float64)
+    m1 = z["mask1"].astype(np.float64)
+    return dict(f=norm(z["img0"]) * m0, g=norm(z["img1"]) * m1, mask=m0,
+                coords=z["coordinatesFEM"], winsize=int(z["winsize"]),
+                winsize_List=z["winsize_List"].astype(np.int64), U1=z["U1"], F1=z["F1"])

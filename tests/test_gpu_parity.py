@@ -173,6 +173,49 @@ def test_hole_masked_path(ctx, oracle):
     _cmp_icgn(sg2, so2, Ug2, Uo2, max_status_mismatch=0.02)
 
 
+def test_reference_frames(ctx, oracle, ref_frames):
+    """The reference's own circular-hole frames 000/001, its 1434-node mesh and
+    winsize_List: integer search (as IntegerSearch.m is called by the driver,
+    g and f swapped, radius 6), the 6-DOF local step (winsize 32 everywhere,
+    masked windows, growth, -1 / 102 codes) and the 2-DOF AL step on the
+    per-node windows, each against the oracle."""
+    from staq_dic_amd import funIntegerSearch
+    from staq_dic_amd.api import local_icgn_raw, subpb1_raw
+    r = ref_frames
+    f, g, m = r["f"], r["g"], r["mask"]
+    H, W = f.shape
+    _print("reference frames: integer search r=6, w=32, step 16 (g, f order)")
+    x, y, u, v, cc = funIntegerSearch(g, f, 6, [1, H], [1, W], 32, 16, ctx=ctx, slot=2)
+    xo, yo, uo, vo, cco = oracle.integer_search(g, f, 6, [1, H], [1, W], 32, 16)
+    np.testing.assert_array_equal(x, xo)
+    np.testing.assert_array_equal(y, yo)
+    fin = np.isfinite(uo)
+    np.testing.assert_array_equal(np.isfinite(u), fin)
+    np.testing.assert_array_equal(np.floor(u[fin]), np.floor(uo[fin]))
+    np.testing.assert_array_equal(np.floor(v[fin]), np.floor(vo[fin]))
+    np.testing.assert_allclose(u[fin], uo[fin], atol=1e-3 + 1e-12)
+    np.testing.assert_allclose(v[fin], vo[fin], atol=1e-3 + 1e-12)
+
+    N = r["coords"].shape[0]
+    U1 = r["U1"]
+    U0 = np.round(U1)
+    win = np.full(N, r["winsize"])
+    _print("reference frames: icgn6, N =", N)
+    Ug, Fg, Hg, sg = local_icgn_raw(U0, r["coords"], win, f, g, m, tol=1e-2, ctx=ctx)
+    of = oracle.Frame(f, g, m)
+    Uo, Fo, Ho, so = oracle.local_icgn(of, U0, r["coords"], win, tol=1e-2)
+    assert (so == -1).sum() > 50 and (so == 102).sum() > 100
+    assert _cmp_icgn(sg, so, Ug, Uo, Fg, Fo, max_status_mismatch=0.01) > 650
+
+    wl = r["winsize_List"]
+    _print("reference frames: icgn2 on winsize_List")
+    vd = np.zeros(2 * N)
+    Ug2, sg2 = subpb1_raw(U1, r["F1"], vd, r["coords"], wl[:, 0], wl[:, 1], 1e-3, f, g, m,
+                          tol=1e-2, ctx=ctx)
+    Uo2, so2 = oracle.subpb1(of, U1, r["F1"], vd, r["coords"], wl[:, 0], wl[:, 1], 1e-3, tol=1e-2)
+    _cmp_icgn(sg2, so2, Ug2, Uo2, max_status_mismatch=0.01)
+
+
 def test_forced_status_codes(ctx, oracle):
     """Each failure code of the reference, forced from the host."""
     from staq_dic_amd.api import local_icgn_raw
@@ -279,3 +322,39 @@ def test_batch_forms_agree(ctx, plain_cfg):
     ok = (ref[3] == got[3]) & (ref[3] >= 1) & (ref[3] <= 100)
     np.testing.assert_allclose(ref[0].reshape(-1, 2)[ok], got[0].reshape(-1, 2)[ok], atol=1e-4)
     b.close()
+
+
+def test_znssd_autograd(ctx, oracle, plain_cfg):
+    """The torch.autograd.Function: forward cost and the gradient reaching a
+    nodal displacement tensor through P[:, 4:6] = U, against the oracle."""
+    import torch
+    from staq_dic_amd import synth
+    from staq_dic_amd.autograd import znssd
+    fr, coords, win = plain_cfg
+    sel = np.arange(0, coords.shape[0], 5)
+    coords, win = coords[sel], win[sel]
+    truth = synth.truth_at(fr.field, coords)
+    rng = np.random.default_rng(4)
+    P0 = truth + rng.normal(0, [0.002, 0.002, 0.002, 0.002, 0.15, 0.15], truth.shape)
+    ctx.set_frame(fr.f, fr.g, fr.mask)
+    _print("znssd autograd, N =", coords.shape[0])
+    dev = torch.device("cuda", 0)
+    U = torch.tensor(P0[:, 4:6], dtype=torch.float64, device=dev, requires_grad=True)
+    A = torch.tensor(P0[:, :4], dtype=torch.float64, device=dev, requires_grad=True)
+    P = torch.cat([A, U], dim=1)
+    cost, status = znssd(P, torch.tensor(coords, device=dev),
+                         torch.tensor(win, dtype=torch.int32, device=dev), ctx)
+    w = torch.linspace(0.5, 1.5, cost.shape[0], dtype=torch.float64, device=dev)
+    (w * cost).sum().backward()
+    torch.cuda.synchronize()
+    of = oracle.Frame(fr.f, fr.g, fr.mask)
+    tot, co, go, so = oracle.znssd(of, P0, coords, win)
+    st = status.cpu().numpy()
+    np.testing.assert_array_equal(st, so)
+    ok = so == 0
+    np.testing.assert_allclose(cost.detach().cpu().numpy()[ok], co[ok], rtol=1e-4, atol=1e-6)
+    wn = w.cpu().numpy()[:, None]
+    gP = np.concatenate([A.grad.cpu().numpy(), U.grad.cpu().numpy()], axis=1)
+    ref = go * wn
+    scale = np.abs(ref[ok]).max(axis=0) + 1e-12
+    assert np.max(np.abs(gP[ok] - ref[ok]) / scale) < 2e-3

@@ -139,7 +139,7 @@ def main():
     kms /= args.steps
     kdom = int(np.argmax(kms[:-1]))
     bdom = bins[kdom]
-    members = (win + 7) // 8 == (bdom["wcap"] + 7) // 8
+    members = b.bin_of() == kdom
     iters = np.where(conv, st, 0).astype(float)
     bytes_launch = algorithmic_bytes(win[members], win[members], iters[members])
     achieved = bytes_launch / (kms[kdom] / 1e3) / 1e9

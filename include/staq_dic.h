@@ -195,6 +195,11 @@ int staq_batch_get_ptrs(staq_batch *b, staq_batch_ptrs *out);
 int staq_batch_bins(staq_batch *b, int *nbins, int *wcap, long *count, int *bs6, int *bs2,
                     int *ppt, int maxbins);
 
+/* Bin index of every subset (N entries, in the caller's subset order).  Bins
+ * group similar window sizes; windows that touch the mask of their frame at
+ * batch creation form bins of their own. */
+int staq_batch_bin_of(staq_batch *b, int *bin);
+
 #ifdef __cplusplus
 }
 #endif

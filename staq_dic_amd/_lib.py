@@ -74,6 +74,7 @@ SIGNATURES = {
     "staq_batch_graph_launch": (C.c_int, [_vp]),
     "staq_batch_get_ptrs": (C.c_int, [_vp, C.POINTER(BatchPtrs)]),
     "staq_batch_bins": (C.c_int, [_vp, _ip, _ip, _lp, _ip, _ip, _ip, C.c_int]),
+    "staq_batch_bin_of": (C.c_int, [_vp, _ip]),
 }
 
 IN_U0, IN_UOLD, IN_FOLD, IN_VDUAL = 0, 1, 2, 3
@@ -281,6 +282,12 @@ class Batch:
         p = BatchPtrs()
         self.ctx.check(self.L.staq_batch_get_ptrs(self.h, C.byref(p)), "staq_batch_get_ptrs")
         return p
+
+    def bin_of(self):
+        """Bin index of every subset, in the caller's order."""
+        out = np.zeros(self.N, dtype=np.int32)
+        self.ctx.check(self.L.staq_batch_bin_of(self.h, ip(out)), "staq_batch_bin_of")
+        return out
 
     def bins(self):
         n = C.c_int()

@@ -106,6 +106,7 @@ struct Bin {
     int npix_cap = 0, tile_cap = 0;
     int bs[2] = {0, 0};  // [mode 6, mode 2]: 0 = wave kernel, else workgroup size
     int ppt = 0;         // pixels per lane of the wave kernel (0: not eligible)
+    int masked = 0;      // the bin's windows touch the mask (their window grows)
     int wave_tile_cap = 0;
 };
 
@@ -520,15 +521,75 @@ int staq_batch_create(staq_ctx *ctx, staq_batch **out, long N, const double *coo
         wmax = std::max(wmax, std::max(hx[i], hy[i]));
         order[i] = (int)i;
     }
-    // largest windows first: the hardware dispatcher then fills the tail of
-    // each bin with short subsets
-    auto key = [&](int i) { return (std::max(hx[i], hy[i]) + 7) / 8; };
-    std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return key(a) > key(b); });
     staq_batch *b = new staq_batch();
     b->ctx = ctx;
     b->N = N;
     b->wmax = wmax;
     b->has_slot = slot != nullptr;
+    int rc = STAQ_OK;
+    auto chk = [&](hipError_t e, const char *w) {
+        if (e != hipSuccess && rc == STAQ_OK) rc = hipfail(ctx, e, w);
+    };
+    chk(b->coords.ensure(N * 16), "coords");
+    chk(b->U0.ensure(N * 16), "U0");
+    chk(b->UOld.ensure(N * 16), "UOld");
+    chk(b->FOld.ensure(N * 32), "FOld");
+    chk(b->vdual.ensure(N * 16), "vdual");
+    chk(b->U.ensure(N * 16), "U");
+    chk(b->F.ensure(N * 32), "F");
+    chk(b->H.ensure(N * 21 * 8), "H");
+    chk(b->status.ensure(N * 4), "status");
+    chk(b->wx.ensure(N * 4), "wx");
+    chk(b->wy.ensure(N * 4), "wy");
+    chk(b->slot.ensure(N * 4), "slot");
+    chk(b->order.ensure(N * 4), "order");
+    // which windows touch the mask of their frame (their window grows at
+    // set-up): classified on the device from the frames resident now
+    std::vector<int> cnt(N, 0);
+    if (rc == STAQ_OK) {
+        hipStream_t s = ctx->stream;
+        DevBuf dcls;
+        chk(dcls.ensure(N * 4), "class");
+        chk(hipMemcpyAsync(b->coords.p, coords, N * 16, hipMemcpyHostToDevice, s), "coords");
+        chk(hipMemcpyAsync(b->wx.p, hx.data(), N * 4, hipMemcpyHostToDevice, s), "wx");
+        chk(hipMemcpyAsync(b->wy.p, hy.data(), N * 4, hipMemcpyHostToDevice, s), "wy");
+        chk(hipMemcpyAsync(b->slot.p, hs.data(), N * 4, hipMemcpyHostToDevice, s), "slot");
+        if (rc == STAQ_OK)
+            chk(mask_class_launch(ctx->d_frames.as<FrameDev>(), b->slot.as<int>(), b->coords.as<double>(),
+                                  b->wx.as<int>(), b->wy.as<int>(), N, dcls.as<int>(), s),
+                "mask classes");
+        if (rc == STAQ_OK)
+            chk(hipMemcpyAsync(cnt.data(), dcls.p, N * 4, hipMemcpyDeviceToHost, s), "class");
+        chk(hipStreamSynchronize(s), "sync");
+        dcls.release();
+    }
+    if (rc != STAQ_OK) {
+        staq_batch_destroy(b);
+        return rc;
+    }
+    // bins of similar window size; windows that touch the mask apart, grouped
+    // by the wavefront kernel their grown window needs (the growth rules of
+    // funICGNQuadtree.m:66-70 and funICGN_Subpb1.m:76-80); largest windows
+    // first, so that the hardware dispatcher fills each bin's tail with short
+    // subsets
+    std::vector<int> gppt(N, 0);
+    for (long i = 0; i < N; ++i) {
+        if (cnt[i] == 0) continue;
+        const double n0 = (double)(hx[i] + 1) * (hy[i] + 1);
+        int gx = hx[i], gy = hy[i];
+        if (cnt[i] > 0 && cnt[i] < 0.4 * n0) {
+            gx = 2 * (int)std::ceil(std::sqrt(n0 / (n0 - cnt[i])) * hx[i] / 2.0);
+            const double n1 = (double)(gx + 1) * (hy[i] + 1);
+            gy = std::max(gx, 2 * (int)std::ceil(std::sqrt(n1 / (n1 - cnt[i])) * hy[i] / 2.0));
+            gx = std::max(gx, gy);
+        }
+        const int pg = icgn_wave_ppt((gx + 1) * (gy + 1));
+        gppt[i] = pg ? pg : 31;
+    }
+    auto key = [&](int i) {
+        return 64 * ((std::max(hx[i], hy[i]) + 7) / 8) + (cnt[i] ? 32 + gppt[i] : 0);
+    };
+    std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return key(a) > key(b); });
     long i0 = 0;
     int fbx = 0, fby = 0;
     while (i0 < N) {
@@ -542,13 +603,14 @@ int staq_batch_create(staq_ctx *ctx, staq_batch **out, long N, const double *coo
         }
         bin.item0 = i0;
         bin.n = i1 - i0;
+        bin.masked = (k & 32) != 0;
         bin.npix_cap = (bin.wxcap + 1) * (bin.wycap + 1);
         bin.tile_cap = tile_side(bin.wxcap + 1) * tile_side(bin.wycap + 1);
-        // the wave kernel covers the bin's windows plus the mask-driven growth
-        // of its smallest class; larger grown windows defer to the fallback
-        const int wg = std::max(grown_window(bin.wxcap), grown_window(bin.wycap)) - 2;
-        bin.ppt = icgn_wave_ppt(std::max(bin.npix_cap, (wg + 1) * (wg + 1)));
-        if (!bin.ppt) bin.ppt = icgn_wave_ppt(bin.npix_cap);
+        // the wave kernel of a bin that touches the mask covers the grown
+        // windows of its smallest class too; a subset that outgrows its bin
+        // (or grows although classified clear) defers to the fallback launch
+        bin.ppt = icgn_wave_ppt(bin.npix_cap);
+        if (bin.masked && (k & 31) != 31) bin.ppt = std::max(bin.ppt, k & 31);
         if (bin.ppt) {
             const int side = tile_side((int)std::sqrt(64.0 * bin.ppt));
             bin.wave_tile_cap = side * side;
@@ -570,30 +632,9 @@ int staq_batch_create(staq_ctx *ctx, staq_batch **out, long N, const double *coo
         hipEvent_t e = nullptr;
         if (hipEventCreateWithFlags(&e, hipEventDisableTiming) == hipSuccess) b->ev.push_back(e);
     }
-    int rc = STAQ_OK;
-    auto chk = [&](hipError_t e, const char *w) {
-        if (e != hipSuccess && rc == STAQ_OK) rc = hipfail(ctx, e, w);
-    };
-    chk(b->coords.ensure(N * 16), "coords");
-    chk(b->U0.ensure(N * 16), "U0");
-    chk(b->UOld.ensure(N * 16), "UOld");
-    chk(b->FOld.ensure(N * 32), "FOld");
-    chk(b->vdual.ensure(N * 16), "vdual");
-    chk(b->U.ensure(N * 16), "U");
-    chk(b->F.ensure(N * 32), "F");
-    chk(b->H.ensure(N * 21 * 8), "H");
-    chk(b->status.ensure(N * 4), "status");
-    chk(b->wx.ensure(N * 4), "wx");
-    chk(b->wy.ensure(N * 4), "wy");
-    chk(b->slot.ensure(N * 4), "slot");
-    chk(b->order.ensure(N * 4), "order");
     chk(b->ints.ensure((nb + 2) * 4), "counters");
     if (rc == STAQ_OK) {
         hipStream_t s = ctx->stream;
-        chk(hipMemcpyAsync(b->coords.p, coords, N * 16, hipMemcpyHostToDevice, s), "coords");
-        chk(hipMemcpyAsync(b->wx.p, hx.data(), N * 4, hipMemcpyHostToDevice, s), "wx");
-        chk(hipMemcpyAsync(b->wy.p, hy.data(), N * 4, hipMemcpyHostToDevice, s), "wy");
-        chk(hipMemcpyAsync(b->slot.p, hs.data(), N * 4, hipMemcpyHostToDevice, s), "slot");
         chk(hipMemcpyAsync(b->order.p, order.data(), N * 4, hipMemcpyHostToDevice, s), "order");
         chk(hipMemsetAsync(b->U0.p, 0, N * 16, s), "zero");
         chk(hipMemsetAsync(b->UOld.p, 0, N * 16, s), "zero");
@@ -693,6 +734,18 @@ int staq_batch_bins(staq_batch *b, int *nbins, int *wcap, long *count, int *bs6,
 }
 
 }  // extern "C"
+
+extern "C" int staq_batch_bin_of(staq_batch *b, int *bin)
+{
+    if (!b || !bin) return STAQ_EINVAL;
+    staq_ctx *ctx = b->ctx;
+    hipSetDevice(ctx->device);
+    std::vector<int> order(b->N);
+    HIPCHK(ctx, hipMemcpy(order.data(), b->order.p, b->N * 4, hipMemcpyDeviceToHost));
+    for (int k = 0; k < (int)b->bins.size(); ++k)
+        for (long t = b->bins[k].item0; t < b->bins[k].item0 + b->bins[k].n; ++t) bin[order[t]] = k;
+    return STAQ_OK;
+}
 
 static IcgnArgs base_args(staq_batch *b, const staq_params *p)
 {

@@ -251,6 +251,43 @@ __global__ __launch_bounds__(BS) void znssd_kernel(ZnssdArgs a)
         for (int k = 0; k < 6; ++k) a.gradP[6 * idx + k] = up * go[k];
 }
 
+// Zeros of f * mask in each subset's window (LocalICGNQuadtree.m:71-82): a
+// window with any grows at set-up (funICGNQuadtree.m:66-70), so the batch
+// bins such subsets by their grown size, apart from the rest.  One wavefront
+// per subset; -1 for a window that does not fit in the image.
+__global__ __launch_bounds__(64) void mask_class_kernel(const FrameDev *frames, const int *slot,
+                                                         const double *coords, const int *wx,
+                                                         const int *wy, long N, int *out)
+{
+    const long i = blockIdx.x;
+    if (i >= N) return;
+    const FrameDev fr = frames[slot ? slot[i] : 0];
+    const int w = wx[i], h = wy[i];
+    const int xa = (int)round(coords[i]) - w / 2, ya = (int)round(coords[i + N]) - h / 2;
+    if (xa < 1 || ya < 1 || xa + w > fr.H || ya + h > fr.W) {
+        if (threadIdx.x == 0) out[i] = -1;
+        return;
+    }
+    const int nx = w + 1, npix = nx * (h + 1);
+    int z = 0;
+    for (int p = threadIdx.x; p < npix; p += 64) {
+        const int j = p / nx, q = p - j * nx;
+        z += fidx(fr.f, fr.H, xa + q, ya + j) * fidx(fr.mask, fr.H, xa + q, ya + j) == 0.f;
+    }
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) z += __shfl_xor(z, o, 64);
+    if (threadIdx.x == 0) out[i] = z;
+}
+
+hipError_t mask_class_launch(const FrameDev *frames, const int *slot, const double *coords,
+                             const int *wx, const int *wy, long N, int *out, hipStream_t s)
+{
+    if (N <= 0) return hipSuccess;
+    hipLaunchKernelGGL(mask_class_kernel, dim3((unsigned)N), dim3(64), 0, s, frames, slot, coords,
+                       wx, wy, N, out);
+    return hipGetLastError();
+}
+
 hipError_t znssd_launch(const ZnssdArgs &a, hipStream_t s)
 {
     if (a.N <= 0) return hipSuccess;

@@ -164,6 +164,7 @@ __global__ __launch_bounds__(64) void icgn_wave_kernel(IcgnArgs a)
 {
     constexpr int NJ = MODE;
     constexpr int CAP = 64 * PPT;
+    constexpr bool BRANCH_FREE = PPT <= 9;
     extern __shared__ __align__(16) unsigned char smem[];
     __shared__ WaveLds L;
     __shared__ long item_sh;
@@ -426,6 +427,11 @@ __global__ __launch_bounds__(64) void icgn_wave_kernel(IcgnArgs a)
         int it = 0, sum_bw2 = 0;
         bool fsel = false;  // 2-DOF: f side taken from the last g region
         unsigned selbits = ~0u;  // pixels of the current g region (bit k = pixel lane + 64 k)
+        // the g region is a function of the zero pattern of g alone: keep the
+        // pattern it was computed from (nzprev) and reuse it while unchanged;
+        // spec_next = the last iteration was masked, so the next pass applies it
+        unsigned nzprev = 0u;
+        bool region_valid = false, spec_next = false, region_new_prev = false;
         // g tile resident in LDS: [tu0, tu0+tnx) x [tv0, tv0+tny) (1-based)
         int tu0 = 0, tv0 = 0, tnx = 0, tny = 0;
         bool have_tile = false;
@@ -443,8 +449,10 @@ __global__ __launch_bounds__(64) void icgn_wave_kernel(IcgnArgs a)
                 break;
             }
             ++it;
-            if (MODE == 6 && quad && it > 1 && count > 0) {
-                // funICGNQuadtree.m:139-145 (see icgn.hip)
+            if (MODE == 6 && quad && it > 1 && count > 0 && region_new_prev) {
+                // funICGNQuadtree.m:139-145 (see icgn.hip); the size depends
+                // on the last g region only, so it is re-derived when that changed
+                region_new_prev = false;
                 const double wn =
                     2.0 * ceil(sqrt((double)(w0 + 1) * (w0 + 1) / (double)sum_bw2) * wx / 2.0);
                 if (wn != (double)wx) {
@@ -485,17 +493,28 @@ __global__ __launch_bounds__(64) void icgn_wave_kernel(IcgnArgs a)
                 }
                 break;
             }
-            double umin = INFINITY, umax = -INFINITY, vmin = INFINITY, vmax = -INFINITY;
-            bool out = false;
-            for (int q = 0; q < 4; ++q) {
+            // the warped window's extremes are at its corners (funICGNQuadtree.m:149-157):
+            // lane q of every quad takes corner q, quad permutations give the extremes
+            double umin, umax, vmin, vmax;
+            bool out;
+            {
+                const int q = lane & 3;
                 const double dx = (q & 1) ? hx : -hx, dy = (q & 2) ? hy : -hy;
                 const double u = (1 + P[0]) * dx + P[2] * dy + (x0 + P[4]);
                 const double v = P[1] * dx + (1 + P[3]) * dy + (y0 + P[5]);
-                if (u < 3 || u > fr.H - 2 || v < 3 || v > fr.W - 2) out = true;
-                umin = fmin(umin, u);
-                umax = fmax(umax, u);
-                vmin = fmin(vmin, v);
-                vmax = fmax(vmax, v);
+                out = wave_any(u < 3 || u > fr.H - 2 || v < 3 || v > fr.W - 2);
+                umin = fmin(u, dpp_f64<0xB1>(u));  // quad_perm [1,0,3,2]
+                umax = fmax(u, dpp_f64<0xB1>(u));
+                vmin = fmin(v, dpp_f64<0xB1>(v));
+                vmax = fmax(v, dpp_f64<0xB1>(v));
+                umin = fmin(umin, dpp_f64<0x4E>(umin));  // quad_perm [2,3,0,1]
+                umax = fmax(umax, dpp_f64<0x4E>(umax));
+                vmin = fmin(vmin, dpp_f64<0x4E>(vmin));
+                vmax = fmax(vmax, dpp_f64<0x4E>(vmax));
+                umin = readlane_f64(umin, 0);  // uniform from here on
+                umax = readlane_f64(umax, 0);
+                vmin = readlane_f64(vmin, 0);
+                vmax = readlane_f64(vmax, 0);
             }
             if (out) {
                 rel2 = 1e12;
@@ -556,8 +575,8 @@ __global__ __launch_bounds__(64) void icgn_wave_kernel(IcgnArgs a)
                 __syncthreads();
             }
             // Keys bicubic of pixel (i, j) of the warped window, from the
-            // tile or, if the warped support outgrew it, from HBM
-            auto gather = [&](int i, int j) {
+            // resident tile or, if the warped support outgrew it, from HBM
+            auto gather = [&](auto from_tile, int i, int j) {
                 const float dx = (float)i - hx, dy = (float)j - hy;
                 const float lu = fmaf(a11, dx, fmaf(a12, dy, cu));
                 const float lv = fmaf(a21, dx, fmaf(a22, dy, cv));
@@ -567,7 +586,7 @@ __global__ __launch_bounds__(64) void icgn_wave_kernel(IcgnArgs a)
                 keys_w(lv - fvv, wv);
                 const int iu = (int)fu - 1, iv = (int)fvv - 1;
                 float g = 0.f;
-                if (use_tile) {
+                if constexpr (decltype(from_tile)::value) {
                     const float *b = tile + iu + iv * tnx;
 #pragma unroll
                     for (int q = 0; q < 4; ++q) {
@@ -587,31 +606,50 @@ __global__ __launch_bounds__(64) void icgn_wave_kernel(IcgnArgs a)
                 }
                 return g;
             };
-            // per-iteration sums: S1 = sum(g - c), S2 = sum((g - c)^2) over the
-            // nonzero g, T_q = sum J_q (g - c) over the selected pixels; the
-            // counts and the zero / NaN tests are wave votes
+            auto gather_rt = [&](int i, int j) {
+                return use_tile ? gather(std::true_type{}, i, j) : gather(std::false_type{}, i, j);
+            };
+            // Per-iteration sums: S1 = sum(g - c), S2 = sum((g - c)^2) over the
+            // nonzero g, T_q = sum J_q (g - c) over the f-side pixels; counts
+            // and the zero / NaN tests are wave votes.  When the last iteration
+            // was masked, its g region is applied here already: the region
+            // depends only on the zero pattern of g, which rarely changes from
+            // one iteration to the next, so this pass is usually the final one.
+            const bool gspec = spec_next;
+            const unsigned fbits = (MODE == 2 && fsel) ? selbits : s0bits;
             float gr[PPT];
             float acc[2 + NJ];
 #pragma unroll
             for (int q = 0; q < 2 + NJ; ++q) acc[q] = 0.f;
             int n_nz = 0;
+            unsigned nzb = 0u;
             bool any_zero = false, any_nan = false;
+            // every lane runs every pixel slot (a lane past the window repeats
+            // its last pixel and contributes nothing): no branch splits the
+            // unrolled slots, so the gathers of several pixels are in flight
+            // together
+            // (up to 9 pixels per lane; with 18 the registers do not allow it
+            // and each slot keeps its branch)
+            auto fused_pass = [&](auto from_tile) {
 #pragma unroll
-            for (int k = 0; k < PPT; ++k) {
-                const int p = lane + 64 * k;
-                gr[k] = 0.f;
-                bool nz = false;
-                if (p < npix) {
-                    const int j = (int)(((float)p + 0.5f) * rnx), i = p - j * nx;
-                    const float g = gather(i, j);
-                    gr[k] = g;
-                    const float d = g - cf;
-                    nz = !quad || fabsf(g) > 1e-10f;
+                for (int k = 0; k < PPT; ++k) {
+                    const int p = lane + 64 * k;
+                    const bool valid = p < npix;
+                    bool nz = false;
+                    if (BRANCH_FREE || valid) {
+                    const int pc = valid ? p : npix - 1;
+                    const int j = (int)(((float)pc + 0.5f) * rnx), i = pc - j * nx;
+                    const float g = gather(from_tile, i, j);
+                    gr[k] = valid ? g : 0.f;
+                    any_zero |= valid && g == 0.f;
+                    any_nan |= valid && isnan(g);
+                    nzb |= (valid && g != 0.f) ? (1u << k) : 0u;
+                    const float ge = (valid && (!gspec || ((selbits >> k) & 1u))) ? g : 0.f;
+                    const float d = ge - cf;
+                    nz = valid && (!quad || fabsf(ge) > 1e-10f);
                     acc[0] += nz ? d : 0.f;
                     acc[1] += nz ? d * d : 0.f;
-                    any_zero |= g == 0.f;
-                    any_nan |= isnan(g);
-                    const bool s0 = (s0bits >> k) & 1u;
+                    const bool s0 = valid && ((fbits >> k) & 1u);
                     const float u = s0 ? fxr[k] * d : 0.f, v = s0 ? fyr[k] * d : 0.f;
                     if (MODE == 6) {
                         const float dx = (float)i - hx, dy = (float)j - hy;
@@ -625,35 +663,44 @@ __global__ __launch_bounds__(64) void icgn_wave_kernel(IcgnArgs a)
                         acc[2] += u;
                         acc[3] += v;
                     }
+                    }
+                    n_nz += (int)__popcll(__ballot(nz));
                 }
-                n_nz += (int)__popcll(__ballot(nz));
-            }
-#pragma unroll
-            for (int q = 0; q < 2 + NJ; ++q) acc[q] = wave_sum_dpp(acc[q]);
+            };
+            if (use_tile)
+                fused_pass(std::true_type{});
+            else
+                fused_pass(std::false_type{});
             if (quad && wave_any(any_nan)) {
                 state = 2;
                 status = ST_ERROR;  // logical(NaN) throws
                 break;
             }
             const bool masked_it = quad && (count > 0 || wave_any(any_zero));
-            if (masked_it) {
+            bool region_new = false;
+            if (masked_it && !(region_valid && !wave_any(nzb != nzprev))) {
                 // region of nonzero g through the seed: 8-connected for the
                 // 6-DOF solver (funICGNQuadtree.m:171), 4-connected for
                 // Subpb1 (funICGN_Subpb1.m:160); if that is the whole window,
                 // the window mask's 8-connected region (:175-176 / :164-165)
-                L.colw[lane] = 0ull;
-                __syncthreads();
-#pragma unroll
-                for (int k = 0; k < PPT; ++k) {
-                    const int p = lane + 64 * k;
-                    if (p < npix && gr[k] != 0.f) {
-                        const int j = (int)(((float)p + 0.5f) * rnx), i = p - j * nx;
-                        atomicOr(&L.colw[j], 1ull << i);
-                    }
-                }
-                __syncthreads();
+                // column words from one ballot per pixel slot: slot k's ballot
+                // is pixels 64k .. 64k+63 of the window in column-major order,
+                // so column j is bits j*nx .. j*nx+nx-1 of that bit stream
                 const unsigned long long full = nx >= 64 ? ~0ull : ((1ull << nx) - 1ull);
-                unsigned long long sw = wave_flood(L.colw[lane], ny, seed_r, seed_c, MODE == 6 ? 8 : 4);
+                unsigned long long colnz = 0ull;
+                {
+                    const int sb = lane * nx, wk = sb >> 6, off = sb & 63;
+                    unsigned long long lo = 0ull, hi = 0ull;
+#pragma unroll
+                    for (int k = 0; k < PPT; ++k) {
+                        const unsigned long long bk = __ballot((nzb >> k) & 1u);
+                        lo = wk == k ? bk : lo;
+                        hi = wk + 1 == k ? bk : hi;
+                    }
+                    colnz = off ? ((lo >> off) | (hi << (64 - off))) : lo;
+                    colnz = lane < ny ? (colnz & full) : 0ull;
+                }
+                unsigned long long sw = wave_flood(colnz, ny, seed_r, seed_c, MODE == 6 ? 8 : 4);
                 const bool all = !wave_any(lane < ny && sw != full);
                 if (all) {
                     if (!masked_f) {
@@ -688,98 +735,122 @@ __global__ __launch_bounds__(64) void icgn_wave_kernel(IcgnArgs a)
                         if ((L.colw[j] >> i) & 1ull) selbits |= 1u << k;
                     }
                 }
+                nzprev = nzb;
+                region_valid = true;
+                region_new = true;
+                region_new_prev = true;
                 if (MODE == 2) fsel = true;
             }
+            // the fused pass stands unless the region changed or the region
+            // was applied to an iteration that turned out unmasked
+            const bool resum = masked_it ? (region_new || !gspec) : gspec;
+            spec_next = masked_it;
             const bool fs = MODE == 2 && fsel;
             // the 2-DOF region rebuild needs f*mask per pixel; windows too large
             // to keep it in registers take the literal path
             bool literal = fs && !FM;
             double b[6] = {0, 0, 0, 0, 0, 0};
             if (!literal) {
-                double n = n_nz, S1 = acc[0], S2 = acc[1];
-                double T[6], SJv[6], FJv[6];
+                double n, S1, S2, T[6], SJv[6], FJv[6];
 #pragma unroll
                 for (int q = 0; q < NJ; ++q) {
-                    T[q] = acc[2 + q];
                     SJv[q] = L.SJ[q];
                     FJv[q] = L.FJ[q];
                 }
-                if (masked_it || fs) {
-                    // the same sums with g (and, for Subpb1, f and grad f) on the region
-                    constexpr int NR = MODE == 6 ? 9 : 16;
-                    float r[NR];
+                if (!resum) {
 #pragma unroll
-                    for (int q = 0; q < NR; ++q) r[q] = 0.f;
+                    for (int q = 0; q < 2 + NJ; ++q) acc[q] = wave_sum_dpp(acc[q]);
+                    n = n_nz;
+                    S1 = acc[0];
+                    S2 = acc[1];
+#pragma unroll
+                    for (int q = 0; q < NJ; ++q) T[q] = acc[2 + q];
+                } else {
+                    // the same sums with g on the current region (or on no
+                    // region); Subpb1 also rebuilds f, grad f, H and the f
+                    // statistics on a new region (funICGN_Subpb1.m:171-190),
+                    // which then persist
+                    const bool fre = MODE == 2 && region_new;
+                    const unsigned fb = (MODE == 2 && fsel) ? selbits : s0bits;
+                    float r[2 + NJ];
+                    float q6[11];
+#pragma unroll
+                    for (int q = 0; q < 2 + NJ; ++q) r[q] = 0.f;
+#pragma unroll
+                    for (int q = 0; q < 11; ++q) q6[q] = 0.f;
+                    int nr = 0;
 #pragma unroll
                     for (int k = 0; k < PPT; ++k) {
                         const int p = lane + 64 * k;
+                        bool nz = false;
                         if (p < npix) {
                             const bool sg = !masked_it || ((selbits >> k) & 1u);
                             const float g = sg ? gr[k] : 0.f;
                             const float d = g - cf;
-                            const bool nz = fabsf(g) > 1e-10f;
-                            r[0] += nz ? 1.f : 0.f;
-                            r[1] += nz ? d : 0.f;
-                            r[2] += nz ? d * d : 0.f;
+                            nz = fabsf(g) > 1e-10f;
+                            r[0] += nz ? d : 0.f;
+                            r[1] += nz ? d * d : 0.f;
+                            const bool sf = (fb >> k) & 1u;
+                            const float fx = sf ? fxr[k] : 0.f, fy = sf ? fyr[k] : 0.f;
                             if (MODE == 6) {
                                 const int j = (int)(((float)p + 0.5f) * rnx), i = p - j * nx;
-                                const bool s0 = (s0bits >> k) & 1u;
-                                const float u = s0 ? fxr[k] * d : 0.f, v = s0 ? fyr[k] * d : 0.f;
                                 const float dx = (float)i - hx, dy = (float)j - hy;
-                                r[3] = fmaf(dx, u, r[3]);
-                                r[4] = fmaf(dx, v, r[4]);
-                                r[5] = fmaf(dy, u, r[5]);
-                                r[6] = fmaf(dy, v, r[6]);
-                                r[7] += u;
-                                r[8] += v;
+                                const float u = fx * d, v = fy * d;
+                                r[2] = fmaf(dx, u, r[2]);
+                                r[3] = fmaf(dx, v, r[3]);
+                                r[4] = fmaf(dy, u, r[4]);
+                                r[5] = fmaf(dy, v, r[5]);
+                                r[6] += u;
+                                r[7] += v;
                             } else {
-                                const bool sf = (selbits >> k) & 1u;
-                                const float fx = sf ? fxr[k] : 0.f, fy = sf ? fyr[k] : 0.f;
-                                const float tf = sf ? fmr[FM ? k : 0] : 0.f;
-                                const float df = tf - cf;
-                                const bool fnz = fabsf(tf) > 1e-10f;
-                                r[3] += fx * d;
-                                r[4] += fy * d;
-                                r[5] += fnz ? 1.f : 0.f;
-                                r[6] += fnz ? df : 0.f;
-                                r[7] += fnz ? df * df : 0.f;
-                                r[8] += fabsf(tf);
-                                r[9] += fx * fx;
-                                r[10] += fx * fy;
-                                r[11] += fy * fy;
-                                r[12] += fx;
-                                r[13] += fy;
-                                r[14] += fx * df;
-                                r[15] += fy * df;
+                                r[2] += fx * d;
+                                r[3] += fy * d;
+                                if (fre) {
+                                    const float tf = sf ? fmr[FM ? k : 0] : 0.f;
+                                    const float df = tf - cf;
+                                    const bool fnz = fabsf(tf) > 1e-10f;
+                                    q6[0] += fnz ? 1.f : 0.f;
+                                    q6[1] += fnz ? df : 0.f;
+                                    q6[2] += fnz ? df * df : 0.f;
+                                    q6[3] += fabsf(tf);
+                                    q6[4] += fx * fx;
+                                    q6[5] += fx * fy;
+                                    q6[6] += fy * fy;
+                                    q6[7] += fx;
+                                    q6[8] += fy;
+                                    q6[9] += fx * df;
+                                    q6[10] += fy * df;
+                                }
                             }
                         }
+                        nr += (int)__popcll(__ballot(nz));
                     }
-                    n = wave_sum_dpp_f64((double)r[0]);
-                    S1 = wave_sum_dpp_f64((double)r[1]);
-                    S2 = wave_sum_dpp_f64((double)r[2]);
-                    if (MODE == 6) {
+                    n = nr;
+                    S1 = wave_sum_dpp(r[0]);
+                    S2 = wave_sum_dpp(r[1]);
 #pragma unroll
-                        for (int q = 0; q < 6; ++q) T[q] = wave_sum_dpp_f64((double)r[3 + q]);
-                    } else {
-                        T[0] = wave_sum_dpp_f64((double)r[3]);
-                        T[1] = wave_sum_dpp_f64((double)r[4]);
-                        // Subpb1 rebuilds f, grad f, H and the f statistics on
-                        // the region (funICGN_Subpb1.m:171-190); they persist
-                        const double nf = wave_sum_dpp_f64((double)r[5]);
-                        const double F1 = wave_sum_dpp_f64((double)r[6]);
-                        const double F2 = wave_sum_dpp_f64((double)r[7]);
-                        sabs = wave_sum_dpp_f64((double)r[8]);
-                        const double h11 = wave_sum_dpp(r[9]), h12 = wave_sum_dpp(r[10]),
-                                     h22 = wave_sum_dpp(r[11]);
+                    for (int q = 0; q < NJ; ++q) T[q] = wave_sum_dpp(r[2 + q]);
+                    if (fre) {
+                        const double nf = wave_sum_dpp_f64((double)q6[0]);
+                        const double F1 = wave_sum_dpp_f64((double)q6[1]);
+                        const double F2 = wave_sum_dpp_f64((double)q6[2]);
+                        sabs = wave_sum_dpp_f64((double)q6[3]);
+                        const double h11 = wave_sum_dpp(q6[4]), h12 = wave_sum_dpp(q6[5]),
+                                     h22 = wave_sum_dpp(q6[6]);
                         meanf = nf > 0 ? (double)cf + F1 / nf : NAN;
                         bottomf = nf >= 2 ? sqrt(fmax(F2 - F1 * F1 / nf, 0.0)) : (nf == 1 ? 0.0 : NAN);
                         const double dmf = (double)cf - meanf;
                         for (int q = 0; q < 2; ++q) {
-                            SJv[q] = wave_sum_dpp_f64((double)r[12 + q]);
-                            FJv[q] = wave_sum_dpp_f64((double)r[14 + q]) + dmf * SJv[q];
+                            SJv[q] = wave_sum_dpp_f64((double)q6[7 + q]);
+                            FJv[q] = wave_sum_dpp_f64((double)q6[9 + q]) + dmf * SJv[q];
                         }
                         __syncthreads();
                         set_h2(L, h11, h12, h22, bottomf, a.mu);
+                        if (lane == 0)
+                            for (int q = 0; q < 2; ++q) {
+                                L.SJ[q] = SJv[q];
+                                L.FJ[q] = FJv[q];
+                            }
                         __syncthreads();
                     }
                 }
@@ -802,7 +873,7 @@ __global__ __launch_bounds__(64) void icgn_wave_kernel(IcgnArgs a)
                 const bool fsl = MODE == 2 && fsel;
                 // g of pixel k under the current region
                 auto gsel = [&](int i, int j) {
-                    const float g = gather(i, j);
+                    const float g = gather_rt(i, j);
                     return (masked_it && !((L.colw[j] >> i) & 1ull)) ? 0.f : g;
                 };
                 // pass A: counts and first moments (2-DOF with the f side on
@@ -1000,6 +1071,8 @@ hipError_t launch_wave(const IcgnArgs &a, int grid, hipStream_t s)
 int icgn_wave_ppt(int npix_cap)
 {
     if (npix_cap <= 64 * 5) return 5;
+    if (npix_cap <= 64 * 6) return 6;
+    if (npix_cap <= 64 * 7) return 7;
     if (npix_cap <= 64 * 9) return 9;
     if (npix_cap <= 64 * 18) return 18;
     return 0;  // too large for the wave kernel
@@ -1010,12 +1083,16 @@ hipError_t icgn_wave_launch(const IcgnArgs &a, int mode, int ppt, int grid, hipS
     if (mode == 6) {
         switch (ppt) {
         case 5: return launch_wave<5, 6>(a, grid, s);
+        case 6: return launch_wave<6, 6>(a, grid, s);
+        case 7: return launch_wave<7, 6>(a, grid, s);
         case 9: return launch_wave<9, 6>(a, grid, s);
         case 18: return launch_wave<18, 6>(a, grid, s);
         }
     } else {
         switch (ppt) {
         case 5: return launch_wave<5, 2>(a, grid, s);
+        case 6: return launch_wave<6, 2>(a, grid, s);
+        case 7: return launch_wave<7, 2>(a, grid, s);
         case 9: return launch_wave<9, 2>(a, grid, s);
         case 18: return launch_wave<18, 2>(a, grid, s);
         }

@@ -65,6 +65,8 @@ hipError_t ba_interp2_launch(const void *F, long M, long N, long P, const void *
 hipError_t img_gradient_launch(const float *f, const float *mask, int H, int W, float *dfdx,
                                float *dfdy, hipStream_t s);
 hipError_t znssd_launch(const ZnssdArgs &a, hipStream_t s);
+hipError_t mask_class_launch(const FrameDev *frames, const int *slot, const double *coords,
+                             const int *wx, const int *wy, long N, int *out, hipStream_t s);
 hipError_t search_launch(const SearchArgs &a, long npoints, hipStream_t s);
 size_t search_lds_bytes(int w, int r1, int r2);
 

@@ -34,6 +34,17 @@ __device__ __forceinline__ T wave_allsum(T v)
 
 __device__ __forceinline__ bool wave_any(bool p) { return __ballot(p) != 0ull; }
 
+// Bits of bw connected to the seeds s (s a subset of bw) along the word: a
+// carry entering a run of ones at a seed runs to the run's top, so
+// (bw + s) ^ bw marks seed-to-top; the bit-reversed word gives seed-to-bottom.
+__device__ __forceinline__ unsigned long long run_fill(unsigned long long bw, unsigned long long s)
+{
+    const unsigned long long up = ((bw + s) ^ bw) & bw;
+    const unsigned long long rb = __builtin_bitreverse64(bw), rs = __builtin_bitreverse64(s);
+    const unsigned long long dn = __builtin_bitreverse64(((rb + rs) ^ rb) & rb);
+    return up | dn | s;
+}
+
 // Connected component (4- or 8-connected) through the seed of the window
 // whose column j is the 64-bit word bw held by lane j (bit i = row i).
 __device__ unsigned long long wave_flood(unsigned long long bw, int ny, int seed_r, int seed_c,
@@ -52,10 +63,8 @@ __device__ unsigned long long wave_flood(unsigned long long bw, int ny, int seed
             l |= (l << 1) | (l >> 1);
             r |= (r << 1) | (r >> 1);
         }
-        // propagate along the column until stable, then across
-        unsigned long long n = bw & (sel | (sel << 1) | (sel >> 1) | l | r);
-        unsigned long long m = n;
-        for (int s = 0; s < 6; ++s) m = bw & (m | (m << 1) | (m >> 1));
+        // across one column, then along whole columns at once
+        const unsigned long long m = run_fill(bw, bw & (sel | l | r));
         const bool ch = m != sel;
         sel = m;
         if (!wave_any(ch)) break;
@@ -725,15 +734,18 @@ __global__ __launch_bounds__(64) void icgn_wave_kernel(IcgnArgs a)
                 __syncthreads();
                 L.colw[lane] = sw;
                 __syncthreads();
-                sum_bw2 = wave_allsum((int)__popcll(sw));
                 selbits = 0u;
+                sum_bw2 = 0;
 #pragma unroll
                 for (int k = 0; k < PPT; ++k) {
                     const int p = lane + 64 * k;
+                    bool in = false;
                     if (p < npix) {
                         const int j = (int)(((float)p + 0.5f) * rnx), i = p - j * nx;
-                        if ((L.colw[j] >> i) & 1ull) selbits |= 1u << k;
+                        in = (L.colw[j] >> i) & 1ull;
                     }
+                    selbits |= in ? (1u << k) : 0u;
+                    sum_bw2 += (int)__popcll(__ballot(in));
                 }
                 nzprev = nzb;
                 region_valid = true;

@@ -54,4 +54,7 @@ def ref_frames():
     m1 = z["mask1"].astype(np.float64)
     return dict(f=norm(z["img0"]) * m0, g=norm(z["img1"]) * m1, mask=m0,
                 coords=z["coordinatesFEM"], winsize=int(z["winsize"]),
-                winsize_List=z["winsize_List"].astype(np.int64), U1=z["U1"], F1=z["F1"])
+                winsize_List=z["winsize_List"].astype(np.int64), U1=z["U1"], F1=z["F1"],
+                mesh=dict(coordinatesFEM=z["coordinatesFEM"], elementsFEM=z["elementsFEM"],
+                          dirichlet=z["dirichlet"], markCoordHoleEdge=z["markCoordHoleEdge"]),
+                winstepsize=float(z["winstepsize"]))

@@ -12,6 +12,9 @@ transpose on load, func_quadtree/ReadImageQuadtree.m:78 and ReadImageMasks.m:78)
   ImgRefMask      uint8 (H, W)      DICpara.ImgRefMask as stored in the .mat
   gridx, gridy    int64 (2,)        DICpara.gridxyROIRange
   coordinatesFEM  float64 (N, 2)    DICmesh.coordinatesFEM (N = 1434)
+  elementsFEM     int32 (E, 8)      DICmesh.elementsFEM (E = 938; columns 5-8 hanging mid-nodes)
+  dirichlet       int32 (D,)        DICmesh.dirichlet
+  markCoordHoleEdge int32 (K,)      DICmesh.markCoordHoleEdge
   winsize_List    float64 (N, 2)    DICpara.winsize_List
   winsize, winstepsize              DICpara scalars
   U1              float64 (2N,)     ResultDisp{1}.U  (frame 001 vs 000, after ADMM)
@@ -50,6 +53,9 @@ def main(ref):
         gridx=np.asarray(roi["gridx"], dtype=np.int64).ravel(),
         gridy=np.asarray(roi["gridy"], dtype=np.int64).ravel(),
         coordinatesFEM=np.asarray(M["coordinatesFEM"], dtype=np.float64),
+        elementsFEM=np.asarray(M["elementsFEM"], dtype=np.int32),
+        dirichlet=np.asarray(M["dirichlet"], dtype=np.int32).ravel(),
+        markCoordHoleEdge=np.asarray(M["markCoordHoleEdge"], dtype=np.int32).ravel(),
         winsize_List=np.asarray(P["winsize_List"], dtype=np.float64),
         winsize=np.float64(P["winsize"][0, 0]), winstepsize=np.float64(P["winstepsize"][0, 0]),
         U1=np.asarray(rd["U"], dtype=np.float64).ravel(),

@@ -49,7 +49,9 @@ typedef struct staq_params {
     int max_iter;    /* 100 in the reference (funICGNQuadtree.m:59)                      */
     int plain;       /* 1: uniform-grid twins funICGN / Subpb1 (no node rounding,
                         whole-window statistics, DfDx==0 mask rule)                      */
-    int block_size;  /* workgroup size 64/128/256/512; 0 = tuned or default              */
+    int block_size;  /* kernel form for every bin: 0 = the batch's (tuned) schedule;
+                        64/128/256/512 = workgroup kernel of that size; -1/-2/-4 = that
+                        many wavefronts per subset                                      */
     int persistent;  /* 1: persistent grid pulling subsets from a device work queue      */
 } staq_params;
 

@@ -104,10 +104,12 @@ struct Bin {
     int wxcap = 0, wycap = 0;
     long item0 = 0, n = 0;
     int npix_cap = 0, tile_cap = 0;
-    int bs[2] = {0, 0};  // [mode 6, mode 2]: 0 = wave kernel, else workgroup size
-    int ppt = 0;         // pixels per lane of the wave kernel (0: not eligible)
+    // [mode 6, mode 2] kernel form: 0 = one wavefront per subset, -n = n
+    // wavefronts per subset, > 0 = workgroup size of the workgroup kernel
+    int bs[2] = {0, 0};
+    int ppt = 0;         // pixels per lane of the one-wavefront kernel (0: not eligible)
+    int wave_npix = 0;   // window pixels the wavefront kernels must hold (grown windows included)
     int masked = 0;      // the bin's windows touch the mask (their window grows)
-    int wave_tile_cap = 0;
 };
 
 struct staq_batch {
@@ -572,8 +574,9 @@ int staq_batch_create(staq_ctx *ctx, staq_batch **out, long N, const double *coo
     // funICGNQuadtree.m:66-70 and funICGN_Subpb1.m:76-80); largest windows
     // first, so that the hardware dispatcher fills each bin's tail with short
     // subsets
-    std::vector<int> gppt(N, 0);
+    std::vector<int> gppt(N, 0), gnp(N, 0);
     for (long i = 0; i < N; ++i) {
+        gnp[i] = (hx[i] + 1) * (hy[i] + 1);
         if (cnt[i] == 0) continue;
         const double n0 = (double)(hx[i] + 1) * (hy[i] + 1);
         int gx = hx[i], gy = hy[i];
@@ -583,7 +586,8 @@ int staq_batch_create(staq_ctx *ctx, staq_batch **out, long N, const double *coo
             gy = std::max(gx, 2 * (int)std::ceil(std::sqrt(n1 / (n1 - cnt[i])) * hy[i] / 2.0));
             gx = std::max(gx, gy);
         }
-        const int pg = icgn_wave_ppt((gx + 1) * (gy + 1));
+        gnp[i] = (gx + 1) * (gy + 1);
+        const int pg = icgn_wave_ppt(gnp[i], 1);
         gppt[i] = pg ? pg : 31;
     }
     auto key = [&](int i) {
@@ -599,6 +603,7 @@ int staq_batch_create(staq_ctx *ctx, staq_batch **out, long N, const double *coo
         while (i1 < N && key(order[i1]) == k) {
             bin.wxcap = std::max(bin.wxcap, hx[order[i1]]);
             bin.wycap = std::max(bin.wycap, hy[order[i1]]);
+            bin.wave_npix = std::max(bin.wave_npix, gnp[order[i1]]);
             ++i1;
         }
         bin.item0 = i0;
@@ -609,12 +614,8 @@ int staq_batch_create(staq_ctx *ctx, staq_batch **out, long N, const double *coo
         // the wave kernel of a bin that touches the mask covers the grown
         // windows of its smallest class too; a subset that outgrows its bin
         // (or grows although classified clear) defers to the fallback launch
-        bin.ppt = icgn_wave_ppt(bin.npix_cap);
-        if (bin.masked && (k & 31) != 31) bin.ppt = std::max(bin.ppt, k & 31);
-        if (bin.ppt) {
-            const int side = tile_side((int)std::sqrt(64.0 * bin.ppt));
-            bin.wave_tile_cap = side * side;
-        }
+        bin.wave_npix = std::max(bin.wave_npix, bin.npix_cap);
+        bin.ppt = icgn_wave_ppt(bin.wave_npix, 1);
         bin.bs[0] = bin.bs[1] = bin.ppt ? 0 : default_bs(bin.npix_cap);
         b->bins.push_back(bin);
         fbx = std::max(fbx, grown_window(bin.wxcap));
@@ -728,7 +729,10 @@ int staq_batch_bins(staq_batch *b, int *nbins, int *wcap, long *count, int *bs6,
         if (count) count[k] = b->bins[k].n;
         if (bs6) bs6[k] = b->bins[k].bs[0];
         if (bs2) bs2[k] = b->bins[k].bs[1];
-        if (ppt) ppt[k] = b->bins[k].ppt;
+        if (ppt) {
+            const int f = b->bins[k].bs[0];
+            ppt[k] = f <= 0 ? icgn_wave_ppt(b->bins[k].wave_npix, f == 0 ? 1 : -f) : 0;
+        }
     }
     return STAQ_OK;
 }
@@ -787,8 +791,10 @@ static int occupancy_grid(int bs, size_t lds, long n)
 // and defer flag, the window bins (concurrently when side streams exist), and
 // the fallback for windows that outgrew their bin.  bin_only >= 0 launches a
 // single bin on s (autotuning), without fallback.
+static constexpr int kNoOverride = 1 << 30;
+
 static int enqueue_step(staq_batch *b, const staq_params *p, int mode, hipStream_t s,
-                        int bin_only = -1, int bs_override = -1, bool timed = false)
+                        int bin_only = -1, int bs_override = kNoOverride, bool timed = false)
 {
     staq_ctx *ctx = b->ctx;
     const int nb = (int)b->bins.size();
@@ -814,18 +820,26 @@ static int enqueue_step(staq_batch *b, const staq_params *p, int mode, hipStream
         a.tile_cap = bin.tile_cap;
         a.defer = 1;
         a.defer_out = ints + nb + 1;
-        int bs = bs_override >= 0 ? bs_override : (p->block_size ? p->block_size : bin.bs[mi]);
-        if (bs == 0 && !bin.ppt) bs = default_bs(bin.npix_cap);
+        int bs = bs_override != kNoOverride ? bs_override
+                                             : (p->block_size ? p->block_size : bin.bs[mi]);
+        if (!(bs == 0 || bs == -1 || bs == -2 || bs == -4 || bs == 64 || bs == 128 || bs == 256 ||
+              bs == 512))
+            return fail(ctx, STAQ_EINVAL, "block_size must be 0, -1, -2, -4, 64, 128, 256 or 512");
+        const int nw = bs <= 0 ? (bs == 0 ? 1 : -bs) : 0;
+        const int wppt = nw ? icgn_wave_ppt(bin.wave_npix, nw) : 0;
+        if (bs <= 0 && !wppt) bs = default_bs(bin.npix_cap);
         int grid = (int)bin.n;
         if (timed) HIPCHK(ctx, hipEventRecord(b->tev[2 * k], sk));
-        if (bs == 0) {
-            // one wavefront per subset
-            a.tile_cap = bin.wave_tile_cap;
+        if (bs <= 0) {
+            // nw wavefronts per subset; the g tile holds the warped window
+            // of the largest window the kernel takes, plus its margin
+            const int side = tile_side((int)std::sqrt(64.0 * wppt * nw));
+            a.tile_cap = side * side;
             if (p->persistent) {
                 a.work_counter = ints + k;
-                grid = occupancy_grid(64, sizeof(float) * a.tile_cap + 2048, bin.n);
+                grid = occupancy_grid(64 * nw, sizeof(float) * a.tile_cap + 8192, bin.n);
             }
-            HIPCHK(ctx, icgn_wave_launch(a, mode, bin.ppt, grid, sk));
+            HIPCHK(ctx, icgn_wave_launch(a, mode, wppt, nw, grid, sk));
         } else {
             if (p->persistent) {
                 a.work_counter = ints + k;
@@ -888,7 +902,7 @@ int staq_batch_step_timed(staq_batch *b, const staq_params *p, int mode, float *
         HIPCHK(ctx, hipEventCreate(&e));
         b->tev.push_back(e);
     }
-    int rc = enqueue_step(b, p, mode, ctx->stream, -1, -1, true);
+    int rc = enqueue_step(b, p, mode, ctx->stream, -1, kNoOverride, true);
     if (rc != STAQ_OK) return rc;
     HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
     for (int k = 0; k <= nb; ++k)
@@ -905,7 +919,7 @@ int staq_batch_icgn6_per_subset(staq_batch *b, const staq_params *p, double *ker
     // Every subset gets the largest caps so that no subset defers: this is the
     // reference's one-subset-per-call structure with one workgroup per call.
     const int npix_cap = b->fb_npix_cap, tile_cap = b->fb_tile_cap;
-    const int bs = p->block_size ? p->block_size : 256;
+    const int bs = p->block_size > 0 ? p->block_size : 256;
     hipEvent_t e0 = nullptr, e1 = nullptr;
     if (kernel_ms_sum) {
         *kernel_ms_sum = 0.0;
@@ -974,55 +988,66 @@ int staq_batch_chain_outputs(staq_batch *b)
 
 int staq_batch_autotune(staq_batch *b, const staq_params *p, int mode)
 {
-    // Coordinate descent over the window bins: for each bin try the wavefront
-    // kernel and each workgroup size, timing the whole step (all bins run
-    // concurrently and the fallback catches deferred windows, so a bin's
-    // choice is judged by its effect on the step), keep the fastest.
+    // Coordinate descent over the window bins: for each bin try the kernel
+    // forms (one, two or four wavefronts per subset, each workgroup size).
+    // The bins of a step run concurrently, so a step lasts as long as its
+    // longest launch: each form is judged by the longest launch of an eager
+    // step, every launch timed by events on its own stream (the host's launch
+    // gaps do not enter), median of three steps; a form replaces the current
+    // one only if it is faster by more than the timing noise.
     if (!b || !p || (mode != 6 && mode != 2)) return STAQ_EINVAL;
     staq_ctx *ctx = b->ctx;
     hipSetDevice(ctx->device);
     hipStream_t s = ctx->stream;
-    hipEvent_t e0, e1;
-    HIPCHK(ctx, hipEventCreate(&e0));
-    HIPCHK(ctx, hipEventCreate(&e1));
-    const int cand[5] = {0, 64, 128, 256, 512};
+    const int nb = (int)b->bins.size();
+    while ((int)b->tev.size() < 2 * nb + 2) {
+        hipEvent_t e;
+        HIPCHK(ctx, hipEventCreate(&e));
+        b->tev.push_back(e);
+    }
+    const int cand[7] = {0, -2, -4, 64, 128, 256, 512};
     const int mi = mode == 6 ? 0 : 1;
     staq_params q = *p;
     q.block_size = 0;
     int rc = STAQ_OK;
     auto time_step = [&](float *ms) {
-        float t[3];
+        float t[3] = {0, 0, 0};
         for (int r = 0; r < 3 && rc == STAQ_OK; ++r) {
-            hipEventRecord(e0, s);
-            rc = enqueue_step(b, &q, mode, s);
-            hipEventRecord(e1, s);
-            hipEventSynchronize(e1);
-            hipEventElapsedTime(&t[r], e0, e1);
+            rc = enqueue_step(b, &q, mode, s, -1, kNoOverride, true);
+            if (rc != STAQ_OK) break;
+            hipError_t e = hipStreamSynchronize(s);
+            for (int k = 0; k <= nb && e == hipSuccess; ++k) {
+                float d = 0.f;
+                e = hipEventElapsedTime(&d, b->tev[2 * k], b->tev[2 * k + 1]);
+                t[r] = std::max(t[r], d);
+            }
+            if (e != hipSuccess) rc = hipfail(ctx, e, "autotune timing");
         }
         std::sort(t, t + 3);
         *ms = t[1];
     };
     for (int k = 0; k < (int)b->bins.size() && rc == STAQ_OK; ++k) {
         Bin &bin = b->bins[k];
-        float best = 1e30f;
-        int best_bs = bin.bs[mi];
+        const int cur = bin.bs[mi];
+        float best = 0.f;
+        time_step(&best);
+        int best_bs = cur;
         for (int c : cand) {
-            if (c == 0 && !bin.ppt) continue;
+            if (c == cur) continue;
+            if (c <= 0 && !icgn_wave_ppt(bin.wave_npix, c == 0 ? 1 : -c)) continue;
             if (c == 512 && bin.npix_cap < 1500) continue;
             if (c == 64 && bin.npix_cap > 4500) continue;
             bin.bs[mi] = c;
             float ms = 0.f;
             time_step(&ms);
             if (rc != STAQ_OK) break;
-            if (ms < best) {
+            if (ms < 0.97f * best) {
                 best = ms;
                 best_bs = c;
             }
         }
         bin.bs[mi] = best_bs;
     }
-    hipEventDestroy(e0);
-    hipEventDestroy(e1);
     if (b->gexec) {  // a captured graph holds the old schedule
         hipGraphExecDestroy(b->gexec);
         hipGraphDestroy(b->graph);

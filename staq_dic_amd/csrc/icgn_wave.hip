@@ -50,7 +50,7 @@ __device__ __forceinline__ unsigned long long run_fill(unsigned long long bw, un
 __device__ unsigned long long wave_flood(unsigned long long bw, int ny, int seed_r, int seed_c,
                                          int conn)
 {
-    const int lane = threadIdx.x;
+    const int lane = threadIdx.x & 63;
     unsigned long long sel = 0ull;
     if (lane < ny && seed_c >= 1 && seed_c <= ny && seed_r >= 1 && seed_r <= 64 &&
         lane == seed_c - 1)
@@ -111,7 +111,60 @@ struct WaveLds {
     unsigned long long colw[64];  // scratch column words (atomic OR target)
     unsigned long long tmw[64];   // window-mask columns
     unsigned long long sel0w[64]; // setup region (4-connected mask region)
+    // exchange between the wavefronts of one subset (double-buffered)
+    float xf[2][4][24];
+    double xd[2][4][16];
+    int xi[2][4][8];
+    unsigned long long bal[64];   // per pixel slot ballots of nonzero g
 };
+
+// Subset-wide sums over the NW wavefronts of a workgroup: f and d hold each
+// lane's partials (reduced within the wavefront by DPP), iv each wavefront's
+// own totals.  Every wavefront adds the same partials in the same order, so
+// all of them hold bit-identical totals afterwards; the buffers alternate, so
+// one barrier per exchange suffices.
+template <int NW, int NF, int ND, int NI>
+__device__ __forceinline__ void group_reduce(WaveLds &L, int &par, float *f, double *d, int *iv)
+{
+#pragma unroll
+    for (int q = 0; q < NF; ++q) f[q] = wave_sum_dpp(f[q]);
+#pragma unroll
+    for (int q = 0; q < ND; ++q) d[q] = wave_sum_dpp_f64(d[q]);
+    if constexpr (NW > 1) {
+        const int w = threadIdx.x >> 6;
+        if ((threadIdx.x & 63) == 0) {
+#pragma unroll
+            for (int q = 0; q < NF; ++q) L.xf[par][w][q] = f[q];
+#pragma unroll
+            for (int q = 0; q < ND; ++q) L.xd[par][w][q] = d[q];
+#pragma unroll
+            for (int q = 0; q < NI; ++q) L.xi[par][w][q] = iv[q];
+        }
+        __syncthreads();
+#pragma unroll
+        for (int q = 0; q < NF; ++q) {
+            float t = L.xf[par][0][q];
+#pragma unroll
+            for (int ww = 1; ww < NW; ++ww) t += L.xf[par][ww][q];
+            f[q] = t;
+        }
+#pragma unroll
+        for (int q = 0; q < ND; ++q) {
+            double t = L.xd[par][0][q];
+#pragma unroll
+            for (int ww = 1; ww < NW; ++ww) t += L.xd[par][ww][q];
+            d[q] = t;
+        }
+#pragma unroll
+        for (int q = 0; q < NI; ++q) {
+            int t = L.xi[par][0][q];
+#pragma unroll
+            for (int ww = 1; ww < NW; ++ww) t += L.xi[par][ww][q];
+            iv[q] = t;
+        }
+        par ^= 1;
+    }
+}
 
 // Solve H d = -b with the factor in LDS (all lanes read it with broadcast loads).
 __device__ __forceinline__ void solve6_lds(const WaveLds &L, const double (&b)[6], double (&d)[6])
@@ -168,21 +221,26 @@ __device__ __forceinline__ void set_h2(WaveLds &L, double h11, double h12, doubl
     }
 }
 
-template <int PPT, int MODE>
-__global__ __launch_bounds__(64) void icgn_wave_kernel(IcgnArgs a)
+template <int PPT, int MODE, int NW>
+__global__ __launch_bounds__(64 * NW) void icgn_wave_kernel(IcgnArgs a)
 {
     constexpr int NJ = MODE;
-    constexpr int CAP = 64 * PPT;
+    constexpr int CAP = 64 * PPT * NW;
     constexpr bool BRANCH_FREE = PPT <= 9;
     extern __shared__ __align__(16) unsigned char smem[];
     __shared__ WaveLds L;
     __shared__ long item_sh;
     float *tile = reinterpret_cast<float *>(smem);
-    const int lane = threadIdx.x;
+    const int tid = threadIdx.x;
+    const int lane = tid & 63;   // lane within the wavefront
+    const int wv = tid >> 6;     // wavefront of the subset
     const bool quad = !(MODE == 6 && a.plain);
+    int par = 0;                 // exchange buffer parity
+    // pixel slot k of this lane: slots are interleaved over the wavefronts
+    auto pixel = [&](int k) { return lane + 64 * (k * NW + wv); };
 
     for (long pass = 0;; ++pass) {
-        if (lane == 0)
+        if (tid == 0)
             item_sh = a.work_counter ? a.item0 + atomicAdd(a.work_counter, 1)
                                      : (pass == 0 ? a.item0 + blockIdx.x : a.item0 + a.n_items);
         __syncthreads();
@@ -235,7 +293,7 @@ __global__ __launch_bounds__(64) void icgn_wave_kernel(IcgnArgs a)
             const float rnx = 1.0f / (float)nx;
 #pragma unroll
             for (int k = 0; k < PPT; ++k) {
-                const int p = lane + 64 * k;
+                const int p = pixel(k);
                 if (p < npix) {
                     const int j = (int)(((float)p + 0.5f) * rnx), i = p - j * nx;
                     const int x = xa + i, y = ya + j;
@@ -245,7 +303,11 @@ __global__ __launch_bounds__(64) void icgn_wave_kernel(IcgnArgs a)
                         cnt += fidx(fr.f, fr.H, x, y) * fidx(fr.mask, fr.H, x, y) == 0.f;
                 }
             }
-            count = wave_allsum(cnt);
+            {
+                int ci[1] = {(int)wave_sum_dpp((float)cnt)};
+                group_reduce<NW, 0, 0, 1>(L, par, nullptr, nullptr, ci);
+                count = ci[0];
+            }
             const double n0 = (double)nx * ny;
             const double cap = (MODE == 6 && a.plain) ? 0.5 * n0 : 0.4 * n0;
             if (!(count < cap)) {
@@ -298,7 +360,7 @@ __global__ __launch_bounds__(64) void icgn_wave_kernel(IcgnArgs a)
             __syncthreads();
 #pragma unroll
             for (int k = 0; k < PPT; ++k) {
-                const int p = lane + 64 * k;
+                const int p = pixel(k);
                 if (p < npix) {
                     const int j = (int)(((float)p + 0.5f) * rnx), i = p - j * nx;
                     if (fidx(fr.mask, fr.H, xa + i, ya + j) != 0.f) atomicOr(&L.colw[j], 1ull << i);
@@ -306,8 +368,12 @@ __global__ __launch_bounds__(64) void icgn_wave_kernel(IcgnArgs a)
             }
             __syncthreads();
             const unsigned long long tm = L.colw[lane];
-            L.tmw[lane] = tm;
-            L.sel0w[lane] = wave_flood(tm, ny, seed_r, seed_c, 4);
+            const unsigned long long s0w = wave_flood(tm, ny, seed_r, seed_c, 4);
+            __syncthreads();
+            if (wv == 0) {
+                L.tmw[lane] = tm;
+                L.sel0w[lane] = s0w;
+            }
             __syncthreads();
         }
 
@@ -336,7 +402,7 @@ __global__ __launch_bounds__(64) void icgn_wave_kernel(IcgnArgs a)
             for (int q = 0; q < NH; ++q) hs[q] = 0.f;
 #pragma unroll
             for (int k = 0; k < PPT; ++k) {
-                const int p = lane + 64 * k;
+                const int p = pixel(k);
                 fxr[k] = fyr[k] = 0.f;
                 if (FM) fmr[FM ? k : 0] = 0.f;
                 if (p < npix) {
@@ -380,19 +446,28 @@ __global__ __launch_bounds__(64) void icgn_wave_kernel(IcgnArgs a)
                     }
                 }
             }
-            const double n = wave_sum_dpp_f64((double)s_n);
-            const double S1 = wave_sum_dpp_f64((double)s_1), S2 = wave_sum_dpp_f64((double)s_2);
-            sabs = wave_sum_dpp_f64((double)s_abs);
+            double sd[4 + 2 * NJ];
+            sd[0] = s_n;
+            sd[1] = s_1;
+            sd[2] = s_2;
+            sd[3] = s_abs;
+#pragma unroll
+            for (int q = 0; q < NJ; ++q) {
+                sd[4 + q] = sj[q];
+                sd[4 + NJ + q] = sjf[q];
+            }
+            group_reduce<NW, NH, 4 + 2 * NJ, 0>(L, par, hs, sd, nullptr);
+            const double n = sd[0];
+            const double S1 = sd[1], S2 = sd[2];
+            sabs = sd[3];
             meanf = n > 0 ? (double)c0 + S1 / n : NAN;
             bottomf = n >= 2 ? sqrt(fmax(S2 - S1 * S1 / n, 0.0)) : (n == 1 ? 0.0 : NAN);
             const double dm = (double)c0 - meanf;
 #pragma unroll
-            for (int q = 0; q < NH; ++q) hs[q] = wave_sum_dpp(hs[q]);
-#pragma unroll
             for (int q = 0; q < NJ; ++q) {
-                const double SJq = wave_sum_dpp_f64((double)sj[q]);
-                const double SJFq = wave_sum_dpp_f64((double)sjf[q]);
-                if (lane == 0) {
+                const double SJq = sd[4 + q];
+                const double SJFq = sd[4 + NJ + q];
+                if (tid == 0) {
                     L.SJ[q] = SJq;
                     L.FJ[q] = SJFq + dm * SJq;  // sum J (f - meanf)
                 }
@@ -401,10 +476,10 @@ __global__ __launch_bounds__(64) void icgn_wave_kernel(IcgnArgs a)
                 double h[21];
 #pragma unroll
                 for (int q = 0; q < 21; ++q) h[q] = hs[q];
-                if (lane == 0)
+                if (tid == 0)
                     for (int q = 0; q < 21; ++q) a.H21[idx + (long)q * a.N] = h[q];
                 const bool spd = chol6(h);
-                if (lane == 0) {
+                if (tid == 0) {
                     if (spd) {
                         for (int q = 0; q < 21; ++q) L.Lp[q] = h[q];
                         for (int c = 0; c < 6; ++c) L.Ld[c] = 1.0 / h[c * 6 - (c * (c - 1)) / 2];
@@ -563,12 +638,12 @@ __global__ __launch_bounds__(64) void icgn_wave_kernel(IcgnArgs a)
                 // flat over the tile, 8 independent loads in flight per lane
                 const int total = tnx * tny;
                 const float rcpn = 1.0f / tnx;
-                for (int t0 = 0; t0 < total; t0 += 64 * 8) {
+                for (int t0 = 0; t0 < total; t0 += 64 * NW * 8) {
                     float v[8];
                     int o[8];
 #pragma unroll
                     for (int q = 0; q < 8; ++q) {
-                        const int t = t0 + lane + 64 * q;
+                        const int t = t0 + tid + 64 * NW * q;
                         o[q] = t;
                         v[q] = 0.f;
                         if (t < total) {
@@ -642,7 +717,7 @@ __global__ __launch_bounds__(64) void icgn_wave_kernel(IcgnArgs a)
             auto fused_pass = [&](auto from_tile) {
 #pragma unroll
                 for (int k = 0; k < PPT; ++k) {
-                    const int p = lane + 64 * k;
+                    const int p = pixel(k);
                     const bool valid = p < npix;
                     bool nz = false;
                     if (BRANCH_FREE || valid) {
@@ -680,14 +755,23 @@ __global__ __launch_bounds__(64) void icgn_wave_kernel(IcgnArgs a)
                 fused_pass(std::true_type{});
             else
                 fused_pass(std::false_type{});
-            if (quad && wave_any(any_nan)) {
+            int xv[4] = {n_nz, wave_any(any_zero) ? 1 : 0, wave_any(any_nan) ? 1 : 0,
+                         wave_any(nzb != nzprev) ? 1 : 0};
+            // one wavefront: the sums wait until it is known that they stand;
+            // several: they travel with the votes, in one exchange
+            if constexpr (NW == 1)
+                group_reduce<NW, 0, 0, 4>(L, par, nullptr, nullptr, xv);
+            else
+                group_reduce<NW, 2 + NJ, 0, 4>(L, par, acc, nullptr, xv);
+            n_nz = xv[0];
+            if (quad && xv[2] > 0) {
                 state = 2;
                 status = ST_ERROR;  // logical(NaN) throws
                 break;
             }
-            const bool masked_it = quad && (count > 0 || wave_any(any_zero));
+            const bool masked_it = quad && (count > 0 || xv[1] > 0);
             bool region_new = false;
-            if (masked_it && !(region_valid && !wave_any(nzb != nzprev))) {
+            if (masked_it && !(region_valid && xv[3] == 0)) {
                 // region of nonzero g through the seed: 8-connected for the
                 // 6-DOF solver (funICGNQuadtree.m:171), 4-connected for
                 // Subpb1 (funICGN_Subpb1.m:160); if that is the whole window,
@@ -700,11 +784,25 @@ __global__ __launch_bounds__(64) void icgn_wave_kernel(IcgnArgs a)
                 {
                     const int sb = lane * nx, wk = sb >> 6, off = sb & 63;
                     unsigned long long lo = 0ull, hi = 0ull;
+                    if constexpr (NW == 1) {
 #pragma unroll
-                    for (int k = 0; k < PPT; ++k) {
-                        const unsigned long long bk = __ballot((nzb >> k) & 1u);
-                        lo = wk == k ? bk : lo;
-                        hi = wk + 1 == k ? bk : hi;
+                        for (int k = 0; k < PPT; ++k) {
+                            const unsigned long long bk = __ballot((nzb >> k) & 1u);
+                            lo = wk == k ? bk : lo;
+                            hi = wk + 1 == k ? bk : hi;
+                        }
+                    } else {
+#pragma unroll
+                        for (int k = 0; k < PPT; ++k) {
+                            const unsigned long long bk = __ballot((nzb >> k) & 1u);
+                            if (lane == 0) L.bal[k * NW + wv] = bk;
+                        }
+                        if (lane == 0) L.bal[PPT * NW] = 0ull;
+                        __syncthreads();
+                        if (wk < PPT * NW) {
+                            lo = L.bal[wk];
+                            hi = L.bal[wk + 1];
+                        }
                     }
                     colnz = off ? ((lo >> off) | (hi << (64 - off))) : lo;
                     colnz = lane < ny ? (colnz & full) : 0ull;
@@ -717,7 +815,7 @@ __global__ __launch_bounds__(64) void icgn_wave_kernel(IcgnArgs a)
                         __syncthreads();
 #pragma unroll
                         for (int k = 0; k < PPT; ++k) {
-                            const int p = lane + 64 * k;
+                            const int p = pixel(k);
                             if (p < npix) {
                                 const int j = (int)(((float)p + 0.5f) * rnx), i = p - j * nx;
                                 if (fidx(fr.mask, fr.H, xa + i, ya + j) != 0.f)
@@ -732,13 +830,13 @@ __global__ __launch_bounds__(64) void icgn_wave_kernel(IcgnArgs a)
                 }
                 if (lane >= ny) sw = 0ull;
                 __syncthreads();
-                L.colw[lane] = sw;
+                if (wv == 0) L.colw[lane] = sw;
                 __syncthreads();
                 selbits = 0u;
                 sum_bw2 = 0;
 #pragma unroll
                 for (int k = 0; k < PPT; ++k) {
-                    const int p = lane + 64 * k;
+                    const int p = pixel(k);
                     bool in = false;
                     if (p < npix) {
                         const int j = (int)(((float)p + 0.5f) * rnx), i = p - j * nx;
@@ -746,6 +844,11 @@ __global__ __launch_bounds__(64) void icgn_wave_kernel(IcgnArgs a)
                     }
                     selbits |= in ? (1u << k) : 0u;
                     sum_bw2 += (int)__popcll(__ballot(in));
+                }
+                {
+                    int sb2[1] = {sum_bw2};
+                    group_reduce<NW, 0, 0, 1>(L, par, nullptr, nullptr, sb2);
+                    sum_bw2 = sb2[0];
                 }
                 nzprev = nzb;
                 region_valid = true;
@@ -770,8 +873,7 @@ __global__ __launch_bounds__(64) void icgn_wave_kernel(IcgnArgs a)
                     FJv[q] = L.FJ[q];
                 }
                 if (!resum) {
-#pragma unroll
-                    for (int q = 0; q < 2 + NJ; ++q) acc[q] = wave_sum_dpp(acc[q]);
+                    if constexpr (NW == 1) group_reduce<NW, 2 + NJ, 0, 0>(L, par, acc, nullptr, nullptr);
                     n = n_nz;
                     S1 = acc[0];
                     S2 = acc[1];
@@ -793,7 +895,7 @@ __global__ __launch_bounds__(64) void icgn_wave_kernel(IcgnArgs a)
                     int nr = 0;
 #pragma unroll
                     for (int k = 0; k < PPT; ++k) {
-                        const int p = lane + 64 * k;
+                        const int p = pixel(k);
                         bool nz = false;
                         if (p < npix) {
                             const bool sg = !masked_it || ((selbits >> k) & 1u);
@@ -837,28 +939,35 @@ __global__ __launch_bounds__(64) void icgn_wave_kernel(IcgnArgs a)
                         }
                         nr += (int)__popcll(__ballot(nz));
                     }
+                    {
+                        int ni[1] = {nr};
+                        group_reduce<NW, 2 + NJ, 0, 1>(L, par, r, nullptr, ni);
+                        nr = ni[0];
+                    }
                     n = nr;
-                    S1 = wave_sum_dpp(r[0]);
-                    S2 = wave_sum_dpp(r[1]);
+                    S1 = r[0];
+                    S2 = r[1];
 #pragma unroll
-                    for (int q = 0; q < NJ; ++q) T[q] = wave_sum_dpp(r[2 + q]);
+                    for (int q = 0; q < NJ; ++q) T[q] = r[2 + q];
                     if (fre) {
-                        const double nf = wave_sum_dpp_f64((double)q6[0]);
-                        const double F1 = wave_sum_dpp_f64((double)q6[1]);
-                        const double F2 = wave_sum_dpp_f64((double)q6[2]);
-                        sabs = wave_sum_dpp_f64((double)q6[3]);
-                        const double h11 = wave_sum_dpp(q6[4]), h12 = wave_sum_dpp(q6[5]),
-                                     h22 = wave_sum_dpp(q6[6]);
+                        double fd[8] = {q6[0], q6[1], q6[2], q6[3], q6[7], q6[8], q6[9], q6[10]};
+                        float fh[3] = {q6[4], q6[5], q6[6]};
+                        group_reduce<NW, 3, 8, 0>(L, par, fh, fd, nullptr);
+                        const double nf = fd[0];
+                        const double F1 = fd[1];
+                        const double F2 = fd[2];
+                        sabs = fd[3];
+                        const double h11 = fh[0], h12 = fh[1], h22 = fh[2];
                         meanf = nf > 0 ? (double)cf + F1 / nf : NAN;
                         bottomf = nf >= 2 ? sqrt(fmax(F2 - F1 * F1 / nf, 0.0)) : (nf == 1 ? 0.0 : NAN);
                         const double dmf = (double)cf - meanf;
                         for (int q = 0; q < 2; ++q) {
-                            SJv[q] = wave_sum_dpp_f64((double)q6[7 + q]);
-                            FJv[q] = wave_sum_dpp_f64((double)q6[9 + q]) + dmf * SJv[q];
+                            SJv[q] = fd[4 + q];
+                            FJv[q] = fd[6 + q] + dmf * SJv[q];
                         }
                         __syncthreads();
                         set_h2(L, h11, h12, h22, bottomf, a.mu);
-                        if (lane == 0)
+                        if (tid == 0)
                             for (int q = 0; q < 2; ++q) {
                                 L.SJ[q] = SJv[q];
                                 L.FJ[q] = FJv[q];
@@ -893,7 +1002,7 @@ __global__ __launch_bounds__(64) void icgn_wave_kernel(IcgnArgs a)
                 double v[7] = {0, 0, 0, 0, 0, 0, 0};
 #pragma unroll 3
                 for (int k = 0; k < PPT; ++k) {
-                    const int p = lane + 64 * k;
+                    const int p = pixel(k);
                     if (p < npix) {
                         const int j = (int)(((float)p + 0.5f) * rnx), i = p - j * nx;
                         const double g = gsel(i, j);
@@ -917,21 +1026,22 @@ __global__ __launch_bounds__(64) void icgn_wave_kernel(IcgnArgs a)
                         }
                     }
                 }
-                const double ng = wave_sum_dpp_f64(v[0]);
-                const double mg = ng > 0 ? wave_sum_dpp_f64(v[1]) / ng : NAN;
+                group_reduce<NW, 0, 7, 0>(L, par, nullptr, v, nullptr);
+                const double ng = v[0];
+                const double mg = ng > 0 ? v[1] / ng : NAN;
                 double mf = meanf, bfx = bottomf, nf = 0, h11 = 0, h12 = 0, h22 = 0;
                 if (fsl) {
-                    nf = wave_sum_dpp_f64(v[2]);
-                    mf = nf > 0 ? wave_sum_dpp_f64(v[3]) / nf : NAN;
-                    h11 = wave_sum_dpp_f64(v[4]);
-                    h12 = wave_sum_dpp_f64(v[5]);
-                    h22 = wave_sum_dpp_f64(v[6]);
+                    nf = v[2];
+                    mf = nf > 0 ? v[3] / nf : NAN;
+                    h11 = v[4];
+                    h12 = v[5];
+                    h22 = v[6];
                 }
                 // pass A2: second moments about the means
                 double s2g = 0.0, s2f = 0.0, sa = 0.0;
 #pragma unroll 3
                 for (int k = 0; k < PPT; ++k) {
-                    const int p = lane + 64 * k;
+                    const int p = pixel(k);
                     if (p < npix) {
                         const int j = (int)(((float)p + 0.5f) * rnx), i = p - j * nx;
                         const double g = gsel(i, j);
@@ -944,11 +1054,16 @@ __global__ __launch_bounds__(64) void icgn_wave_kernel(IcgnArgs a)
                         }
                     }
                 }
-                s2g = wave_sum_dpp_f64(s2g);
+                {
+                    double t3[3] = {s2g, s2f, sa};
+                    group_reduce<NW, 0, 3, 0>(L, par, nullptr, t3, nullptr);
+                    s2g = t3[0];
+                    s2f = t3[1];
+                    sa = t3[2];
+                }
                 const double bg = ng >= 2 ? sqrt(s2g) : (ng == 1 ? 0.0 : NAN);
                 if (fsl) {
-                    s2f = wave_sum_dpp_f64(s2f);
-                    sabs = wave_sum_dpp_f64(sa);
+                    sabs = sa;
                     bfx = nf >= 2 ? sqrt(s2f) : (nf == 1 ? 0.0 : NAN);
                     meanf = mf;
                     bottomf = bfx;
@@ -960,7 +1075,7 @@ __global__ __launch_bounds__(64) void icgn_wave_kernel(IcgnArgs a)
                 double vb[6] = {0, 0, 0, 0, 0, 0};
 #pragma unroll 3
                 for (int k = 0; k < PPT; ++k) {
-                    const int p = lane + 64 * k;
+                    const int p = pixel(k);
                     if (p < npix) {
                         const int j = (int)(((float)p + 0.5f) * rnx), i = p - j * nx;
                         const int x = xa + i, y = ya + j;
@@ -995,8 +1110,9 @@ __global__ __launch_bounds__(64) void icgn_wave_kernel(IcgnArgs a)
                         }
                     }
                 }
+                group_reduce<NW, 0, NJ, 0>(L, par, nullptr, vb, nullptr);
 #pragma unroll
-                for (int q = 0; q < NJ; ++q) b[q] = bfx * wave_sum_dpp_f64(vb[q]);
+                for (int q = 0; q < NJ; ++q) b[q] = bfx * vb[q];
             }
             // norm test, Gauss-Newton step, inverse-compositional update
             double nb2 = 0.0;
@@ -1041,7 +1157,7 @@ __global__ __launch_bounds__(64) void icgn_wave_kernel(IcgnArgs a)
             if (quad && sabs < 1e-6) st = a.max_it + 3;
             status = st;
         }
-        if (lane == 0) {
+        if (tid == 0) {
             if (state == 3) {
                 a.status[idx] = ST_DEFER;
                 if (a.defer_out) atomicOr(a.defer_out, 1);
@@ -1070,44 +1186,69 @@ __global__ __launch_bounds__(64) void icgn_wave_kernel(IcgnArgs a)
     }
 }
 
-template <int PPT, int MODE>
+template <int PPT, int MODE, int NW>
 hipError_t launch_wave(const IcgnArgs &a, int grid, hipStream_t s)
 {
     const size_t lds = sizeof(float) * (size_t)a.tile_cap;
-    hipLaunchKernelGGL((icgn_wave_kernel<PPT, MODE>), dim3(grid), dim3(64), lds, s, a);
+    hipLaunchKernelGGL((icgn_wave_kernel<PPT, MODE, NW>), dim3(grid), dim3(64 * NW), lds, s, a);
     return hipGetLastError();
+}
+
+// pixel slots per lane available for each number of wavefronts per subset
+template <int MODE, int NW>
+hipError_t launch_wave_ppt(const IcgnArgs &a, int ppt, int grid, hipStream_t s)
+{
+    if constexpr (NW == 1) {
+        switch (ppt) {
+        case 5: return launch_wave<5, MODE, 1>(a, grid, s);
+        case 6: return launch_wave<6, MODE, 1>(a, grid, s);
+        case 7: return launch_wave<7, MODE, 1>(a, grid, s);
+        case 9: return launch_wave<9, MODE, 1>(a, grid, s);
+        case 18: return launch_wave<18, MODE, 1>(a, grid, s);
+        }
+    } else if constexpr (NW == 2) {
+        switch (ppt) {
+        case 3: return launch_wave<3, MODE, 2>(a, grid, s);
+        case 4: return launch_wave<4, MODE, 2>(a, grid, s);
+        case 5: return launch_wave<5, MODE, 2>(a, grid, s);
+        case 7: return launch_wave<7, MODE, 2>(a, grid, s);
+        case 9: return launch_wave<9, MODE, 2>(a, grid, s);
+        }
+    } else {
+        switch (ppt) {
+        case 2: return launch_wave<2, MODE, 4>(a, grid, s);
+        case 3: return launch_wave<3, MODE, 4>(a, grid, s);
+        case 5: return launch_wave<5, MODE, 4>(a, grid, s);
+        }
+    }
+    return hipErrorInvalidValue;
 }
 
 }  // namespace
 
-int icgn_wave_ppt(int npix_cap)
+int icgn_wave_ppt(int npix_cap, int nw)
 {
-    if (npix_cap <= 64 * 5) return 5;
-    if (npix_cap <= 64 * 6) return 6;
-    if (npix_cap <= 64 * 7) return 7;
-    if (npix_cap <= 64 * 9) return 9;
-    if (npix_cap <= 64 * 18) return 18;
+    // smallest instantiated pixels-per-lane count that holds npix_cap pixels
+    // over nw wavefronts
+    static const int s1[] = {5, 6, 7, 9, 18}, s2[] = {3, 4, 5, 7, 9}, s4[] = {2, 3, 5};
+    const int *set = nw == 1 ? s1 : (nw == 2 ? s2 : s4);
+    const int n = nw == 1 ? 5 : (nw == 2 ? 5 : 3);
+    if (nw != 1 && nw != 2 && nw != 4) return 0;
+    for (int i = 0; i < n; ++i)
+        if (64 * nw * set[i] >= npix_cap) return set[i];
     return 0;  // too large for the wave kernel
 }
 
-hipError_t icgn_wave_launch(const IcgnArgs &a, int mode, int ppt, int grid, hipStream_t s)
+hipError_t icgn_wave_launch(const IcgnArgs &a, int mode, int ppt, int nw, int grid, hipStream_t s)
 {
     if (mode == 6) {
-        switch (ppt) {
-        case 5: return launch_wave<5, 6>(a, grid, s);
-        case 6: return launch_wave<6, 6>(a, grid, s);
-        case 7: return launch_wave<7, 6>(a, grid, s);
-        case 9: return launch_wave<9, 6>(a, grid, s);
-        case 18: return launch_wave<18, 6>(a, grid, s);
-        }
+        if (nw == 1) return launch_wave_ppt<6, 1>(a, ppt, grid, s);
+        if (nw == 2) return launch_wave_ppt<6, 2>(a, ppt, grid, s);
+        if (nw == 4) return launch_wave_ppt<6, 4>(a, ppt, grid, s);
     } else {
-        switch (ppt) {
-        case 5: return launch_wave<5, 2>(a, grid, s);
-        case 6: return launch_wave<6, 2>(a, grid, s);
-        case 7: return launch_wave<7, 2>(a, grid, s);
-        case 9: return launch_wave<9, 2>(a, grid, s);
-        case 18: return launch_wave<18, 2>(a, grid, s);
-        }
+        if (nw == 1) return launch_wave_ppt<2, 1>(a, ppt, grid, s);
+        if (nw == 2) return launch_wave_ppt<2, 2>(a, ppt, grid, s);
+        if (nw == 4) return launch_wave_ppt<2, 4>(a, ppt, grid, s);
     }
     return hipErrorInvalidValue;
 }

@@ -57,8 +57,10 @@ struct SearchArgs {
 
 hipError_t icgn_launch(const IcgnArgs &a, int mode, int bs, int grid, hipStream_t s);
 // wave-per-subset kernel: pixels per lane for a window of npix_cap pixels (0 = too large)
-int icgn_wave_ppt(int npix_cap);
-hipError_t icgn_wave_launch(const IcgnArgs &a, int mode, int ppt, int grid, hipStream_t s);
+// pixels per lane for windows of npix_cap pixels spread over nw wavefronts
+// (0: too large for the wavefront kernel)
+int icgn_wave_ppt(int npix_cap, int nw);
+hipError_t icgn_wave_launch(const IcgnArgs &a, int mode, int ppt, int nw, int grid, hipStream_t s);
 size_t icgn_lds_bytes(int npix_cap, int tile_cap);
 hipError_t ba_interp2_launch(const void *F, long M, long N, long P, const void *X, const void *Y,
                              long nd, void *out, bool single, hipStream_t s);

@@ -314,13 +314,51 @@ def test_batch_forms_agree(ctx, plain_cfg):
         got = b.outputs()
         for r, g in zip(ref, got):
             np.testing.assert_array_equal(r, g)
+    # two and four wavefronts per subset (the schedule for long subsets), and
+    # the autotuned schedule: the FP32 summation order changes, nothing else
+    for form in (-2, -4):
+        b.icgn6(params(tol=1e-3, block_size=form))
+        got = b.outputs()
+        assert np.mean(ref[3] != got[3]) < 0.01, form
+        ok = (ref[3] == got[3]) & (ref[3] >= 1) & (ref[3] <= 100)
+        np.testing.assert_allclose(ref[0].reshape(-1, 2)[ok], got[0].reshape(-1, 2)[ok], atol=1e-4)
+        np.testing.assert_allclose(ref[1].reshape(-1, 4)[ok], got[1].reshape(-1, 4)[ok], atol=1e-5)
     b.autotune(p, 6)
     b.icgn6(p)
     got = b.outputs()
-    # a different workgroup size changes the FP32 summation order only
     assert np.mean(ref[3] != got[3]) < 0.01
     ok = (ref[3] == got[3]) & (ref[3] >= 1) & (ref[3] <= 100)
     np.testing.assert_allclose(ref[0].reshape(-1, 2)[ok], got[0].reshape(-1, 2)[ok], atol=1e-4)
+    b.close()
+
+
+@pytest.mark.parametrize("form", [-2, -4])
+def test_multi_wave_forms_on_masked_subsets(ctx, oracle, form):
+    """Two / four wavefronts per subset through the masked path (region
+    floods, window growth, -1 and 102) and the 2-DOF step, against the oracle."""
+    from staq_dic_amd import Batch, params, synth
+    from staq_dic_amd._lib import IN_FOLD, IN_U0, IN_UOLD
+    fr, coords, win = synth.config_hole(H=384, W=384, n_speckles=1700, max_disp=3.0, radius=60,
+                                        margin=24, band=(96, 40))
+    truth = synth.truth_at(fr.field, coords)
+    U0 = np.round(truth[:, 4:6]).reshape(-1)
+    _print("hole 384^2 with", -form, "wavefronts per subset, N =", coords.shape[0])
+    ctx.set_frame(fr.f, fr.g, fr.mask)
+    b = Batch(ctx, coords, win)
+    b.set_input(IN_U0, U0)
+    b.icgn6(params(tol=1e-3, block_size=form))
+    Ug, Fg, Hg, sg = b.outputs()
+    of = oracle.Frame(fr.f, fr.g, fr.mask)
+    Uo, Fo, Ho, so = oracle.local_icgn(of, U0, coords, win, tol=1e-3)
+    _cmp_icgn(sg, so, Ug, Uo, Fg, Fo, max_status_mismatch=0.02)
+    FOld = np.where(np.isfinite(Fo), Fo, 0.0)
+    UOld = np.where(np.isfinite(Uo), Uo, U0)
+    b.set_input(IN_UOLD, UOld)
+    b.set_input(IN_FOLD, FOld)
+    b.icgn2(params(tol=1e-3, mu=1e-3, block_size=form))
+    Ug2, _, _, sg2 = b.outputs()
+    Uo2, so2 = oracle.subpb1(of, UOld, FOld, np.zeros_like(UOld), coords, win, win, 1e-3, tol=1e-3)
+    _cmp_icgn(sg2, so2, Ug2, Uo2, max_status_mismatch=0.02)
     b.close()
 
 

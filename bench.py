@@ -103,17 +103,42 @@ def main():
     p6 = params(tol=1e-3)
     stream = torch.cuda.ExternalStream(ctx.stream)
 
-    # schedule: autotune the workgroup size per window bin, order each bin
-    # longest-first from a first pass, then capture the step in a graph
-    b.autotune(p6, 6)
-    b.icgn6(p6)
-    b.reorder()
-    b.graph_capture(p6, 6)
+    # schedule: order each bin longest-first from a first pass, autotune the
+    # kernel form per bin, keep whichever of the default and the tuned
+    # schedule replays faster as a graph, then capture the step in a graph
+    ev0 = torch.cuda.Event(enable_timing=True)
+    ev1 = torch.cuda.Event(enable_timing=True)
+
+    def graph_ms(p, mode, reps=10):
+        b.graph_capture(p, mode)
+        for _ in range(2):
+            b.graph_launch()
+        torch.cuda.synchronize()
+        ev0.record(stream)
+        for _ in range(reps):
+            b.graph_launch()
+        ev1.record(stream)
+        torch.cuda.synchronize()
+        return ev0.elapsed_time(ev1) / reps
+
+    def schedule(p, mode):
+        key = "bs6" if mode == 6 else "bs2"
+        (b.icgn6 if mode == 6 else b.icgn2)(p)
+        b.reorder()
+        default_forms = [x[key] for x in b.bins()]
+        t_default = graph_ms(p, mode)
+        b.autotune(p, mode)
+        t_tuned = graph_ms(p, mode)
+        if t_tuned > t_default:
+            b.set_forms(default_forms, mode)
+        log(f"[bench] mode {mode} schedule: default {t_default:.3f} ms, tuned {t_tuned:.3f} ms; "
+            f"forms {[x[key] for x in b.bins()]}")
+        b.graph_capture(p, mode)
+
+    schedule(p6, 6)
     for _ in range(args.warmup):
         b.graph_launch()
     torch.cuda.synchronize()
-    ev0 = torch.cuda.Event(enable_timing=True)
-    ev1 = torch.cuda.Event(enable_timing=True)
     ev0.record(stream)
     for _ in range(args.steps):
         b.graph_launch()
@@ -130,23 +155,28 @@ def main():
     value = n_conv / (ms / 1e3)
     log(f"[bench] icgn6 step {ms:.3f} ms, {n_conv}/{N} converged, rms {rms:.4f} px")
 
-    # roofline of the dominant kernel: every launch of the step timed with
-    # events on the stream it runs on, over the same number of eager steps
+    # roofline of the dominant kernel (the launch that carries the most
+    # algorithmic bytes): every launch of the step timed with events on the
+    # stream it runs on, over the same number of eager steps
     bins = b.bins()
     kms = np.zeros(len(bins) + 1)
     for _ in range(args.steps):
         kms += b.step_timed(p6, 6)
     kms /= args.steps
-    kdom = int(np.argmax(kms[:-1]))
-    bdom = bins[kdom]
-    members = b.bin_of() == kdom
+    bin_of = b.bin_of()
     iters = np.where(conv, st, 0).astype(float)
-    bytes_launch = algorithmic_bytes(win[members], win[members], iters[members])
+    bytes_bin = np.array([algorithmic_bytes(win[bin_of == k], win[bin_of == k], iters[bin_of == k])
+                          for k in range(len(bins))])
+    kdom = int(np.argmax(bytes_bin))
+    bdom = bins[kdom]
+    bytes_launch = float(bytes_bin[kdom])
     achieved = bytes_launch / (kms[kdom] / 1e3) / 1e9
-    kname = (f"icgn_wave_kernel<{bdom['ppt']},6>" if bdom["bs6"] == 0
+    nw = 1 if bdom["bs6"] == 0 else -bdom["bs6"]
+    kname = (f"icgn_wave_kernel<{bdom['ppt']},6,{nw}>" if bdom["bs6"] <= 0
              else f"icgn_kernel<{bdom['bs6']},6>")
+    step_bytes = float(bytes_bin.sum())
     log(f"[bench] dominant launch {kname}: {kms[kdom]:.3f} ms, {bytes_launch / 1e6:.1f} MB, "
-        f"{achieved:.1f} GB/s; bins {[round(float(x), 3) for x in kms]}")
+        f"{achieved:.1f} GB/s; launches {[round(float(x), 3) for x in kms]} ms")
     traffic = _profiled_traffic(kname)
 
     result = {
@@ -161,22 +191,28 @@ def main():
         "converged": n_conv, "rms_px": rms, "rms_px_best99": rms99,
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
-                     "kernel": kname, "subsets_per_launch": int(members.sum()),
+                     "kernel": kname, "subsets_per_launch": int((bin_of == kdom).sum()),
                      "algorithmic_bytes_per_launch": bytes_launch,
                      "launch_ms": float(kms[kdom]),
-                     "launch_ms_all": [float(x) for x in kms]},
+                     "launch_ms_all": [float(x) for x in kms],
+                     "step_algorithmic_bytes": step_bytes,
+                     "step_achieved_gbs": step_bytes / (ms / 1e3) / 1e9},
     }
 
     if not args.no_extras:
-        # 2-DOF AL step on the 6-DOF result (identity global step, vdual = 0)
+        # 2-DOF AL step on the 6-DOF result (identity global step, vdual = 0),
+        # scheduled and replayed the same way
+        b.set_input(IN_U0, U0)
+        b.icgn6(p6)
         b.chain()
         p2 = params(tol=1e-3, mu=1e-3)
-        b.autotune(p2, 2)
-        b.icgn2(p2)
+        schedule(p2, 2)
+        for _ in range(args.warmup):
+            b.graph_launch()
         torch.cuda.synchronize()
         ev0.record(stream)
         for _ in range(args.steps):
-            b.icgn2(p2)
+            b.graph_launch()
         ev1.record(stream)
         torch.cuda.synchronize()
         ms2 = ev0.elapsed_time(ev1) / args.steps

@@ -197,6 +197,10 @@ int staq_batch_get_ptrs(staq_batch *b, staq_batch_ptrs *out);
 int staq_batch_bins(staq_batch *b, int *nbins, int *wcap, long *count, int *bs6, int *bs2,
                     int *ppt, int maxbins);
 
+/* Set the kernel form of bin `bin` for mode 6 or 2 (0 = one wavefront per
+ * subset, -1/-2/-4 = that many wavefronts, 64..512 = workgroup kernel), as
+ * staq_batch_autotune would; drops a captured graph. */
+int staq_batch_set_form(staq_batch *b, int bin, int mode, int form);
 /* Bin index of every subset (N entries, in the caller's subset order).  Bins
  * group similar window sizes; windows that touch the mask of their frame at
  * batch creation form bins of their own. */

@@ -75,6 +75,7 @@ SIGNATURES = {
     "staq_batch_get_ptrs": (C.c_int, [_vp, C.POINTER(BatchPtrs)]),
     "staq_batch_bins": (C.c_int, [_vp, _ip, _ip, _lp, _ip, _ip, _ip, C.c_int]),
     "staq_batch_bin_of": (C.c_int, [_vp, _ip]),
+    "staq_batch_set_form": (C.c_int, [_vp, C.c_int, C.c_int, C.c_int]),
 }
 
 IN_U0, IN_UOLD, IN_FOLD, IN_VDUAL = 0, 1, 2, 3
@@ -282,6 +283,11 @@ class Batch:
         p = BatchPtrs()
         self.ctx.check(self.L.staq_batch_get_ptrs(self.h, C.byref(p)), "staq_batch_get_ptrs")
         return p
+
+    def set_forms(self, forms, mode):
+        """Kernel form of every bin for mode 6 or 2 (a list as in bins())."""
+        for k, f in enumerate(forms):
+            self.ctx.check(self.L.staq_batch_set_form(self.h, k, mode, int(f)), "staq_batch_set_form")
 
     def bin_of(self):
         """Bin index of every subset, in the caller's order."""

@@ -739,6 +739,23 @@ int staq_batch_bins(staq_batch *b, int *nbins, int *wcap, long *count, int *bs6,
 
 }  // extern "C"
 
+extern "C" int staq_batch_set_form(staq_batch *b, int bin, int mode, int form)
+{
+    if (!b || bin < 0 || bin >= (int)b->bins.size() || (mode != 6 && mode != 2))
+        return STAQ_EINVAL;
+    if (!(form == 0 || form == -1 || form == -2 || form == -4 || form == 64 || form == 128 ||
+          form == 256 || form == 512))
+        return fail(b->ctx, STAQ_EINVAL, "kernel form must be 0, -1, -2, -4, 64, 128, 256 or 512");
+    b->bins[bin].bs[mode == 6 ? 0 : 1] = form;
+    if (b->gexec) {  // a captured graph holds the old schedule
+        hipGraphExecDestroy(b->gexec);
+        hipGraphDestroy(b->graph);
+        b->gexec = nullptr;
+        b->graph = nullptr;
+    }
+    return STAQ_OK;
+}
+
 extern "C" int staq_batch_bin_of(staq_batch *b, int *bin)
 {
     if (!b || !bin) return STAQ_EINVAL;

@@ -52,10 +52,19 @@ __device__ unsigned long long wave_flood(unsigned long long bw, int ny, int seed
 {
     const int lane = threadIdx.x & 63;
     unsigned long long sel = 0ull;
-    if (lane < ny && seed_c >= 1 && seed_c <= ny && seed_r >= 1 && seed_r <= 64 &&
-        lane == seed_c - 1)
-        sel = bw & (1ull << (seed_r - 1));
     if (lane >= ny) bw = 0ull;
+    if (seed_c >= 1 && seed_c <= ny && seed_r >= 1 && seed_r <= 64) {
+        // start from a cross: the seed row's horizontal run through the seed
+        // (one ballot gives the row as a word) and the vertical runs of its
+        // columns; every pixel of it is 4-connected to the seed
+        const unsigned long long row = __ballot((bw >> (seed_r - 1)) & 1ull);
+        const unsigned long long sb = 1ull << (seed_c - 1);
+        if (row & sb) {
+            const unsigned long long run = run_fill(row, sb);
+            sel = ((run >> lane) & 1ull) ? (1ull << (seed_r - 1)) : 0ull;
+            sel = run_fill(bw, sel);
+        }
+    }
     for (;;) {
         unsigned long long l = dpp_u64<0x138>(sel);  // wave_shr:1 - column j-1
         unsigned long long r = dpp_u64<0x130>(sel);  // wave_shl:1 - column j+1

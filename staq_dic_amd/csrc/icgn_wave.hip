@@ -525,6 +525,10 @@ __global__ __launch_bounds__(64 * NW) void icgn_wave_kernel(IcgnArgs a)
         // spec_next = the last iteration was masked, so the next pass applies it
         unsigned nzprev = 0u;
         bool region_valid = false, spec_next = false, region_new_prev = false;
+        // the region's column word (this lane's column) and whether the
+        // region is the whole nonzero set of g, the usual case
+        unsigned long long regw = 0ull;
+        bool region_is_nz = false;
         // g tile resident in LDS: [tu0, tu0+tnx) x [tv0, tv0+tny) (1-based)
         int tu0 = 0, tv0 = 0, tnx = 0, tny = 0;
         bool have_tile = false;
@@ -838,9 +842,15 @@ __global__ __launch_bounds__(64 * NW) void icgn_wave_kernel(IcgnArgs a)
                     }
                 }
                 if (lane >= ny) sw = 0ull;
+                // (2-DOF treats every flood as a new region and rebuilds its
+                // f side; the prediction below is for the 6-DOF solver)
+                const bool changed = MODE == 2 || !region_valid || wave_any(sw != regw);
+                region_is_nz = !wave_any(sw != colnz);
+                regw = sw;
                 __syncthreads();
                 if (wv == 0) L.colw[lane] = sw;
                 __syncthreads();
+                if (changed) {
                 selbits = 0u;
                 sum_bw2 = 0;
 #pragma unroll
@@ -859,16 +869,24 @@ __global__ __launch_bounds__(64 * NW) void icgn_wave_kernel(IcgnArgs a)
                     group_reduce<NW, 0, 0, 1>(L, par, nullptr, nullptr, sb2);
                     sum_bw2 = sb2[0];
                 }
-                nzprev = nzb;
-                region_valid = true;
                 region_new = true;
                 region_new_prev = true;
                 if (MODE == 2) fsel = true;
+                }
+                nzprev = nzb;
+                region_valid = true;
             }
-            // the fused pass stands unless the region changed or the region
-            // was applied to an iteration that turned out unmasked
-            const bool resum = masked_it ? (region_new || !gspec) : gspec;
-            spec_next = masked_it;
+            // The fused pass stands if it applied the region that holds now:
+            // the last region (gspec) when that did not change, or (6-DOF) no
+            // region when the region is the whole nonzero set of g (the pixels
+            // outside it are zero anyway).
+            bool stands;
+            if (masked_it)
+                stands = MODE == 6 ? (gspec ? !region_new : region_is_nz) : (gspec && !region_new);
+            else
+                stands = !gspec;
+            const bool resum = !stands;
+            spec_next = masked_it && (MODE == 2 || !region_is_nz);
             const bool fs = MODE == 2 && fsel;
             // the 2-DOF region rebuild needs f*mask per pixel; windows too large
             // to keep it in registers take the literal path

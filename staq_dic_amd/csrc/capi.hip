@@ -616,7 +616,17 @@ int staq_batch_create(staq_ctx *ctx, staq_batch **out, long N, const double *coo
         // (or grows although classified clear) defers to the fallback launch
         bin.wave_npix = std::max(bin.wave_npix, bin.npix_cap);
         bin.ppt = icgn_wave_ppt(bin.wave_npix, 1);
-        bin.bs[0] = bin.bs[1] = bin.ppt ? 0 : default_bs(bin.npix_cap);
+        // default form: one wavefront while its pixel slots stay within 9
+        // (branch-free slots), else two or four wavefronts, else the
+        // workgroup kernel
+        int form = default_bs(bin.npix_cap);
+        if (bin.ppt && bin.ppt <= 9)
+            form = 0;
+        else if (icgn_wave_ppt(bin.wave_npix, 2))
+            form = -2;
+        else if (icgn_wave_ppt(bin.wave_npix, 4))
+            form = -4;
+        bin.bs[0] = bin.bs[1] = form;
         b->bins.push_back(bin);
         fbx = std::max(fbx, grown_window(bin.wxcap));
         fby = std::max(fby, grown_window(bin.wycap));

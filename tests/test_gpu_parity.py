@@ -97,11 +97,21 @@ def _cmp_icgn(st_g, st_o, U_g, U_o, F_g=None, F_o=None, max_status_mismatch=0.01
     assert np.mean(np.abs(st_g - st_o) > 1) <= max_status_mismatch
     both = conv_g & conv_o
     assert both.sum() > 0
-    du = np.abs(U_g.reshape(-1, 2) - U_o.reshape(-1, 2))[both]
-    assert np.nanmax(du) < 1e-3, np.nanmax(du)
+    # same iteration count: the FP32 sums move the converged point by ~1e-5 px;
+    # one iteration more or less (the stopping test within FP32 rounding of
+    # tol): the difference is one Gauss-Newton step at the tolerance
+    same = both & (st_g == st_o)
+    near = both & (st_g != st_o)
+    assert same.sum() >= 0.95 * both.sum()
+    du = np.abs(U_g.reshape(-1, 2) - U_o.reshape(-1, 2))
+    assert np.nanmax(du[same]) < 1e-3, np.nanmax(du[same])
+    if near.any():
+        assert np.nanmax(du[near]) < 1e-2, np.nanmax(du[near])
     if F_g is not None:
-        dF = np.abs(F_g.reshape(-1, 4) - F_o.reshape(-1, 4))[both]
-        assert np.nanmax(dF) < 1e-4, np.nanmax(dF)
+        dF = np.abs(F_g.reshape(-1, 4) - F_o.reshape(-1, 4))
+        assert np.nanmax(dF[same]) < 1e-4, np.nanmax(dF[same])
+        if near.any():
+            assert np.nanmax(dF[near]) < 1e-3, np.nanmax(dF[near])
     err = (st_g == -1)
     assert np.all(np.isnan(U_g.reshape(-1, 2)[err]))
     return both.sum()
@@ -396,3 +406,76 @@ def test_znssd_autograd(ctx, oracle, plain_cfg):
     ref = go * wn
     scale = np.abs(ref[ok]).max(axis=0) + 1e-12
     assert np.max(np.abs(gP[ok] - ref[ok]) / scale) < 2e-3
+
+
+def test_multi_pair_batch(ctx, oracle):
+    """Several frame pairs resident at once, one batch over all of them (each
+    subset names its pair by slot), against the oracle pair by pair: the
+    multi-instance form, 6-DOF and 2-DOF."""
+    from staq_dic_amd import Batch, params, synth
+    from staq_dic_amd._lib import IN_FOLD, IN_U0, IN_UOLD
+    pairs = []
+    for j, (H, W, field) in enumerate([
+            (384, 448, synth.AffineField(2.2, -1.1, 0.015, -0.008, 0.004, 0.012)),
+            (448, 384, synth.AffineField(-3.1, 0.7, -0.01, 0.006, 0.009, -0.004))]):
+        fr = synth.make_frame(H, W, 2000, 5000 + j, field)
+        coords = synth.plain_grid_nodes(H, W, 24, 24, margin=24)
+        coords = coords[(coords[:, 0] <= H - 28) & (coords[:, 1] <= W - 28)]
+        pairs.append((fr, coords))
+    slots = [3, 1]
+    for (fr, _), sl in zip(pairs, slots):
+        ctx.set_frame(fr.f, fr.g, fr.mask, slot=sl)
+    coords = np.concatenate([c for _, c in pairs])
+    slot = np.concatenate([np.full(len(c), sl, dtype=np.int32) for (_, c), sl in zip(pairs, slots)])
+    win = np.where(np.arange(len(coords)) % 2 == 0, 24, 16).astype(np.int64)
+    truth = np.concatenate([synth.truth_at(fr.field, c) for fr, c in pairs])
+    U0 = np.round(truth[:, 4:6]).reshape(-1)
+    _print("two pairs in slots", slots, "N =", len(coords))
+    b = Batch(ctx, coords, win, slot=slot)
+    b.set_input(IN_U0, U0)
+    b.icgn6(params(tol=1e-3))
+    Ug, Fg, Hg, sg = b.outputs()
+    b.set_input(IN_UOLD, Ug)
+    b.set_input(IN_FOLD, np.where(np.isfinite(Fg), Fg, 0.0))
+    b.icgn2(params(tol=1e-3, mu=1e-3))
+    Ug2, _, _, sg2 = b.outputs()
+    b.close()
+    i0 = 0
+    for fr, c in pairs:
+        n = len(c)
+        sel = slice(i0, i0 + n)
+        of = oracle.Frame(fr.f, fr.g, fr.mask)
+        Uo, Fo, Ho, so = oracle.local_icgn(of, U0.reshape(-1, 2)[sel].reshape(-1), c, win[sel],
+                                           tol=1e-3)
+        _cmp_icgn(sg[sel], so, Ug.reshape(-1, 2)[sel].reshape(-1), Uo,
+                  Fg.reshape(-1, 4)[sel].reshape(-1), Fo)
+        UOld = Ug.reshape(-1, 2)[sel].reshape(-1)
+        FOld = np.where(np.isfinite(Fg), Fg, 0.0).reshape(-1, 4)[sel].reshape(-1)
+        Uo2, so2 = oracle.subpb1(of, UOld, FOld, np.zeros_like(UOld), c, win[sel], win[sel], 1e-3,
+                                 tol=1e-3)
+        _cmp_icgn(sg2[sel], so2, Ug2.reshape(-1, 2)[sel].reshape(-1), Uo2)
+        i0 += n
+
+
+def test_integer_search_large_radius(ctx, oracle):
+    """The large-deformation search geometry: window 64, radius 48, a field
+    with ~30 px translation, against the oracle point by point."""
+    from staq_dic_amd import funIntegerSearch, synth
+    field = synth.AffineField(30.0, -25.0, 0.01, -0.005, 0.004, 0.008)
+    fr = synth.make_frame(640, 640, 6000, 6001, field)
+    H, W = fr.f.shape
+    _print("integer search 640^2, r=48, w=64, step 32")
+    x, y, u, v, cc = funIntegerSearch(fr.f, fr.g, 48, [1, H], [1, W], 64, 32, ctx=ctx, slot=2)
+    xo, yo, uo, vo, cco = oracle.integer_search(fr.f, fr.g, 48, [1, H], [1, W], 64, 32)
+    assert x.size >= 100
+    np.testing.assert_array_equal(x, xo)
+    np.testing.assert_array_equal(y, yo)
+    fin = np.isfinite(uo)
+    np.testing.assert_array_equal(np.isfinite(u), fin)
+    np.testing.assert_array_equal(np.floor(u[fin]), np.floor(uo[fin]))
+    np.testing.assert_array_equal(np.floor(v[fin]), np.floor(vo[fin]))
+    np.testing.assert_allclose(u[fin], uo[fin], atol=1e-3 + 1e-12)
+    np.testing.assert_allclose(v[fin], vo[fin], atol=1e-3 + 1e-12)
+    tu = 30.0 + 0.01 * x - 0.005 * y
+    tv = -25.0 + 0.004 * x + 0.008 * y
+    assert np.max(np.abs(u - tu)[fin]) < 0.75 and np.max(np.abs(v - tv)[fin]) < 0.75

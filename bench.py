@@ -245,6 +245,36 @@ def main():
         result["integer_search"] = {"points": int(x.size), "seconds_host_to_host": dt,
                                     "points_per_s": x.size / dt}
 
+        # adjoint ZNSSD (forward + vector-Jacobian product through the bicubic
+        # warp) on the 2048^2 sinusoidal-field pair with its 16-px node grid,
+        # inputs resident, through the torch.autograd.Function
+        from staq_dic_amd.autograd import znssd
+        fa, ca, wa = synth.config_adjoint()
+        ta = synth.truth_at(fa.field, ca)
+        dev = torch.device("cuda", 0)
+        with torch.cuda.stream(stream):
+            P = torch.tensor(ta, dtype=torch.float64, device=dev, requires_grad=True)
+            cc_ = torch.tensor(ca, device=dev)
+            ww = torch.tensor(wa, dtype=torch.int32, device=dev)
+            ctx.set_frame(fa.f, fa.g, fa.mask, slot=0)
+            for _ in range(2):
+                cost, _ = znssd(P, cc_, ww, ctx)
+                cost.sum().backward()
+            torch.cuda.synchronize()
+            ev0.record(stream)
+            for _ in range(args.steps):
+                P.grad = None
+                cost, _ = znssd(P, cc_, ww, ctx)
+                cost.sum().backward()
+            ev1.record(stream)
+            torch.cuda.synchronize()
+        msa = ev0.elapsed_time(ev1) / args.steps
+        result["adjoint_znssd"] = {"workload": "2048x2048 sinusoidal field, 16-px grid, window 16",
+                                   "subsets": int(len(ca)), "ms_fwd_bwd": msa,
+                                   "subsets_per_s": len(ca) / (msa / 1e3)}
+        log(f"[bench] adjoint fwd+bwd {msa:.3f} ms for {len(ca)} subsets")
+        ctx.set_frame(fr.f, fr.g, fr.mask, slot=0)
+
     if not args.no_cpu:
         from oracle import pyoracle
         of = pyoracle.Frame(fr.f, fr.g, fr.mask)

@@ -8,9 +8,10 @@ the reference's layout: images are ``(H, W)`` arrays indexed ``[x-1, y-1]``
 
 What stays on the host is the reference's bookkeeping around the per-subset
 solver: bad-subset classification (LocalICGNQuadtree.m:151-177,
-Subpb1Quadtree.m:168-187), the hole mask of the integer search
-(IntegerSearch.m:69-77) and timing.  The RBF / scatteredInterpolant refill of
-bad subsets is outside this path; bad subsets come back as NaN.
+Subpb1Quadtree.m:168-187) and the refill of bad subsets (refill.py), the hole
+mask of the integer search (IntegerSearch.m:69-77) and timing.  The plain
+twins' refills (plane fit and rmoutliers, gridfit) are not restated: their bad
+subsets come back as NaN.
 """
 from __future__ import annotations
 
@@ -19,6 +20,7 @@ import time
 import numpy as np
 
 from . import _lib
+from .refill import refill_bad
 from ._lib import Batch, Context, IN_FOLD, IN_U0, IN_UOLD, IN_VDUAL, d, ip, lp
 
 __all__ = [
@@ -237,9 +239,10 @@ def _hole_nodes(coords, winsize, f, mask):
 
 
 def LocalICGNQuadtree(U0, coordinatesFEM, Df, ImgRef, ImgDef, DICpara, ICGNmethod="GaussNewton",
-                      tol=1e-2, ctx=None):
+                      tol=1e-2, ctx=None, refill=True):
     """[U,F,HtempPar,LocalTime,ConvItPerEle,LocalICGNBadPtNum,markCoordHoleStrain] =
-    LocalICGNQuadtree(U0,coordinatesFEM,Df,ImgRef,ImgDef,DICpara,ICGNmethod,tol)."""
+    LocalICGNQuadtree(U0,coordinatesFEM,Df,ImgRef,ImgDef,DICpara,ICGNmethod,tol).
+    Bad subsets are refilled as the reference does (refill=False leaves them NaN)."""
     if ICGNmethod != "GaussNewton":
         raise NotImplementedError("only the Gauss-Newton scheme runs on this path")
     coords = np.asarray(coordinatesFEM, dtype=np.float64)
@@ -257,6 +260,8 @@ def LocalICGNQuadtree(U0, coordinatesFEM, Df, ImgRef, ImgDef, DICpara, ICGNmetho
     for k in range(4):
         F[4 * np.flatnonzero(bad) + k] = np.nan
     m = np.ones(np.shape(ImgRef)) if mask is None else np.asarray(mask)
+    if refill:
+        U, F = refill_bad(U, F, coords, m)  # LocalICGNQuadtree.m:177-333
     hole = _hole_nodes(coords, win, np.asarray(ImgRef), m)
     return U, F, H, LocalTime, st.astype(np.float64), nbad, hole
 
@@ -287,7 +292,7 @@ def LocalICGN(U0, coordinatesFEM, Df, ImgRef, ImgDef, DICpara, ICGNmethod="Gauss
 
 
 def Subpb1Quadtree(UOld, FOld, udual, vdual, coordinatesFEM, Df, ImgRef, ImgDef, mu, beta, HPar,
-                   ALSolveStep, DICpara, ICGNmethod="GaussNewton", tol=1e-2, ctx=None):
+                   ALSolveStep, DICpara, ICGNmethod="GaussNewton", tol=1e-2, ctx=None, refill=True):
     """[U,HPar,ALSub1Time,ConvItPerEle,LocalICGNBadPtNum] = Subpb1Quadtree(...).
     Window sizes come from DICpara['winsize_List'] (N x 2), else winsize."""
     if ICGNmethod != "GaussNewton":
@@ -312,6 +317,9 @@ def Subpb1Quadtree(UOld, FOld, udual, vdual, coordinatesFEM, Df, ImgRef, ImgDef,
     idx = np.flatnonzero(bad)
     U[2 * idx] = np.nan
     U[2 * idx + 1] = np.nan
+    if refill:
+        m = np.ones(np.shape(ImgRef)) if mask is None else np.asarray(mask)
+        U, _ = refill_bad(U, None, coords, m)  # Subpb1Quadtree.m:189-256
     return U, HPar, ALSub1Time, st.astype(np.float64), nbad
 
 

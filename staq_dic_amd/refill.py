@@ -1,0 +1,114 @@
+"""Refill of bad subsets after a local step, on the host (FP64).
+
+LocalICGNQuadtree (func_quadtree/LocalICGNQuadtree.m:177-333) and
+Subpb1Quadtree (func_quadtree/Subpb1Quadtree.m:189-256) replace the values of
+bad subsets (NaN after the bad-subset classification) in two stages:
+
+1. for every 8-connected component of the reference mask, a thin-plate radial
+   basis function phi(r) = r^2 log(r + 1) with a linear polynomial term
+   (rbfinterp/rbfcreate.m:148-200) is fitted to the good nodes whose rounded
+   position lies in the component, and evaluated at every node of the
+   component (good nodes get their own values back);
+2. if NaNs remain (nodes outside every component, or a component with too few
+   good nodes), every node is re-interpolated from the non-NaN nodes with
+   scatteredInterpolant(..., 'natural', 'nearest').
+
+Stage 1 solves the reference's dense (n+3) x (n+3) system [A P; P' 0]; the
+matrix is the same for every field of a component, so it is factored once and
+solved for all fields together.  Stage 2 uses linear interpolation on the
+Delaunay triangulation with nearest-node extrapolation: SciPy has no
+natural-neighbour interpolant, so this stage is an approximation of the
+reference (exact at the data nodes, different inside triangles).
+"""
+from __future__ import annotations
+
+import numpy as np
+import scipy.linalg as sla
+from scipy import ndimage
+from scipy.interpolate import LinearNDInterpolator, NearestNDInterpolator
+from scipy.spatial.distance import cdist
+
+__all__ = ["thinplate_fit", "thinplate_eval", "refill_bad"]
+
+
+def _phi(r):
+    return r * r * np.log(r + 1.0)
+
+
+def thinplate_fit(centers, values):
+    """Coefficients of rbfcreate(centers', values', 'RBFFunction', 'thinplate'):
+    centers (n, 2), values (n, m) -> (n + 3, m)."""
+    c = np.asarray(centers, dtype=np.float64)
+    n = c.shape[0]
+    r = cdist(c, c)
+    A = np.zeros((n + 3, n + 3))
+    A[:n, :n] = _phi(r)
+    A[:n, n] = 1.0
+    A[:n, n + 1:] = c
+    A[n, :n] = 1.0
+    A[n + 1:, :n] = c.T
+    b = np.zeros((n + 3, values.shape[1]))
+    b[:n] = values
+    return sla.solve(A, b)
+
+
+def thinplate_eval(centers, coeff, points):
+    """rbfinterp(points', op): coefficient order [w_1..w_n, c0, cx, cy]."""
+    c = np.asarray(centers, dtype=np.float64)
+    p = np.asarray(points, dtype=np.float64)
+    n = c.shape[0]
+    r = cdist(p, c)
+    return _phi(r) @ coeff[:n] + coeff[n] + p[:, :1] * coeff[n + 1] + p[:, 1:2] * coeff[n + 2]
+
+
+def _natural_nearest(xy_known, vals, xy_all):
+    lin = LinearNDInterpolator(xy_known, vals)
+    out = lin(xy_all)
+    bad = ~np.isfinite(out).all(axis=1) if out.ndim > 1 else ~np.isfinite(out)
+    if bad.any():
+        out[bad] = NearestNDInterpolator(xy_known, vals)(xy_all[bad])
+    return out
+
+
+def refill_bad(U, F, coordinatesFEM, mask):
+    """U (2N) and F (4N or None) with NaN at bad subsets -> refilled copies.
+    mask is the reference mask (H, W) indexed [x-1, y-1]."""
+    coords = np.asarray(coordinatesFEM, dtype=np.float64)
+    N = coords.shape[0]
+    U = np.array(U, dtype=np.float64).reshape(N, 2)
+    Fm = None if F is None else np.array(F, dtype=np.float64).reshape(N, 4)
+    m = np.asarray(mask) != 0
+    H, W = m.shape
+    lab, ncomp = ndimage.label(m, structure=np.ones((3, 3), dtype=bool))  # 8-connected
+    rc = np.round(coords).astype(np.int64)
+    inside = (rc[:, 0] >= 1) & (rc[:, 0] <= H) & (rc[:, 1] >= 1) & (rc[:, 1] <= W)
+    comp = np.zeros(N, dtype=np.int64)
+    comp[inside] = lab[rc[inside, 0] - 1, rc[inside, 1] - 1]
+    good_u = np.isfinite(U[:, 0])
+    good_f = good_u if Fm is None else np.isfinite(Fm[:, 0])
+
+    # stage 1: thin-plate RBF per mask component (:189-255)
+    for k in range(1, ncomp + 1):
+        members = np.flatnonzero(comp == k)
+        if members.size == 0:
+            continue
+        for vals, good, cols in ((U, good_u, (0, 1)),) + (((Fm, good_f, (0, 1, 2, 3)),)
+                                                          if Fm is not None else ()):
+            src = members[good[members]]
+            if src.size < 3:
+                continue  # the reference's try/catch leaves the component as it is
+            try:
+                coeff = thinplate_fit(np.round(coords[src]), vals[src][:, list(cols)])
+            except (np.linalg.LinAlgError, ValueError):
+                continue
+            vals[np.ix_(members, list(cols))] = thinplate_eval(np.round(coords[src]), coeff,
+                                                               coords[members])
+
+    # stage 2: scatteredInterpolant over all nodes if NaNs remain (:313-330)
+    nan_u = ~np.isfinite(U[:, 0])
+    nan_f = np.zeros(N, dtype=bool) if Fm is None else ~np.isfinite(Fm[:, 0])
+    if nan_u.any() or nan_f.any():
+        U = _natural_nearest(coords[~nan_u], U[~nan_u], coords)
+        if Fm is not None:
+            Fm = _natural_nearest(coords[~nan_f], Fm[~nan_f], coords)
+    return U.reshape(-1), (None if Fm is None else Fm.reshape(-1))

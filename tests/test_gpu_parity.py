@@ -479,3 +479,34 @@ def test_integer_search_large_radius(ctx, oracle):
     tu = 30.0 + 0.01 * x - 0.005 * y
     tv = -25.0 + 0.004 * x + 0.008 * y
     assert np.max(np.abs(u - tu)[fin]) < 0.75 and np.max(np.abs(v - tv)[fin]) < 0.75
+
+
+def test_reference_level_local_steps(ctx, ref_frames):
+    """LocalICGNQuadtree and Subpb1Quadtree as the reference's driver calls them
+    (bad-subset classification and refill included) on the reference's own
+    frames: no NaN left, and the result lands on the stored answer."""
+    from staq_dic_amd import LocalICGNQuadtree, Subpb1Quadtree
+    r = ref_frames
+    N = r["coords"].shape[0]
+    Df = {"ImgRefMask": r["mask"]}
+    DICpara = {"winsize": r["winsize"], "winsize_List": r["winsize_List"]}
+    _print("LocalICGNQuadtree on the reference frames")
+    U, F, Hp, t, conv, nbad, hole = LocalICGNQuadtree(np.round(r["U1"]), r["coords"], Df, r["f"],
+                                                      r["g"], DICpara, "GaussNewton", 1e-2, ctx=ctx)
+    assert np.isfinite(U).all() and np.isfinite(F).all()
+    assert nbad == int(((conv < 0) | (conv > 99)).sum() - (conv == 102).sum())
+    from staq_dic_amd import classify_local, classify_subpb1
+    bad, _ = classify_local(conv)
+    d = np.hypot(U[0::2] - r["U1"][0::2], U[1::2] - r["U1"][1::2])
+    # kept subsets land on the stored answer; refilled ones (about half of the
+    # nodes here: the masked 102 and -1 subsets) follow it loosely
+    assert np.median(d[~bad]) < 5e-3 and np.percentile(d[~bad], 90) < 0.06
+    assert np.median(d[bad]) < 0.1 and np.percentile(d, 90) < 0.25
+    _print("Subpb1Quadtree on the reference frames")
+    U2, _, _, conv2, nbad2 = Subpb1Quadtree(U, F, 0 * F, 0 * U, r["coords"], Df, r["f"], r["g"],
+                                            1e-3, 0.0, None, 2, DICpara, "GaussNewton", 1e-2,
+                                            ctx=ctx)
+    assert np.isfinite(U2).all()
+    bad2, _ = classify_subpb1(conv2)
+    d2 = np.hypot(U2[0::2] - r["U1"][0::2], U2[1::2] - r["U1"][1::2])
+    assert np.median(d2[~bad2]) < 0.02 and np.percentile(d2, 90) < 0.25

@@ -1,0 +1,76 @@
+"""Bad-subset refill (thin-plate RBF per mask component, then scattered
+interpolation of leftovers) against closed forms and invariants."""
+import numpy as np
+
+from staq_dic_amd.refill import refill_bad, thinplate_eval, thinplate_fit
+
+
+def test_thinplate_interpolates_and_reproduces_linear():
+    rng = np.random.default_rng(0)
+    c = rng.uniform(0, 100, (60, 2)).round()
+    c = np.unique(c, axis=0)
+    y = np.stack([np.sin(c[:, 0] / 17.0) + c[:, 1] / 50.0, 0.3 + 0.01 * c[:, 0] - 0.02 * c[:, 1]], 1)
+    coeff = thinplate_fit(c, y)
+    np.testing.assert_allclose(thinplate_eval(c, coeff, c), y, atol=1e-8)
+    # the linear column is reproduced everywhere with zero radial weights
+    p = rng.uniform(0, 100, (40, 2))
+    np.testing.assert_allclose(thinplate_eval(c, coeff, p)[:, 1],
+                               0.3 + 0.01 * p[:, 0] - 0.02 * p[:, 1], atol=1e-8)
+    np.testing.assert_allclose(coeff[:len(c), 1], 0.0, atol=1e-9)
+    # the radial part: phi(r) = r^2 log(r + 1) (rbfcreate.m:195)
+    c2 = np.array([[0.0, 0.0], [3.0, 4.0], [6.0, 0.0], [0.0, 8.0]])
+    y2 = np.array([[1.0], [0.0], [0.0], [0.0]])
+    w = thinplate_fit(c2, y2)
+    r = 5.0
+    val = w[0, 0] * 25 * np.log(r + 1) + w[1, 0] * 0 + w[2, 0] * 25 * np.log(6) + \
+        w[3, 0] * (9 + 16) * np.log(6) + w[4, 0] + 3 * w[5, 0] + 4 * w[6, 0]
+    np.testing.assert_allclose(val, 0.0, atol=1e-10)
+
+
+def test_refill_recovers_linear_fields_per_component(ref_frames):
+    """Two mask components carrying different linear fields: each component is
+    refilled from its own good nodes only (8-connected components of the mask,
+    LocalICGNQuadtree.m:189-195)."""
+    c = ref_frames["coords"]
+    N = c.shape[0]
+    mask = np.ones((400, 400))
+    mask[199:201, :] = 0  # split the image into two components along x = 200
+    left = np.round(c[:, 0]) < 200
+    U = np.where(left[:, None], np.stack([0.5 + 0.01 * c[:, 0], -0.2 + 0.02 * c[:, 1]], 1),
+                 np.stack([-1.0 - 0.03 * c[:, 1], 2.0 + 0.005 * c[:, 0]], 1)).ravel()
+    F = np.where(left[:, None], np.tile([0.01, 0.0, 0.0, 0.02], (N, 1)),
+                 np.tile([0.0, 0.005, -0.03, 0.0], (N, 1))).ravel()
+    rng = np.random.default_rng(3)
+    bad = rng.random(N) < 0.15
+    on_line = np.isin(np.round(c[:, 0]), [199, 200])
+    Ub = U.copy().reshape(N, 2)
+    Fb = F.copy().reshape(N, 4)
+    Ub[bad] = np.nan
+    Fb[bad] = np.nan
+    Uf, Ff = refill_bad(Ub.ravel(), Fb.ravel(), c, mask)
+    ok = ~on_line
+    np.testing.assert_allclose(Uf.reshape(N, 2)[ok], U.reshape(N, 2)[ok], atol=1e-6)
+    np.testing.assert_allclose(Ff.reshape(N, 4)[ok], F.reshape(N, 4)[ok], atol=1e-8)
+    assert np.isfinite(Uf).all() and np.isfinite(Ff).all()
+
+
+def test_refill_leftovers_outside_every_component(ref_frames):
+    """Nodes whose rounded position is outside the mask keep NaN after the RBF
+    stage and are then interpolated from the non-NaN nodes (:313-330); for a
+    linear field the interpolation inside the node hull is exact."""
+    c = ref_frames["coords"]
+    N = c.shape[0]
+    mask = ref_frames["mask"]
+    U = np.stack([0.1 + 0.002 * c[:, 0] - 0.001 * c[:, 1], -0.3 + 0.004 * c[:, 1]], 1).ravel()
+    rc = np.round(c).astype(int)
+    out = mask[rc[:, 0] - 1, rc[:, 1] - 1] == 0
+    Ub = U.copy().reshape(N, 2)
+    Ub[out] = np.nan
+    Uf, _ = refill_bad(Ub.ravel(), None, c, mask)
+    assert np.isfinite(Uf).all()
+    np.testing.assert_allclose(Uf.reshape(N, 2)[~out], U.reshape(N, 2)[~out], atol=1e-6)
+    from scipy.spatial import Delaunay
+    hull = Delaunay(c[~out]).find_simplex(c[out]) >= 0
+    if hull.any():
+        np.testing.assert_allclose(Uf.reshape(N, 2)[out][hull], U.reshape(N, 2)[out][hull],
+                                   atol=1e-9)

@@ -20,6 +20,7 @@ import time
 import numpy as np
 
 from . import _lib
+from .outliers import funRemoveOutliers
 from .refill import refill_bad
 from ._lib import Batch, Context, IN_FOLD, IN_U0, IN_UOLD, IN_VDUAL, d, ip, lp
 
@@ -130,7 +131,7 @@ def IntegerSearch(ImgRef, ImgDef, file_name=None, DICpara=None, ctx=None):
     whole-field mode (InitFFTSearchMethod = 1).  The search radius is
     DICpara['SizeOfFFTSearchRegion'] (the reference hard-codes 6,
     IntegerSearch.m:49).  Grid points on a hole of DICpara['ImgRefMask'] are
-    NaN (:69-77)."""
+    NaN (:69-77); the median-test outlier filter follows (:116-119)."""
     P = dict(DICpara or {})
     H, W = np.shape(ImgRef)
     roi = P.get("gridxyROIRange", {"gridx": [1, H], "gridy": [1, W]})
@@ -143,6 +144,9 @@ def IntegerSearch(ImgRef, ImgDef, file_name=None, DICpara=None, ctx=None):
         hole = (m == 0).reshape(x0.shape)
         u = np.where(hole, np.nan, u)
         v = np.where(hole, np.nan, v)
+    # median-test outlier filter, threshold 100 (IntegerSearch.m:116-119)
+    if np.isfinite(u).sum() >= 4 and min(np.shape(u)) >= 2:
+        u, v, cc, _, _, _ = funRemoveOutliers(u, v, cc, 0, 100)
     P["InitFFTSearchMethod"] = 1
     P["SizeOfFFTSearchRegion"] = r
     return P, x0, y0, u, v, cc

@@ -510,3 +510,28 @@ def test_reference_level_local_steps(ctx, ref_frames):
     bad2, _ = classify_subpb1(conv2)
     d2 = np.hypot(U2[0::2] - r["U1"][0::2], U2[1::2] - r["U1"][1::2])
     assert np.median(d2[~bad2]) < 0.02 and np.percentile(d2, 90) < 0.25
+
+
+def test_reference_level_integer_search(ctx, oracle, ref_frames):
+    """IntegerSearch as the reference's driver calls it (g and f swapped,
+    radius 6, hole points NaN, median-test outlier filter): the device search
+    followed by the host filter equals the oracle search followed by the same
+    filter."""
+    from staq_dic_amd import IntegerSearch
+    from staq_dic_amd.outliers import funRemoveOutliers
+    r = ref_frames
+    H, W = r["f"].shape
+    P = {"winsize": 32, "winstepsize": 16, "SizeOfFFTSearchRegion": 6,
+         "gridxyROIRange": {"gridx": [1, H], "gridy": [1, W]}, "ImgRefMask": r["mask"]}
+    _print("IntegerSearch on the reference frames")
+    P2, x0, y0, u, v, cc = IntegerSearch(r["g"], r["f"], None, P, ctx=ctx)
+    xo, yo, uo, vo, cco = oracle.integer_search(r["g"], r["f"], 6, [1, H], [1, W], 32, 16)
+    hole = r["mask"][xo.astype(int) - 1, yo.astype(int) - 1] == 0
+    uo = np.where(hole, np.nan, uo)
+    vo = np.where(hole, np.nan, vo)
+    uo, vo, _, _, _, _ = funRemoveOutliers(uo, vo, {}, 0, 100)
+    np.testing.assert_array_equal(np.isnan(u), np.isnan(uo))
+    fin = np.isfinite(uo)
+    np.testing.assert_allclose(u[fin], uo[fin], atol=1e-3 + 1e-9)
+    np.testing.assert_allclose(v[fin], vo[fin], atol=1e-3 + 1e-9)
+    assert np.isnan(u[hole]).all() and fin.sum() > 100

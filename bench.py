@@ -176,9 +176,13 @@ def main():
     bdom = bins[kdom]
     bytes_launch = float(bytes_bin[kdom])
     achieved = bytes_launch / (kms[kdom] / 1e3) / 1e9
-    nw = 1 if bdom["bs6"] == 0 else -bdom["bs6"]
-    kname = (f"icgn_wave_kernel<{bdom['ppt']},6,{nw}>" if bdom["bs6"] <= 0
-             else f"icgn_kernel<{bdom['bs6']},6>")
+    f6 = bdom["bs6"]
+    if f6 in (1, 2, 4):
+        kname = f"icgn_lean_kernel<{bdom['ppt']},6,{f6}>"
+    elif f6 <= 0:
+        kname = f"icgn_wave_kernel<{bdom['ppt']},6,{1 if f6 == 0 else -f6}>"
+    else:
+        kname = f"icgn_kernel<{f6},6>"
     step_bytes = float(bytes_bin.sum())
     log(f"[bench] dominant launch {kname}: {kms[kdom]:.3f} ms, {bytes_launch / 1e6:.1f} MB, "
         f"{achieved:.1f} GB/s; launches {[round(float(x), 3) for x in kms]} ms")

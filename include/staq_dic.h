@@ -51,7 +51,9 @@ typedef struct staq_params {
                         whole-window statistics, DfDx==0 mask rule)                      */
     int block_size;  /* kernel form for every bin: 0 = the batch's (tuned) schedule;
                         64/128/256/512 = workgroup kernel of that size; -1/-2/-4 = that
-                        many wavefronts per subset                                      */
+                        many wavefronts per subset; 1/2/4 = the lean kernel for windows
+                        without masked pixels with that many wavefronts (a subset it
+                        cannot finish goes to the fallback launch)                       */
     int persistent;  /* 1: persistent grid pulling subsets from a device work queue      */
 } staq_params;
 
@@ -171,13 +173,16 @@ int staq_batch_step_timed(staq_batch *b, const staq_params *p, int mode, float *
 int staq_batch_icgn6_per_subset(staq_batch *b, const staq_params *p, double *kernel_ms_sum);
 /* Copy outputs to host (any pointer may be NULL). */
 int staq_batch_get_output(staq_batch *b, double *U, double *F, double *Hpar, int *status);
-/* Reorder each window bin longest-first, predicted from the previous step's
- * iteration counts and window sizes (scheduling only; results are unchanged). */
+/* Scheduling from the previous step (results are unchanged): subsets the
+ * lean kernel handed to the fallback launch move to the general kernel's bins
+ * (the bins are then rebuilt with their default forms and a captured graph is
+ * dropped), and each bin is ordered longest-first, predicted from the
+ * iteration counts and window sizes. */
 int staq_batch_reorder(staq_batch *b);
 /* Feed the 6-DOF outputs (U, F) of this batch to its 2-DOF inputs (UOld, FOld)
  * on the device, vdual = 0 — the first AL step with an identity global step. */
 int staq_batch_chain_outputs(staq_batch *b);
-/* Time each candidate workgroup size per window bin on the resident inputs
+/* Time each candidate kernel form per window bin on the resident inputs
  * and keep the fastest (mode 6 or 2). */
 int staq_batch_autotune(staq_batch *b, const staq_params *p, int mode);
 /* Capture one local step (mode 6 or 2) into a hipGraph, then replay it. */
@@ -192,14 +197,15 @@ typedef struct staq_batch_ptrs {
 } staq_batch_ptrs;
 int staq_batch_get_ptrs(staq_batch *b, staq_batch_ptrs *out);
 /* Number of window bins and, for bin k, its window size cap, item count,
- * workgroup size chosen for mode 6 / mode 2 (0 = one wavefront per subset) and
- * pixels per lane of the wavefront kernel (0 = the bin has none). */
+ * kernel form chosen for mode 6 / mode 2 (as staq_batch_set_form) and pixels
+ * per lane of the wavefront kernel of the mode-6 form (0 = workgroup kernel). */
 int staq_batch_bins(staq_batch *b, int *nbins, int *wcap, long *count, int *bs6, int *bs2,
                     int *ppt, int maxbins);
 
 /* Set the kernel form of bin `bin` for mode 6 or 2 (0 = one wavefront per
- * subset, -1/-2/-4 = that many wavefronts, 64..512 = workgroup kernel), as
- * staq_batch_autotune would; drops a captured graph. */
+ * subset, -1/-2/-4 = that many wavefronts, 1/2/4 = the lean kernel for windows
+ * without masked pixels with that many wavefronts, 64..512 = workgroup
+ * kernel), as staq_batch_autotune would; drops a captured graph. */
 int staq_batch_set_form(staq_batch *b, int bin, int mode, int form);
 /* Bin index of every subset (N entries, in the caller's subset order).  Bins
  * group similar window sizes; windows that touch the mask of their frame at

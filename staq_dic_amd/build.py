@@ -9,7 +9,7 @@ import sys
 HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libstaqdic.so")
-SOURCES = ["capi.hip", "icgn.hip", "icgn_wave.hip", "gather.hip", "search.hip"]
+SOURCES = ["capi.hip", "icgn.hip", "icgn_wave.hip", "icgn_lean.hip", "gather.hip", "search.hip"]
 ARCH = os.environ.get("STAQ_OFFLOAD_ARCH", "gfx950")
 
 HIPCC_FLAGS = [

@@ -115,8 +115,14 @@ struct Bin {
 struct staq_batch {
     staq_ctx *ctx = nullptr;
     long N = 0;
-    DevBuf coords, U0, UOld, FOld, vdual, U, F, H, status, wx, wy, slot, order, ints;
+    DevBuf coords, U0, UOld, FOld, vdual, U, F, H, status, wx, wy, slot, order, ints, handed;
     std::vector<Bin> bins;
+    // host copies behind the bins: window sizes, zeros of f*mask in each
+    // window, the grown window's pixels and pixel slots, and whether the
+    // subset goes to the general kernel (mask zeros, or handed back by the
+    // lean kernel in an earlier step)
+    std::vector<int> hx, hy, cnt, gnp, gppt;
+    std::vector<char> general;
     int wmax = 0;
     int fb_npix_cap = 0, fb_tile_cap = 0;
     bool has_slot = false;
@@ -502,6 +508,120 @@ static int default_bs(int npix)
     return 256;
 }
 
+// Kernel forms of a bin: 0 / -1 = one wavefront per subset, -2 / -4 = two or
+// four (icgn_wave.hip); 1 / 2 / 4 = the lean kernel with that many wavefronts,
+// for windows without masked pixels (icgn_lean.hip); 64 .. 512 = one workgroup
+// of that size per subset (icgn.hip).
+static bool valid_form(int f)
+{
+    return f == 0 || f == -1 || f == -2 || f == -4 || f == 1 || f == 2 || f == 4 || f == 64 ||
+           f == 128 || f == 256 || f == 512;
+}
+static bool lean_form(int f) { return f == 1 || f == 2 || f == 4; }
+// wavefronts per subset of a wavefront form, 0 for a workgroup form
+static int form_nw(int f)
+{
+    if (lean_form(f)) return f;
+    return f <= 0 ? (f == 0 ? 1 : -f) : 0;
+}
+
+
+// Bins of similar window size; windows for the general kernel apart, grouped
+// by the wavefront kernel their grown window needs (the growth rules of
+// funICGNQuadtree.m:66-70 and funICGN_Subpb1.m:76-80); largest windows first,
+// so that the hardware dispatcher fills each bin's tail with short subsets.
+// (Re)builds the bins, their default forms, the side streams and events of
+// the concurrent launches, the queue heads and the processing order.
+static int build_bins(staq_batch *b)
+{
+    staq_ctx *ctx = b->ctx;
+    const long N = b->N;
+    const std::vector<int> &hx = b->hx, &hy = b->hy, &gnp = b->gnp, &gppt = b->gppt;
+    const std::vector<char> &gen = b->general;
+    std::vector<int> order(N);
+    for (long i = 0; i < N; ++i) order[i] = (int)i;
+    auto key = [&](int i) {
+        return 64 * ((std::max(hx[i], hy[i]) + 7) / 8) + (gen[i] ? 32 + gppt[i] : 0);
+    };
+    std::stable_sort(order.begin(), order.end(), [&](int x, int y) { return key(x) > key(y); });
+    if (b->gexec) {  // a captured graph holds the old schedule
+        hipGraphExecDestroy(b->gexec);
+        hipGraphDestroy(b->graph);
+        b->gexec = nullptr;
+        b->graph = nullptr;
+    }
+    b->bins.clear();
+    long i0 = 0;
+    int fbx = 0, fby = 0;
+    while (i0 < N) {
+        const int k = key(order[i0]);
+        long i1 = i0;
+        Bin bin;
+        while (i1 < N && key(order[i1]) == k) {
+            bin.wxcap = std::max(bin.wxcap, hx[order[i1]]);
+            bin.wycap = std::max(bin.wycap, hy[order[i1]]);
+            bin.wave_npix = std::max(bin.wave_npix, gnp[order[i1]]);
+            ++i1;
+        }
+        bin.item0 = i0;
+        bin.n = i1 - i0;
+        bin.masked = (k & 32) != 0;
+        bin.npix_cap = (bin.wxcap + 1) * (bin.wycap + 1);
+        bin.tile_cap = tile_side(bin.wxcap + 1) * tile_side(bin.wycap + 1);
+        // the wave kernel of a bin that touches the mask covers the grown
+        // windows of its smallest class too; a subset that outgrows its bin
+        // (or grows although classified clear) defers to the fallback launch
+        bin.wave_npix = std::max(bin.wave_npix, bin.npix_cap);
+        bin.ppt = icgn_wave_ppt(bin.wave_npix, 1);
+        // default form (measured on the hole workload): bins without masked
+        // pixels take the lean kernel, one wavefront per subset while five
+        // pixel slots hold the window, else four (two) wavefronts; bins for
+        // the general kernel take four (two, one) wavefronts; windows beyond
+        // that take the workgroup kernel
+        int form = default_bs(bin.npix_cap);
+        const int g = bin.masked ? -1 : 1;
+        if (!bin.masked && bin.ppt && bin.ppt <= 5)
+            form = 1;
+        else if (icgn_wave_ppt(bin.wave_npix, 4))
+            form = 4 * g;
+        else if (icgn_wave_ppt(bin.wave_npix, 2))
+            form = 2 * g;
+        else if (bin.ppt)
+            form = bin.masked ? 0 : 1;
+        bin.bs[0] = bin.bs[1] = form;
+        b->bins.push_back(bin);
+        fbx = std::max(fbx, grown_window(bin.wxcap));
+        fby = std::max(fby, grown_window(bin.wycap));
+        i0 = i1;
+    }
+    b->fb_npix_cap = (fbx + 1) * (fby + 1);
+    b->fb_tile_cap = tile_side(fbx + 1) * tile_side(fby + 1);
+    const size_t nb = b->bins.size();
+    HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+    while (b->side.size() + 1 > std::max<size_t>(nb, 1)) {
+        hipStreamSynchronize(b->side.back());
+        hipStreamDestroy(b->side.back());
+        b->side.pop_back();
+    }
+    while (b->side.size() + 1 < nb) {
+        hipStream_t st = nullptr;
+        HIPCHK(ctx, hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
+        b->side.push_back(st);
+    }
+    while (b->ev.size() > nb) {
+        hipEventDestroy(b->ev.back());
+        b->ev.pop_back();
+    }
+    while (b->ev.size() < nb) {
+        hipEvent_t e = nullptr;
+        HIPCHK(ctx, hipEventCreateWithFlags(&e, hipEventDisableTiming));
+        b->ev.push_back(e);
+    }
+    HIPCHK(ctx, b->ints.ensure((nb + 2) * 4));
+    HIPCHK(ctx, hipMemcpy(b->order.p, order.data(), N * 4, hipMemcpyHostToDevice));
+    return STAQ_OK;
+}
+
 extern "C" {
 
 int staq_batch_create(staq_ctx *ctx, staq_batch **out, long N, const double *coords,
@@ -510,7 +630,7 @@ int staq_batch_create(staq_ctx *ctx, staq_batch **out, long N, const double *coo
     if (!ctx || !out || N <= 0 || !coords || !wx) return fail(ctx, STAQ_EINVAL, "bad batch arguments");
     *out = nullptr;
     hipSetDevice(ctx->device);
-    std::vector<int> hx(N), hy(N), hs(N, 0), order(N);
+    std::vector<int> hx(N), hy(N), hs(N, 0);
     int wmax = 0;
     for (long i = 0; i < N; ++i) {
         hx[i] = (int)wx[i];
@@ -521,7 +641,6 @@ int staq_batch_create(staq_ctx *ctx, staq_batch **out, long N, const double *coo
         if (hs[i] < 0 || (size_t)hs[i] >= ctx->frames.size() || !ctx->frames[hs[i]].set)
             return fail(ctx, STAQ_ENOFRAME, "subset refers to a frame slot that is not set");
         wmax = std::max(wmax, std::max(hx[i], hy[i]));
-        order[i] = (int)i;
     }
     staq_batch *b = new staq_batch();
     b->ctx = ctx;
@@ -569,15 +688,16 @@ int staq_batch_create(staq_ctx *ctx, staq_batch **out, long N, const double *coo
         staq_batch_destroy(b);
         return rc;
     }
-    // bins of similar window size; windows that touch the mask apart, grouped
-    // by the wavefront kernel their grown window needs (the growth rules of
-    // funICGNQuadtree.m:66-70 and funICGN_Subpb1.m:76-80); largest windows
-    // first, so that the hardware dispatcher fills each bin's tail with short
-    // subsets
+    // the window each subset's kernel must hold: windows that touch the mask
+    // grow at set-up (funICGNQuadtree.m:66-70, funICGN_Subpb1.m:76-80)
     std::vector<int> gppt(N, 0), gnp(N, 0);
     for (long i = 0; i < N; ++i) {
         gnp[i] = (hx[i] + 1) * (hy[i] + 1);
-        if (cnt[i] == 0) continue;
+        if (cnt[i] == 0) {
+            const int pg = icgn_wave_ppt(gnp[i], 1);
+            gppt[i] = pg ? pg : 31;
+            continue;
+        }
         const double n0 = (double)(hx[i] + 1) * (hy[i] + 1);
         int gx = hx[i], gy = hy[i];
         if (cnt[i] > 0 && cnt[i] < 0.4 * n0) {
@@ -590,63 +710,18 @@ int staq_batch_create(staq_ctx *ctx, staq_batch **out, long N, const double *coo
         const int pg = icgn_wave_ppt(gnp[i], 1);
         gppt[i] = pg ? pg : 31;
     }
-    auto key = [&](int i) {
-        return 64 * ((std::max(hx[i], hy[i]) + 7) / 8) + (cnt[i] ? 32 + gppt[i] : 0);
-    };
-    std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return key(a) > key(b); });
-    long i0 = 0;
-    int fbx = 0, fby = 0;
-    while (i0 < N) {
-        const int k = key(order[i0]);
-        long i1 = i0;
-        Bin bin;
-        while (i1 < N && key(order[i1]) == k) {
-            bin.wxcap = std::max(bin.wxcap, hx[order[i1]]);
-            bin.wycap = std::max(bin.wycap, hy[order[i1]]);
-            bin.wave_npix = std::max(bin.wave_npix, gnp[order[i1]]);
-            ++i1;
-        }
-        bin.item0 = i0;
-        bin.n = i1 - i0;
-        bin.masked = (k & 32) != 0;
-        bin.npix_cap = (bin.wxcap + 1) * (bin.wycap + 1);
-        bin.tile_cap = tile_side(bin.wxcap + 1) * tile_side(bin.wycap + 1);
-        // the wave kernel of a bin that touches the mask covers the grown
-        // windows of its smallest class too; a subset that outgrows its bin
-        // (or grows although classified clear) defers to the fallback launch
-        bin.wave_npix = std::max(bin.wave_npix, bin.npix_cap);
-        bin.ppt = icgn_wave_ppt(bin.wave_npix, 1);
-        // default form: one wavefront while its pixel slots stay within 9
-        // (branch-free slots), else two or four wavefronts, else the
-        // workgroup kernel
-        int form = default_bs(bin.npix_cap);
-        if (bin.ppt && bin.ppt <= 9)
-            form = 0;
-        else if (icgn_wave_ppt(bin.wave_npix, 2))
-            form = -2;
-        else if (icgn_wave_ppt(bin.wave_npix, 4))
-            form = -4;
-        bin.bs[0] = bin.bs[1] = form;
-        b->bins.push_back(bin);
-        fbx = std::max(fbx, grown_window(bin.wxcap));
-        fby = std::max(fby, grown_window(bin.wycap));
-        i0 = i1;
-    }
-    b->fb_npix_cap = (fbx + 1) * (fby + 1);
-    b->fb_tile_cap = tile_side(fbx + 1) * tile_side(fby + 1);
-    const size_t nb = b->bins.size();
-    for (size_t k = 1; k < nb; ++k) {
-        hipStream_t st = nullptr;
-        if (hipStreamCreateWithFlags(&st, hipStreamNonBlocking) == hipSuccess) b->side.push_back(st);
-    }
-    for (size_t k = 0; k < nb; ++k) {
-        hipEvent_t e = nullptr;
-        if (hipEventCreateWithFlags(&e, hipEventDisableTiming) == hipSuccess) b->ev.push_back(e);
-    }
-    chk(b->ints.ensure((nb + 2) * 4), "counters");
+    b->hx = hx;
+    b->hy = hy;
+    b->cnt = cnt;
+    b->gnp = gnp;
+    b->gppt = gppt;
+    b->general.assign(N, 0);
+    for (long i = 0; i < N; ++i) b->general[i] = cnt[i] != 0;
+    chk(b->handed.ensure(N * 4), "handed");
+    if (rc == STAQ_OK) rc = build_bins(b);
     if (rc == STAQ_OK) {
         hipStream_t s = ctx->stream;
-        chk(hipMemcpyAsync(b->order.p, order.data(), N * 4, hipMemcpyHostToDevice, s), "order");
+        chk(hipMemsetAsync(b->handed.p, 0, N * 4, s), "zero");
         chk(hipMemsetAsync(b->U0.p, 0, N * 16, s), "zero");
         chk(hipMemsetAsync(b->UOld.p, 0, N * 16, s), "zero");
         chk(hipMemsetAsync(b->FOld.p, 0, N * 32, s), "zero");
@@ -740,8 +815,8 @@ int staq_batch_bins(staq_batch *b, int *nbins, int *wcap, long *count, int *bs6,
         if (bs6) bs6[k] = b->bins[k].bs[0];
         if (bs2) bs2[k] = b->bins[k].bs[1];
         if (ppt) {
-            const int f = b->bins[k].bs[0];
-            ppt[k] = f <= 0 ? icgn_wave_ppt(b->bins[k].wave_npix, f == 0 ? 1 : -f) : 0;
+            const int nw = form_nw(b->bins[k].bs[0]);
+            ppt[k] = nw ? icgn_wave_ppt(b->bins[k].wave_npix, nw) : 0;
         }
     }
     return STAQ_OK;
@@ -753,9 +828,9 @@ extern "C" int staq_batch_set_form(staq_batch *b, int bin, int mode, int form)
 {
     if (!b || bin < 0 || bin >= (int)b->bins.size() || (mode != 6 && mode != 2))
         return STAQ_EINVAL;
-    if (!(form == 0 || form == -1 || form == -2 || form == -4 || form == 64 || form == 128 ||
-          form == 256 || form == 512))
-        return fail(b->ctx, STAQ_EINVAL, "kernel form must be 0, -1, -2, -4, 64, 128, 256 or 512");
+    if (!valid_form(form))
+        return fail(b->ctx, STAQ_EINVAL,
+                    "kernel form must be 0, -1, -2, -4, 1, 2, 4, 64, 128, 256 or 512");
     b->bins[bin].bs[mode == 6 ? 0 : 1] = form;
     if (b->gexec) {  // a captured graph holds the old schedule
         hipGraphExecDestroy(b->gexec);
@@ -796,6 +871,7 @@ static IcgnArgs base_args(staq_batch *b, const staq_params *p)
     a.F = b->F.as<double>();
     a.H21 = b->H.as<double>();
     a.status = b->status.as<int>();
+    a.handed = b->handed.as<int>();
     a.N = b->N;
     a.tol = p->tol;
     a.mu = p->mu;
@@ -849,15 +925,16 @@ static int enqueue_step(staq_batch *b, const staq_params *p, int mode, hipStream
         a.defer_out = ints + nb + 1;
         int bs = bs_override != kNoOverride ? bs_override
                                              : (p->block_size ? p->block_size : bin.bs[mi]);
-        if (!(bs == 0 || bs == -1 || bs == -2 || bs == -4 || bs == 64 || bs == 128 || bs == 256 ||
-              bs == 512))
-            return fail(ctx, STAQ_EINVAL, "block_size must be 0, -1, -2, -4, 64, 128, 256 or 512");
-        const int nw = bs <= 0 ? (bs == 0 ? 1 : -bs) : 0;
+        if (!valid_form(bs))
+            return fail(ctx, STAQ_EINVAL,
+                        "block_size must be 0, -1, -2, -4, 1, 2, 4, 64, 128, 256 or 512");
+        const int nw = form_nw(bs);
+        const bool lean = lean_form(bs);
         const int wppt = nw ? icgn_wave_ppt(bin.wave_npix, nw) : 0;
-        if (bs <= 0 && !wppt) bs = default_bs(bin.npix_cap);
+        if (nw && !wppt) bs = default_bs(bin.npix_cap);
         int grid = (int)bin.n;
         if (timed) HIPCHK(ctx, hipEventRecord(b->tev[2 * k], sk));
-        if (bs <= 0) {
+        if (nw && wppt) {
             // nw wavefronts per subset; the g tile holds the warped window
             // of the largest window the kernel takes, plus its margin
             const int side = tile_side((int)std::sqrt(64.0 * wppt * nw));
@@ -866,7 +943,10 @@ static int enqueue_step(staq_batch *b, const staq_params *p, int mode, hipStream
                 a.work_counter = ints + k;
                 grid = occupancy_grid(64 * nw, sizeof(float) * a.tile_cap + 8192, bin.n);
             }
-            HIPCHK(ctx, icgn_wave_launch(a, mode, wppt, nw, grid, sk));
+            if (lean)
+                HIPCHK(ctx, icgn_lean_launch(a, mode, wppt, nw, grid, sk));
+            else
+                HIPCHK(ctx, icgn_wave_launch(a, mode, wppt, nw, grid, sk));
         } else {
             if (p->persistent) {
                 a.work_counter = ints + k;
@@ -946,7 +1026,7 @@ int staq_batch_icgn6_per_subset(staq_batch *b, const staq_params *p, double *ker
     // Every subset gets the largest caps so that no subset defers: this is the
     // reference's one-subset-per-call structure with one workgroup per call.
     const int npix_cap = b->fb_npix_cap, tile_cap = b->fb_tile_cap;
-    const int bs = p->block_size > 0 ? p->block_size : 256;
+    const int bs = p->block_size >= 64 ? p->block_size : 256;
     hipEvent_t e0 = nullptr, e1 = nullptr;
     if (kernel_ms_sum) {
         *kernel_ms_sum = 0.0;
@@ -976,13 +1056,33 @@ int staq_batch_icgn6_per_subset(staq_batch *b, const staq_params *p, double *ker
 
 int staq_batch_reorder(staq_batch *b)
 {
-    // Longest-processing-time-first order inside each bin, predicted from the
-    // last step: iterations used (101 for a subset that ran out of
-    // iterations) times window pixels.  Results do not depend on the order.
+    // Subsets the lean kernel handed to the fallback launch in the last step
+    // (a zero of the warped g, degenerate statistics) move to the general
+    // kernel's bins, so that later steps solve them in their bin instead of
+    // after every bin.  Then longest-processing-time-first order inside each
+    // bin, predicted from the last step: iterations used (101 for a subset
+    // that ran out of iterations) times window pixels.  Results do not
+    // depend on the order.
     if (!b) return STAQ_EINVAL;
     staq_ctx *ctx = b->ctx;
     hipSetDevice(ctx->device);
     const long N = b->N;
+    {
+        std::vector<int> hd(N);
+        HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+        HIPCHK(ctx, hipMemcpy(hd.data(), b->handed.p, N * 4, hipMemcpyDeviceToHost));
+        bool moved = false;
+        for (long i = 0; i < N; ++i)
+            if (hd[i] && !b->general[i]) {
+                b->general[i] = 1;
+                moved = true;
+            }
+        if (moved) {
+            const int rc = build_bins(b);
+            if (rc != STAQ_OK) return rc;
+            HIPCHK(ctx, hipMemset(b->handed.p, 0, N * 4));
+        }
+    }
     std::vector<int> st(N), order(N), wx(N), wy(N);
     HIPCHK(ctx, hipMemcpy(st.data(), b->status.p, N * 4, hipMemcpyDeviceToHost));
     HIPCHK(ctx, hipMemcpy(order.data(), b->order.p, N * 4, hipMemcpyDeviceToHost));
@@ -1032,7 +1132,7 @@ int staq_batch_autotune(staq_batch *b, const staq_params *p, int mode)
         HIPCHK(ctx, hipEventCreate(&e));
         b->tev.push_back(e);
     }
-    const int cand[7] = {0, -2, -4, 64, 128, 256, 512};
+    const int cand[10] = {0, -2, -4, 1, 2, 4, 64, 128, 256, 512};
     const int mi = mode == 6 ? 0 : 1;
     staq_params q = *p;
     q.block_size = 0;
@@ -1061,7 +1161,8 @@ int staq_batch_autotune(staq_batch *b, const staq_params *p, int mode)
         int best_bs = cur;
         for (int c : cand) {
             if (c == cur) continue;
-            if (c <= 0 && !icgn_wave_ppt(bin.wave_npix, c == 0 ? 1 : -c)) continue;
+            if (form_nw(c) && !icgn_wave_ppt(bin.wave_npix, form_nw(c))) continue;
+            if (lean_form(c) && bin.masked) continue;  // its subsets would all defer
             if (c == 512 && bin.npix_cap < 1500) continue;
             if (c == 64 && bin.npix_cap > 4500) continue;
             bin.bs[mi] = c;

@@ -36,6 +36,23 @@ __device__ __forceinline__ T gld(const T *p, size_t i)
     return ((const __attribute__((address_space(1))) T *)p)[i];
 }
 
+// One frame descriptor, field by field through the global address space (the
+// loads are then known to be the same in every lane when slot is).
+__device__ __forceinline__ FrameDev load_frame(const FrameDev *frames, int slot)
+{
+    const __attribute__((address_space(1))) FrameDev *q =
+        (const __attribute__((address_space(1))) FrameDev *)frames + slot;
+    FrameDev fr;
+    fr.f = q->f;
+    fr.g = q->g;
+    fr.mask = q->mask;
+    fr.dfdx = q->dfdx;
+    fr.dfdy = q->dfdy;
+    fr.H = q->H;
+    fr.W = q->W;
+    return fr;
+}
+
 __device__ __forceinline__ float fidx(const float *a, int H, int x, int y)
 {
     // 1-based (x, y) in an H-row column-major array
@@ -79,7 +96,8 @@ __device__ __forceinline__ double wave_sum_dpp_f64(double v)
 }
 
 // Sum over the 64 lanes of a wavefront, result in every lane: DPP row shifts
-// build each 16-lane row's sum in its last lane, four readlanes add the rows.
+// build each 16-lane row's sum in its last lane, two row broadcasts add the
+// rows into lane 63 ((r3 + r2) + (r1 + r0)), one readlane returns it.
 __device__ __forceinline__ float wave_sum_dpp(float v)
 {
     int x = __float_as_int(v);
@@ -91,11 +109,11 @@ __device__ __forceinline__ float wave_sum_dpp(float v)
     x = __float_as_int(v);
     v += __int_as_float(__builtin_amdgcn_update_dpp(0, x, 0x118, 0xf, 0xf, true));  // row_shr:8
     x = __float_as_int(v);
-    const float r0 = __int_as_float(__builtin_amdgcn_readlane(x, 15));
-    const float r1 = __int_as_float(__builtin_amdgcn_readlane(x, 31));
-    const float r2 = __int_as_float(__builtin_amdgcn_readlane(x, 47));
-    const float r3 = __int_as_float(__builtin_amdgcn_readlane(x, 63));
-    return (r0 + r1) + (r2 + r3);
+    // rows 1 and 3 add lane 15 of rows 0 and 2; rows 2 and 3 then add lane 31
+    v += __int_as_float(__builtin_amdgcn_update_dpp(0, x, 0x142, 0xa, 0xf, false));  // row_bcast:15
+    x = __float_as_int(v);
+    v += __int_as_float(__builtin_amdgcn_update_dpp(0, x, 0x143, 0xc, 0xf, false));  // row_bcast:31
+    return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), 63));
 }
 
 // Keys cubic-convolution weights, a = -0.5 (ba_interp2.cpp:204-213).

@@ -1,0 +1,588 @@
+// Lean wavefront-per-subset IC-GN kernel for subsets whose window holds no
+// masked pixel (gfx950).  It is the unmasked path of icgn_wave.hip and nothing
+// else, which keeps its register file small (several waves per SIMD instead of
+// one) and its iteration short (it is latency-bound: one subset's iterations
+// are a dependent chain):
+//
+//   * no bwselect regions, no window growth, no FP64 residual pass: a subset
+//     that meets any of them (a zero of f*mask in its window, a zero of the
+//     warped g under the quadtree rules, degenerate statistics, a Hessian that
+//     is not positive definite, a window beyond 64 * PPT * NW pixels) is left
+//     for the fallback launch (status -2), which solves it from the start;
+//   * per pixel slot the lane keeps df/dx, df/dy and the window offsets
+//     (x - x0, y - y0) in registers, so an iteration is the warp, the Keys
+//     gather from the resident g tile, eight sums, and the FP64 step;
+//   * the sums, the statistics, the Gauss-Newton step and the compose are the
+//     same operations in the same order as in icgn_wave.hip (shared through
+//     wave_common.h), so for these subsets both kernels give bit-identical
+//     results.
+//
+// Semantics: funICGNQuadtree.m:40-451 / funICGN.m:40-323 (6-DOF) and
+// funICGN_Subpb1.m:40-474 (2-DOF) on windows without masked pixels.
+#include "wave_common.h"
+
+namespace staq {
+
+namespace {
+
+struct LeanLds {
+    double Lp[21];   // packed Cholesky factor of H (column-major lower triangle)
+    double Ld[6];    // 1 / diag(Lp)
+    double H2i[4];   // 2-DOF: inverse of the damped 2x2 system
+    double FJ[6], SJ[6];
+    // exchange between the wavefronts of one subset (double-buffered)
+    float xf[2][4][24];
+    double xd[2][4][16];
+    int xi[2][4][8];
+};
+
+template <int PPT, int MODE, int NW>
+__global__ __launch_bounds__(64 * NW) void icgn_lean_kernel(IcgnArgs a)
+{
+    constexpr int NJ = MODE;
+    constexpr int CAP = 64 * PPT * NW;
+    extern __shared__ __align__(16) unsigned char smem[];
+    __shared__ LeanLds L;
+    __shared__ long item_sh;
+    float *tile = reinterpret_cast<float *>(smem);
+    const int tid = threadIdx.x;
+    const int lane = tid & 63;
+    const int wv = tid >> 6;
+    const bool quad = !(MODE == 6 && a.plain);
+    int par = 0;
+    auto pixel = [&](int k) { return lane + 64 * (k * NW + wv); };
+
+    for (long pass = 0;; ++pass) {
+        if (tid == 0)
+            item_sh = a.work_counter ? a.item0 + atomicAdd(a.work_counter, 1)
+                                     : (pass == 0 ? a.item0 + blockIdx.x : a.item0 + a.n_items);
+        __syncthreads();
+        const long item = item_sh;
+        if (item >= a.item0 + a.n_items) break;
+        const long idx = a.order ? gld(a.order, (size_t)(item)) : item;
+        const int slot = a.slot ? gld(a.slot, (size_t)(idx)) : 0;
+        const FrameDev fr = load_frame(a.frames, slot);
+        const int Hd = fr.H - 6;
+
+        double x0 = gld(a.coords, (size_t)(idx)), y0 = gld(a.coords, (size_t)(idx + a.N));
+        if (!a.plain) {
+            x0 = round(x0);  // LocalICGNQuadtree.m:69
+            y0 = round(y0);
+        }
+        const int wx = gld(a.wx, (size_t)(idx));
+        const int wy = MODE == 2 ? gld(a.wy, (size_t)(idx)) : wx;
+        double P[6];
+        if (MODE == 6) {
+            P[0] = P[1] = P[2] = P[3] = 0.0;
+            P[4] = gld(a.U0, (size_t)(2 * idx));
+            P[5] = gld(a.U0, (size_t)(2 * idx + 1));
+        } else {
+            for (int k = 0; k < 4; ++k) P[k] = gld(a.FOld, (size_t)(4 * idx + k));
+            P[4] = gld(a.UOld, (size_t)(2 * idx));
+            P[5] = gld(a.UOld, (size_t)(2 * idx + 1));
+        }
+        int state = 0, status = 0;  // 0 run, 1 loop done, 2 final status, 3 deferred
+        {
+            const double xa = x0 - wx * 0.5, xb = x0 + wx * 0.5, ya = y0 - wy * 0.5,
+                         yb = y0 + wy * 0.5;
+            const bool fits = xa == floor(xa) && ya == floor(ya) && xb == floor(xb) &&
+                              yb == floor(yb) && xa >= 4 && ya >= 4 && xb <= fr.H - 3 &&
+                              yb <= fr.W - 3;
+            if (!fits) {
+                state = 2;
+                status = ST_ERROR;
+            } else if ((wx + 1) * (wy + 1) > CAP || wx + 1 > 64 || wy + 1 > 64) {
+                state = 3;
+            }
+        }
+
+        // ------------------------------------------------------------ setup
+        const int nx = wx + 1, ny = wy + 1, npix = nx * ny;
+        const int xa = (int)x0 - wx / 2, ya = (int)y0 - wy / 2;
+        const float rnx = 1.0f / (float)nx;
+        const float hx = wx * 0.5f, hy = wy * 0.5f;
+        if (state == 0) {
+            // masked-pixel count (funICGNQuadtree.m:61 / funICGN.m:54)
+            int cnt = 0;
+#pragma unroll
+            for (int k = 0; k < PPT; ++k) {
+                const int p = pixel(k);
+                if (p < npix) {
+                    const int j = (int)(((float)p + 0.5f) * rnx), i = p - j * nx;
+                    const int x = xa + i, y = ya + j;
+                    if (MODE == 6 && a.plain)
+                        cnt += gld(fr.dfdx, (size_t)((x - 4) + (y - 4) * Hd)) == 0.f;
+                    else
+                        cnt += fidx(fr.f, fr.H, x, y) * fidx(fr.mask, fr.H, x, y) == 0.f;
+                }
+            }
+            int ci[1] = {(int)wave_sum_dpp((float)cnt)};
+            group_reduce<NW, 0, 0, 1>(L, par, nullptr, nullptr, ci);
+            const int count = ci[0];
+            const double n0 = (double)nx * ny;
+            const double cap = (MODE == 6 && a.plain) ? 0.5 * n0 : 0.4 * n0;
+            if (!(count < cap)) {
+                state = 2;
+                status = ST_MASKED;
+            } else if ((MODE == 6 && a.plain) ? count > 0.1 * n0 : count > 0) {
+                state = 3;  // window growth and mask regions: the general kernel
+            }
+        }
+
+        // per-slot df/dx, df/dy and window offsets; f statistics and
+        // steepest-descent sums in one pass about c0 = f at the centre
+        float fxr[PPT], fyr[PPT], dxr[PPT], dyr[PPT];
+        unsigned vbits = 0u;  // bit k: slot k holds a window pixel
+        double meanf = 0, bottomf = 0, sabs = 0, rel2 = 1, abs2 = 1, inv_init2 = 0;
+        bool init_ok = false;
+        const double tol2 = a.tol * a.tol;
+#pragma unroll
+        for (int k = 0; k < PPT; ++k) {
+            // a slot past the window repeats the last pixel and adds nothing
+            const int p = pixel(k);
+            const int pc = p < npix ? p : npix - 1;
+            const int j = (int)(((float)pc + 0.5f) * rnx), i = pc - j * nx;
+            dxr[k] = (float)i - hx;
+            dyr[k] = (float)j - hy;
+            fxr[k] = fyr[k] = 0.f;
+            vbits |= p < npix ? (1u << k) : 0u;
+        }
+        if (state == 0) {
+            const float c0 = fidx(fr.f, fr.H, xa + nx / 2, ya + ny / 2) *
+                             (quad ? fidx(fr.mask, fr.H, xa + nx / 2, ya + ny / 2) : 1.f);
+            float s_n = 0.f, s_1 = 0.f, s_2 = 0.f, s_abs = 0.f;
+            float sj[NJ], sjf[NJ];
+            constexpr int NH = MODE == 6 ? 21 : 3;
+            float hs[NH];
+#pragma unroll
+            for (int q = 0; q < NJ; ++q) sj[q] = sjf[q] = 0.f;
+#pragma unroll
+            for (int q = 0; q < NH; ++q) hs[q] = 0.f;
+#pragma unroll
+            for (int k = 0; k < PPT; ++k) {
+                const int p = pixel(k);
+                if (p < npix) {
+                    const int j = (int)(((float)p + 0.5f) * rnx), i = p - j * nx;
+                    const int x = xa + i, y = ya + j;
+                    const size_t gi = (size_t)((x - 4) + (y - 4) * Hd);
+                    fxr[k] = gld(fr.dfdx, gi);
+                    fyr[k] = gld(fr.dfdy, gi);
+                    float f = fidx(fr.f, fr.H, x, y);
+                    if (quad) f *= fidx(fr.mask, fr.H, x, y);
+                    const float fx = fxr[k], fy = fyr[k];
+                    const bool nz = !quad || fabsf(f) > 1e-10f;
+                    const float df = f - c0;
+                    s_n += nz ? 1.f : 0.f;
+                    s_1 += nz ? df : 0.f;
+                    s_2 += nz ? df * df : 0.f;
+                    s_abs += fabsf(f);
+                    if (MODE == 6) {
+                        const float dx = (float)i - hx, dy = (float)j - hy;
+                        const float J[6] = {fx * dx, fy * dx, fx * dy, fy * dy, fx, fy};
+                        int t = 0;
+#pragma unroll
+                        for (int c = 0; c < 6; ++c) {
+                            sj[c] += J[c];
+                            sjf[c] += J[c] * df;
+#pragma unroll
+                            for (int r = c; r < 6; ++r) hs[t++] += J[r] * J[c];
+                        }
+                    } else {
+                        sj[0] += fx;
+                        sj[1] += fy;
+                        sjf[0] += fx * df;
+                        sjf[1] += fy * df;
+                        hs[0] += fx * fx;
+                        hs[1] += fx * fy;
+                        hs[2] += fy * fy;
+                    }
+                }
+            }
+            double sd[4 + 2 * NJ];
+            sd[0] = s_n;
+            sd[1] = s_1;
+            sd[2] = s_2;
+            sd[3] = s_abs;
+#pragma unroll
+            for (int q = 0; q < NJ; ++q) {
+                sd[4 + q] = sj[q];
+                sd[4 + NJ + q] = sjf[q];
+            }
+            group_reduce<NW, NH, 4 + 2 * NJ, 0>(L, par, hs, sd, nullptr);
+            const double n = sd[0];
+            const double S1 = sd[1], S2 = sd[2];
+            sabs = sd[3];
+            meanf = n > 0 ? (double)c0 + S1 / n : NAN;
+            bottomf = n >= 2 ? sqrt(fmax(S2 - S1 * S1 / n, 0.0)) : (n == 1 ? 0.0 : NAN);
+            const double dm = (double)c0 - meanf;
+            if (tid == 0) {
+#pragma unroll
+                for (int q = 0; q < NJ; ++q) {
+                    L.SJ[q] = sd[4 + q];
+                    L.FJ[q] = sd[4 + NJ + q] + dm * sd[4 + q];  // sum J (f - meanf)
+                }
+            }
+            if (MODE == 6) {
+                double h[21];
+#pragma unroll
+                for (int q = 0; q < 21; ++q) h[q] = hs[q];
+                if (tid == 0)
+                    for (int q = 0; q < 21; ++q) a.H21[idx + (long)q * a.N] = h[q];
+                if (chol6(h)) {
+                    if (tid == 0) {
+                        for (int q = 0; q < 21; ++q) L.Lp[q] = h[q];
+                        for (int c = 0; c < 6; ++c) L.Ld[c] = 1.0 / h[c * 6 - (c * (c - 1)) / 2];
+                    }
+                } else {
+                    state = 3;  // LU of an indefinite H: the general kernel
+                }
+            } else {
+                set_h2(L, hs[0], hs[1], hs[2], bottomf, a.mu);
+            }
+            __syncthreads();
+        }
+
+        // ---------------------------------------------------------- iterate
+        int it = 0;
+        int tu0 = 0, tv0 = 0, tnx = 0, tny = 0;  // resident g tile (1-based origin, extents)
+        bool have_tile = false;
+        const float cf = isfinite(meanf) ? (float)meanf : 0.f;
+        double uo_u = 0.0, uo_v = 0.0, vd_u = 0.0, vd_v = 0.0;  // 2-DOF penalty targets
+        if (MODE == 2) {
+            uo_u = gld(a.UOld, (size_t)(2 * idx));
+            uo_v = gld(a.UOld, (size_t)(2 * idx + 1));
+            vd_u = gld(a.vdual, (size_t)(2 * idx));
+            vd_v = gld(a.vdual, (size_t)(2 * idx + 1));
+        }
+        // state is the same in every lane; saying so keeps the iteration a
+        // uniform loop (scalar branches), which the compiler cannot prove
+        // through the lane-0 writes of the set-up
+        state = __builtin_amdgcn_readfirstlane(state);
+        while (state == 0) {
+            if (!(it <= a.max_it && rel2 > tol2 && abs2 > tol2)) {
+                state = 1;
+                break;
+            }
+            ++it;
+            bool nanP = false;
+            for (int k = 0; k < 6; ++k) nanP |= isnan(P[k]);
+            if (nanP) {
+                if (MODE == 2 || quad) {
+                    state = 2;
+                    status = ST_ERROR;
+                } else {
+                    rel2 = it == 1 ? 0.0 : NAN;
+                    abs2 = NAN;
+                    state = 1;
+                }
+                break;
+            }
+            // bounds of the warped window from its corners (funICGNQuadtree.m:149-157)
+            double umin, umax, vmin, vmax;
+            bool out;
+            {
+                // the four corners, alike in every lane (no cross-lane step
+                // on the iteration's critical path)
+                double u[4], v[4];
+#pragma unroll
+                for (int q = 0; q < 4; ++q) {
+                    const double dx = (q & 1) ? hx : -hx, dy = (q & 2) ? hy : -hy;
+                    u[q] = (1 + P[0]) * dx + P[2] * dy + (x0 + P[4]);
+                    v[q] = P[1] * dx + (1 + P[3]) * dy + (y0 + P[5]);
+                }
+                umin = fmin(fmin(u[0], u[1]), fmin(u[2], u[3]));
+                umax = fmax(fmax(u[0], u[1]), fmax(u[2], u[3]));
+                vmin = fmin(fmin(v[0], v[1]), fmin(v[2], v[3]));
+                vmax = fmax(fmax(v[0], v[1]), fmax(v[2], v[3]));
+                // NaN corners compare false, as in the reference's test
+                bool o = false;
+#pragma unroll
+                for (int q = 0; q < 4; ++q) o |= (u[q] < 3) | (u[q] > fr.H - 2) | (v[q] < 3) | (v[q] > fr.W - 2);
+                out = o;
+            }
+            if (out) {
+                rel2 = 1e12;  // normOfWNew = 1e6 (:156)
+                state = 1;
+                break;
+            }
+            // support of this iteration's taps, one pixel of slack for the
+            // FP32 local coordinates; the tile keeps a margin of up to 4
+            const int nu0 = (int)floor(umin) - 2, nv0 = (int)floor(vmin) - 2;
+            const int nu1 = (int)floor(umax) + 3, nv1 = (int)floor(vmax) + 3;
+            bool use_tile = true;
+            const bool reload = !have_tile || nu0 < tu0 || nv0 < tv0 || nu1 >= tu0 + tnx ||
+                                nv1 >= tv0 + tny;
+            if (reload) {
+                int m = 4;
+                while (m > 0 && (nu1 - nu0 + 1 + 2 * m) * (nv1 - nv0 + 1 + 2 * m) > a.tile_cap) --m;
+                tu0 = nu0 - m;
+                tv0 = nv0 - m;
+                tnx = nu1 - nu0 + 1 + 2 * m;
+                tny = nv1 - nv0 + 1 + 2 * m;
+                use_tile = tnx * tny <= a.tile_cap;
+                have_tile = use_tile;
+                if (!use_tile) {  // gather from HBM; origin as for a tile
+                    tu0 = nu0;
+                    tv0 = nv0;
+                    tnx = nu1 - nu0 + 1;
+                    tny = nv1 - nv0 + 1;
+                }
+            }
+            const float a11 = (float)(1 + P[0]), a12 = (float)P[2], a21 = (float)P[1],
+                        a22 = (float)(1 + P[3]);
+            const float cu = (float)((x0 + P[4]) - tu0), cv = (float)((y0 + P[5]) - tv0);
+            if (use_tile && reload) {
+                __syncthreads();  // the last iteration's gathers are done with the old tile
+                const int total = tnx * tny;
+                const float rcpn = 1.0f / tnx;
+                for (int t0 = 0; t0 < total; t0 += 64 * NW * 8) {
+                    float v[8];
+                    int o[8];
+#pragma unroll
+                    for (int q = 0; q < 8; ++q) {
+                        const int t = t0 + tid + 64 * NW * q;
+                        o[q] = t;
+                        v[q] = 0.f;
+                        if (t < total) {
+                            const int sj = (int)(((float)t + 0.5f) * rcpn), ri = t - sj * tnx;
+                            const int y = min(max(tv0 + sj, 1), fr.W), x = min(max(tu0 + ri, 1), fr.H);
+                            v[q] = gld(fr.g, (size_t)((x - 1) + (y - 1) * fr.H));
+                        }
+                    }
+#pragma unroll
+                    for (int q = 0; q < 8; ++q)
+                        if (o[q] < total) tile[o[q]] = v[q];
+                }
+                __syncthreads();
+            }
+            // Keys bicubic of the window pixel at offset (dx, dy), from the
+            // resident tile or, if the warped support outgrew it, from HBM
+            auto gather = [&](auto from_tile, float dx, float dy) {
+                const float lu = fmaf(a11, dx, fmaf(a12, dy, cu));
+                const float lv = fmaf(a21, dx, fmaf(a22, dy, cv));
+                const float fu = floorf(lu), fvv = floorf(lv);
+                float wu[4], wv[4];
+                keys_w(lu - fu, wu);
+                keys_w(lv - fvv, wv);
+                const int iu = (int)fu - 1, iv = (int)fvv - 1;
+                float g = 0.f;
+                if constexpr (decltype(from_tile)::value) {
+                    const float *b = tile + iu + iv * tnx;
+#pragma unroll
+                    for (int q = 0; q < 4; ++q) {
+                        const float *c4 = b + q * tnx;
+                        g = fmaf(wv[q], fmaf(wu[0], c4[0], fmaf(wu[1], c4[1], fmaf(wu[2], c4[2], wu[3] * c4[3]))), g);
+                    }
+                } else {
+#pragma unroll
+                    for (int q = 0; q < 4; ++q) {
+                        const int y = min(max(tv0 + iv + q, 1), fr.W);
+                        const float *col = fr.g + (size_t)(y - 1) * fr.H;
+                        float s = 0.f;
+#pragma unroll
+                        for (int r = 0; r < 4; ++r) s = fmaf(wu[r], gld(col, (size_t)(min(max(tu0 + iu + r, 1), fr.H) - 1)), s);
+                        g = fmaf(wv[q], s, g);
+                    }
+                }
+                return g;
+            };
+            // S1 = sum(g - c), S2 = sum((g - c)^2) over the nonzero g,
+            // T_q = sum J_q (g - c); every lane runs every slot (no branch
+            // splits the unrolled slots, so their gathers overlap)
+            float acc[2 + NJ];
+#pragma unroll
+            for (int q = 0; q < 2 + NJ; ++q) acc[q] = 0.f;
+            int n_nz = 0;
+            bool any_zero = false, any_nan = false;
+            auto fused_pass = [&](auto from_tile) {
+#pragma unroll
+                for (int k = 0; k < PPT; ++k) {
+                    const bool valid = (vbits >> k) & 1u;
+                    const float g = gather(from_tile, dxr[k], dyr[k]);
+                    any_zero |= valid && g == 0.f;
+                    any_nan |= valid && isnan(g);
+                    const float ge = valid ? g : 0.f;
+                    const float d = ge - cf;
+                    const bool nz = valid && (!quad || fabsf(ge) > 1e-10f);
+                    acc[0] += nz ? d : 0.f;
+                    acc[1] += nz ? d * d : 0.f;
+                    const float u = valid ? fxr[k] * d : 0.f, v = valid ? fyr[k] * d : 0.f;
+                    if (MODE == 6) {
+                        acc[2] = fmaf(dxr[k], u, acc[2]);
+                        acc[3] = fmaf(dxr[k], v, acc[3]);
+                        acc[4] = fmaf(dyr[k], u, acc[4]);
+                        acc[5] = fmaf(dyr[k], v, acc[5]);
+                        acc[6] += u;
+                        acc[7] += v;
+                    } else {
+                        acc[2] += u;
+                        acc[3] += v;
+                    }
+                    n_nz += (int)__popcll(__ballot(nz));
+                }
+            };
+            if (use_tile)
+                fused_pass(std::true_type{});
+            else
+                fused_pass(std::false_type{});
+            int xv[4] = {n_nz, wave_any(any_zero) ? 1 : 0, wave_any(any_nan) ? 1 : 0, 0};
+            if constexpr (NW == 1)
+                group_reduce<NW, 0, 0, 4>(L, par, nullptr, nullptr, xv);
+            else
+                group_reduce<NW, 2 + NJ, 0, 4>(L, par, acc, nullptr, xv);
+            if (quad && xv[2] > 0) {
+                state = 2;
+                status = ST_ERROR;  // logical(NaN) throws
+                break;
+            }
+            if (quad && xv[1] > 0) {
+                state = 3;  // a zero of g: bwselect regions, the general kernel
+                break;
+            }
+            // gbar - c = S1/n, bottomg^2 = S2 - S1^2/n; b = FJ - (bf/bg) (T - (gbar-c) SJ)
+            if constexpr (NW == 1) group_reduce<NW, 2 + NJ, 0, 0>(L, par, acc, nullptr, nullptr);
+            const double n = xv[0];
+            const double S1 = acc[0], S2 = acc[1];
+            const double inv_n = __builtin_amdgcn_rcp(n);
+            const double dg = S1 * inv_n;
+            const double var = S2 - S1 * dg;
+            const double bf = bottomf;
+            if (!(n >= 2) || !(bf > 0) || !(var > 0) || !isfinite(bf) || !isfinite(var) ||
+                !isfinite(dg)) {
+                state = 3;  // degenerate statistics: the reference's FP64 residual
+                break;
+            }
+            double b[6] = {0, 0, 0, 0, 0, 0};
+            {
+                const double ratio = bf * __builtin_amdgcn_rsq(var);
+#pragma unroll
+                for (int q = 0; q < NJ; ++q) b[q] = L.FJ[q] - ratio * ((double)acc[2 + q] - dg * L.SJ[q]);
+            }
+            // norm test, Gauss-Newton step, inverse-compositional update
+            double nb2 = 0.0;
+            if (MODE == 6) {
+#pragma unroll
+                for (int q = 0; q < 6; ++q) nb2 += b[q] * b[q];
+            } else {
+                // :379 tempb = 2 b / bottomf^2 + mu (P - UOld - vdual)
+                const double s2 = 2.0 * __builtin_amdgcn_rcp(bottomf * bottomf);
+                b[0] = b[0] * s2 + a.mu * ((P[4] - uo_u) - vd_u);
+                b[1] = b[1] * s2 + a.mu * ((P[5] - uo_v) - vd_v);
+                nb2 = b[0] * b[0] + b[1] * b[1];
+            }
+            abs2 = nb2;
+            if (it == 1) {
+                init_ok = nb2 > tol2;  // normOfWNewInit > tol
+                inv_init2 = __builtin_amdgcn_rcp(nb2);
+            }
+            rel2 = init_ok ? nb2 * inv_init2 : 0.0;
+            if (rel2 < tol2 || abs2 < tol2) {
+                state = 1;
+                break;
+            }
+            double d[6] = {0, 0, 0, 0, 0, 0};
+            if (MODE == 6) {
+                solve_chol6(L.Lp, L.Ld, b, d);
+            } else {
+                d[4] = -(L.H2i[0] * b[0] + L.H2i[2] * b[1]);
+                d[5] = -(L.H2i[1] * b[0] + L.H2i[3] * b[1]);
+            }
+            if (!ic_compose(P, d)) {
+                state = 1;
+                break;
+            }
+        }
+
+        // ---------------------------------------------------------- outputs
+        if (state == 1) {
+            int st = it;
+            if (!(rel2 < tol2 || abs2 < tol2)) st = a.max_it + 1;
+            if (isnan(rel2)) st = a.max_it + 1;
+            if (quad && sabs < 1e-6) st = a.max_it + 3;
+            status = st;
+        }
+        if (tid == 0) {
+            if (state == 3) {
+                a.status[idx] = ST_DEFER;
+                if (a.defer_out) atomicOr(a.defer_out, 1);
+                if (a.handed) a.handed[idx] = 1;  // the schedule moves it to the general kernel
+            } else {
+                double Pout[6];
+                for (int k = 0; k < 6; ++k) Pout[k] = P[k];
+                if (status == ST_ERROR)
+                    for (int k = 0; k < 6; ++k) Pout[k] = NAN;
+                if (status == ST_MASKED && MODE == 6) {
+                    Pout[0] = Pout[1] = Pout[2] = Pout[3] = 0.0;
+                    Pout[4] = gld(a.U0, (size_t)(2 * idx));
+                    Pout[5] = gld(a.U0, (size_t)(2 * idx + 1));
+                }
+                a.U[2 * idx] = Pout[4];
+                a.U[2 * idx + 1] = Pout[5];
+                if (MODE == 6) {
+                    for (int k = 0; k < 4; ++k) a.F[4 * idx + k] = Pout[k];
+                    if (status == ST_ERROR || status == ST_MASKED)
+                        for (int k = 0; k < 21; ++k) a.H21[idx + (long)k * a.N] = 0.0;
+                }
+                a.status[idx] = status;
+            }
+        }
+        __syncthreads();
+        if (!a.work_counter) break;
+    }
+}
+
+template <int PPT, int MODE, int NW>
+hipError_t launch_lean(const IcgnArgs &a, int grid, hipStream_t s)
+{
+    const size_t lds = sizeof(float) * (size_t)a.tile_cap;
+    hipLaunchKernelGGL((icgn_lean_kernel<PPT, MODE, NW>), dim3(grid), dim3(64 * NW), lds, s, a);
+    return hipGetLastError();
+}
+
+// the pixel-slot counts of icgn_wave.hip, per number of wavefronts
+template <int MODE, int NW>
+hipError_t launch_lean_ppt(const IcgnArgs &a, int ppt, int grid, hipStream_t s)
+{
+    if constexpr (NW == 1) {
+        switch (ppt) {
+        case 5: return launch_lean<5, MODE, 1>(a, grid, s);
+        case 6: return launch_lean<6, MODE, 1>(a, grid, s);
+        case 7: return launch_lean<7, MODE, 1>(a, grid, s);
+        case 9: return launch_lean<9, MODE, 1>(a, grid, s);
+        case 18: return launch_lean<18, MODE, 1>(a, grid, s);
+        }
+    } else if constexpr (NW == 2) {
+        switch (ppt) {
+        case 3: return launch_lean<3, MODE, 2>(a, grid, s);
+        case 4: return launch_lean<4, MODE, 2>(a, grid, s);
+        case 5: return launch_lean<5, MODE, 2>(a, grid, s);
+        case 7: return launch_lean<7, MODE, 2>(a, grid, s);
+        case 9: return launch_lean<9, MODE, 2>(a, grid, s);
+        }
+    } else {
+        switch (ppt) {
+        case 2: return launch_lean<2, MODE, 4>(a, grid, s);
+        case 3: return launch_lean<3, MODE, 4>(a, grid, s);
+        case 5: return launch_lean<5, MODE, 4>(a, grid, s);
+        }
+    }
+    return hipErrorInvalidValue;
+}
+
+}  // namespace
+
+hipError_t icgn_lean_launch(const IcgnArgs &a, int mode, int ppt, int nw, int grid, hipStream_t s)
+{
+    if (mode == 6) {
+        if (nw == 1) return launch_lean_ppt<6, 1>(a, ppt, grid, s);
+        if (nw == 2) return launch_lean_ppt<6, 2>(a, ppt, grid, s);
+        if (nw == 4) return launch_lean_ppt<6, 4>(a, ppt, grid, s);
+    } else {
+        if (nw == 1) return launch_lean_ppt<2, 1>(a, ppt, grid, s);
+        if (nw == 2) return launch_lean_ppt<2, 2>(a, ppt, grid, s);
+        if (nw == 4) return launch_lean_ppt<2, 4>(a, ppt, grid, s);
+    }
+    return hipErrorInvalidValue;
+}
+
+}  // namespace staq

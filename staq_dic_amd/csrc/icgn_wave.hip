@@ -18,21 +18,11 @@
 //
 // Windows that do not fit 64 * PPT pixels after the mask-driven growth are
 // deferred (status -2) to the workgroup kernel of icgn.hip.
-#include "kernels.h"
+#include "wave_common.h"
 
 namespace staq {
 
 namespace {
-
-template <typename T>
-__device__ __forceinline__ T wave_allsum(T v)
-{
-#pragma unroll
-    for (int o = 1; o < kWave; o <<= 1) v += __shfl_xor(v, o, kWave);
-    return v;
-}
-
-__device__ __forceinline__ bool wave_any(bool p) { return __ballot(p) != 0ull; }
 
 // Bits of bw connected to the seeds s (s a subset of bw) along the word: a
 // carry entering a run of ones at a seed runs to the run's top, so
@@ -81,32 +71,6 @@ __device__ unsigned long long wave_flood(unsigned long long bw, int ny, int seed
     return sel;
 }
 
-// In-register Cholesky of a 6x6 SPD matrix given by its packed lower
-// triangle (column-major: (0,0),(1,0)..(5,0),(1,1)..); false if not SPD.
-__device__ __forceinline__ bool chol6(double (&h)[21])
-{
-#define PK(r, c) h[(c) * 6 - ((c) * ((c) - 1)) / 2 + ((r) - (c))]
-#pragma unroll
-    for (int j = 0; j < 6; ++j) {
-        double d = PK(j, j);
-#pragma unroll
-        for (int k = 0; k < j; ++k) d -= PK(j, k) * PK(j, k);
-        if (!(d > 0)) return false;
-        const double l = sqrt(d);
-        PK(j, j) = l;
-        const double il = 1.0 / l;
-#pragma unroll
-        for (int i = j + 1; i < 6; ++i) {
-            double s = PK(i, j);
-#pragma unroll
-            for (int k = 0; k < j; ++k) s -= PK(i, k) * PK(j, k);
-            PK(i, j) = s * il;
-        }
-    }
-#undef PK
-    return true;
-}
-
 struct WaveLds;
 
 struct WaveLds {
@@ -127,75 +91,12 @@ struct WaveLds {
     unsigned long long bal[64];   // per pixel slot ballots of nonzero g
 };
 
-// Subset-wide sums over the NW wavefronts of a workgroup: f and d hold each
-// lane's partials (reduced within the wavefront by DPP), iv each wavefront's
-// own totals.  Every wavefront adds the same partials in the same order, so
-// all of them hold bit-identical totals afterwards; the buffers alternate, so
-// one barrier per exchange suffices.
-template <int NW, int NF, int ND, int NI>
-__device__ __forceinline__ void group_reduce(WaveLds &L, int &par, float *f, double *d, int *iv)
-{
-#pragma unroll
-    for (int q = 0; q < NF; ++q) f[q] = wave_sum_dpp(f[q]);
-#pragma unroll
-    for (int q = 0; q < ND; ++q) d[q] = wave_sum_dpp_f64(d[q]);
-    if constexpr (NW > 1) {
-        const int w = threadIdx.x >> 6;
-        if ((threadIdx.x & 63) == 0) {
-#pragma unroll
-            for (int q = 0; q < NF; ++q) L.xf[par][w][q] = f[q];
-#pragma unroll
-            for (int q = 0; q < ND; ++q) L.xd[par][w][q] = d[q];
-#pragma unroll
-            for (int q = 0; q < NI; ++q) L.xi[par][w][q] = iv[q];
-        }
-        __syncthreads();
-#pragma unroll
-        for (int q = 0; q < NF; ++q) {
-            float t = L.xf[par][0][q];
-#pragma unroll
-            for (int ww = 1; ww < NW; ++ww) t += L.xf[par][ww][q];
-            f[q] = t;
-        }
-#pragma unroll
-        for (int q = 0; q < ND; ++q) {
-            double t = L.xd[par][0][q];
-#pragma unroll
-            for (int ww = 1; ww < NW; ++ww) t += L.xd[par][ww][q];
-            d[q] = t;
-        }
-#pragma unroll
-        for (int q = 0; q < NI; ++q) {
-            int t = L.xi[par][0][q];
-#pragma unroll
-            for (int ww = 1; ww < NW; ++ww) t += L.xi[par][ww][q];
-            iv[q] = t;
-        }
-        par ^= 1;
-    }
-}
-
 // Solve H d = -b with the factor in LDS (all lanes read it with broadcast loads).
 __device__ __forceinline__ void solve6_lds(const WaveLds &L, const double (&b)[6], double (&d)[6])
 {
     double y[6];
     if (L.kind == 0) {
-#define LP(r, c) L.Lp[(c) * 6 - ((c) * ((c) - 1)) / 2 + ((r) - (c))]
-#pragma unroll
-        for (int i = 0; i < 6; ++i) {
-            double s = -b[i];
-#pragma unroll
-            for (int k = 0; k < i; ++k) s -= LP(i, k) * y[k];
-            y[i] = s * L.Ld[i];
-        }
-#pragma unroll
-        for (int i = 5; i >= 0; --i) {
-            double s = y[i];
-#pragma unroll
-            for (int k = i + 1; k < 6; ++k) s -= LP(k, i) * d[k];
-            d[i] = s * L.Ld[i];
-        }
-#undef LP
+        solve_chol6(L.Lp, L.Ld, b, d);
     } else {
 #pragma unroll
         for (int i = 0; i < 6; ++i) {
@@ -207,25 +108,6 @@ __device__ __forceinline__ void solve6_lds(const WaveLds &L, const double (&b)[6
             double s = y[i];
             for (int k = i + 1; k < 6; ++k) s -= L.A[i + k * 6] * d[k];
             d[i] = s / L.A[i + i * 6];
-        }
-    }
-}
-
-// 2x2 damped system of funICGN_Subpb1.m:111,391-394, inverted by lane 0.
-__device__ __forceinline__ void set_h2(WaveLds &L, double h11, double h12, double h22, double bf,
-                                       double mu)
-{
-    const double s2 = 2.0 / (bf * bf);
-    const double H2a = h11 * s2 + mu, H2b = h12 * s2, H2d = h22 * s2 + mu;
-    const double dl = 1e-3 * (H2a > H2d ? H2a : H2d);
-    if (threadIdx.x == 0) {
-        const double A[4] = {H2a + dl, H2b, H2b, H2d + dl};
-        for (int c = 0; c < 2; ++c) {
-            double e[2] = {0, 0}, xs[2];
-            e[c] = 1.0;
-            solve_small<2>(A, e, xs);
-            L.H2i[0 + c * 2] = xs[0];
-            L.H2i[1 + c * 2] = xs[1];
         }
     }
 }
@@ -255,28 +137,28 @@ __global__ __launch_bounds__(64 * NW) void icgn_wave_kernel(IcgnArgs a)
         __syncthreads();
         const long item = item_sh;
         if (item >= a.item0 + a.n_items) break;
-        const long idx = a.order ? a.order[item] : item;
-        const int slot = a.slot ? a.slot[idx] : 0;
-        const FrameDev fr = a.frames[slot];
+        const long idx = a.order ? gld(a.order, (size_t)(item)) : item;
+        const int slot = a.slot ? gld(a.slot, (size_t)(idx)) : 0;
+        const FrameDev fr = load_frame(a.frames, slot);
         const int Hd = fr.H - 6;
 
-        double x0 = a.coords[idx], y0 = a.coords[idx + a.N];
+        double x0 = gld(a.coords, (size_t)(idx)), y0 = gld(a.coords, (size_t)(idx + a.N));
         if (!a.plain) {
             x0 = round(x0);  // LocalICGNQuadtree.m:69
             y0 = round(y0);
         }
-        int wx = a.wx[idx];
-        int wy = MODE == 2 ? a.wy[idx] : wx;
+        int wx = gld(a.wx, (size_t)(idx));
+        int wy = MODE == 2 ? gld(a.wy, (size_t)(idx)) : wx;
         const int w0 = wx;
         double P[6];
         if (MODE == 6) {
             P[0] = P[1] = P[2] = P[3] = 0.0;
-            P[4] = a.U0[2 * idx];
-            P[5] = a.U0[2 * idx + 1];
+            P[4] = gld(a.U0, (size_t)(2 * idx));
+            P[5] = gld(a.U0, (size_t)(2 * idx + 1));
         } else {
-            for (int k = 0; k < 4; ++k) P[k] = a.FOld[4 * idx + k];
-            P[4] = a.UOld[2 * idx];
-            P[5] = a.UOld[2 * idx + 1];
+            for (int k = 0; k < 4; ++k) P[k] = gld(a.FOld, (size_t)(4 * idx + k));
+            P[4] = gld(a.UOld, (size_t)(2 * idx));
+            P[5] = gld(a.UOld, (size_t)(2 * idx + 1));
         }
         int state = 0, status = 0;  // 0 run, 1 loop done, 2 final status, 3 deferred
         auto win_fits = [&](int wxx, int wyy) {
@@ -535,11 +417,15 @@ __global__ __launch_bounds__(64 * NW) void icgn_wave_kernel(IcgnArgs a)
         const float cf = isfinite(meanf) ? (float)meanf : 0.f;
         double uo_u = 0.0, uo_v = 0.0, vd_u = 0.0, vd_v = 0.0;  // 2-DOF penalty targets
         if (MODE == 2) {
-            uo_u = a.UOld[2 * idx];
-            uo_v = a.UOld[2 * idx + 1];
-            vd_u = a.vdual[2 * idx];
-            vd_v = a.vdual[2 * idx + 1];
+            uo_u = gld(a.UOld, (size_t)(2 * idx));
+            uo_v = gld(a.UOld, (size_t)(2 * idx + 1));
+            vd_u = gld(a.vdual, (size_t)(2 * idx));
+            vd_v = gld(a.vdual, (size_t)(2 * idx + 1));
         }
+        // state is the same in every lane; saying so keeps the iteration a
+        // uniform loop (scalar branches), which the compiler cannot prove
+        // through the lane-0 writes of the set-up
+        state = __builtin_amdgcn_readfirstlane(state);
         while (state == 0) {
             if (!(it <= a.max_it && rel2 > tol2 && abs2 > tol2)) {
                 state = 1;
@@ -590,28 +476,27 @@ __global__ __launch_bounds__(64 * NW) void icgn_wave_kernel(IcgnArgs a)
                 }
                 break;
             }
-            // the warped window's extremes are at its corners (funICGNQuadtree.m:149-157):
-            // lane q of every quad takes corner q, quad permutations give the extremes
+            // the warped window's extremes are at its corners (funICGNQuadtree.m:149-157),
+            // computed alike in every lane (no cross-lane step on the
+            // iteration's critical path)
             double umin, umax, vmin, vmax;
             bool out;
             {
-                const int q = lane & 3;
-                const double dx = (q & 1) ? hx : -hx, dy = (q & 2) ? hy : -hy;
-                const double u = (1 + P[0]) * dx + P[2] * dy + (x0 + P[4]);
-                const double v = P[1] * dx + (1 + P[3]) * dy + (y0 + P[5]);
-                out = wave_any(u < 3 || u > fr.H - 2 || v < 3 || v > fr.W - 2);
-                umin = fmin(u, dpp_f64<0xB1>(u));  // quad_perm [1,0,3,2]
-                umax = fmax(u, dpp_f64<0xB1>(u));
-                vmin = fmin(v, dpp_f64<0xB1>(v));
-                vmax = fmax(v, dpp_f64<0xB1>(v));
-                umin = fmin(umin, dpp_f64<0x4E>(umin));  // quad_perm [2,3,0,1]
-                umax = fmax(umax, dpp_f64<0x4E>(umax));
-                vmin = fmin(vmin, dpp_f64<0x4E>(vmin));
-                vmax = fmax(vmax, dpp_f64<0x4E>(vmax));
-                umin = readlane_f64(umin, 0);  // uniform from here on
-                umax = readlane_f64(umax, 0);
-                vmin = readlane_f64(vmin, 0);
-                vmax = readlane_f64(vmax, 0);
+                double u[4], v[4];
+#pragma unroll
+                for (int q = 0; q < 4; ++q) {
+                    const double dx = (q & 1) ? hx : -hx, dy = (q & 2) ? hy : -hy;
+                    u[q] = (1 + P[0]) * dx + P[2] * dy + (x0 + P[4]);
+                    v[q] = P[1] * dx + (1 + P[3]) * dy + (y0 + P[5]);
+                }
+                umin = fmin(fmin(u[0], u[1]), fmin(u[2], u[3]));
+                umax = fmax(fmax(u[0], u[1]), fmax(u[2], u[3]));
+                vmin = fmin(fmin(v[0], v[1]), fmin(v[2], v[3]));
+                vmax = fmax(fmax(v[0], v[1]), fmax(v[2], v[3]));
+                bool o = false;
+#pragma unroll
+                for (int q = 0; q < 4; ++q) o |= (u[q] < 3) | (u[q] > fr.H - 2) | (v[q] < 3) | (v[q] > fr.W - 2);
+                out = o;
             }
             if (out) {
                 rel2 = 1e12;
@@ -1195,8 +1080,8 @@ __global__ __launch_bounds__(64 * NW) void icgn_wave_kernel(IcgnArgs a)
                     for (int k = 0; k < 6; ++k) Pout[k] = NAN;
                 if (status == ST_MASKED && MODE == 6) {
                     Pout[0] = Pout[1] = Pout[2] = Pout[3] = 0.0;
-                    Pout[4] = a.U0[2 * idx];
-                    Pout[5] = a.U0[2 * idx + 1];
+                    Pout[4] = gld(a.U0, (size_t)(2 * idx));
+                    Pout[5] = gld(a.U0, (size_t)(2 * idx + 1));
                 }
                 a.U[2 * idx] = Pout[4];
                 a.U[2 * idx + 1] = Pout[5];

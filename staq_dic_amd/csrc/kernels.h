@@ -29,6 +29,7 @@ struct IcgnArgs {
     int npix_cap, tile_cap; // dynamic LDS sizing (pixels, floats)
     int defer;              // a window larger than npix_cap is left for a fallback launch
     int only_deferred;      // fallback launch: process only subsets left by the main launch
+    int *handed;            // lean kernel: set to 1 for a subset it leaves to the fallback (may be null)
 };
 
 struct ZnssdArgs {
@@ -61,6 +62,9 @@ hipError_t icgn_launch(const IcgnArgs &a, int mode, int bs, int grid, hipStream_
 // (0: too large for the wavefront kernel)
 int icgn_wave_ppt(int npix_cap, int nw);
 hipError_t icgn_wave_launch(const IcgnArgs &a, int mode, int ppt, int nw, int grid, hipStream_t s);
+// lean wavefront kernel for windows without masked pixels (icgn_lean.hip);
+// the pixel-slot counts are those of icgn_wave_ppt
+hipError_t icgn_lean_launch(const IcgnArgs &a, int mode, int ppt, int nw, int grid, hipStream_t s);
 size_t icgn_lds_bytes(int npix_cap, int tile_cap);
 hipError_t ba_interp2_launch(const void *F, long M, long N, long P, const void *X, const void *Y,
                              long nd, void *out, bool single, hipStream_t s);

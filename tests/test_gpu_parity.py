@@ -342,17 +342,19 @@ def test_batch_forms_agree(ctx, plain_cfg):
     b.close()
 
 
-@pytest.mark.parametrize("form", [-2, -4])
+@pytest.mark.parametrize("form", [-2, -4, 1, 4])
 def test_multi_wave_forms_on_masked_subsets(ctx, oracle, form):
     """Two / four wavefronts per subset through the masked path (region
-    floods, window growth, -1 and 102) and the 2-DOF step, against the oracle."""
+    floods, window growth, -1 and 102) and the 2-DOF step, against the oracle;
+    forms 1 and 4 force the lean kernel on every bin, so every subset that
+    touches the mask goes through its hand-over to the fallback launch."""
     from staq_dic_amd import Batch, params, synth
     from staq_dic_amd._lib import IN_FOLD, IN_U0, IN_UOLD
     fr, coords, win = synth.config_hole(H=384, W=384, n_speckles=1700, max_disp=3.0, radius=60,
                                         margin=24, band=(96, 40))
     truth = synth.truth_at(fr.field, coords)
     U0 = np.round(truth[:, 4:6]).reshape(-1)
-    _print("hole 384^2 with", -form, "wavefronts per subset, N =", coords.shape[0])
+    _print("hole 384^2 with kernel form", form, "N =", coords.shape[0])
     ctx.set_frame(fr.f, fr.g, fr.mask)
     b = Batch(ctx, coords, win)
     b.set_input(IN_U0, U0)
@@ -535,3 +537,99 @@ def test_reference_level_integer_search(ctx, oracle, ref_frames):
     np.testing.assert_allclose(u[fin], uo[fin], atol=1e-3 + 1e-9)
     np.testing.assert_allclose(v[fin], vo[fin], atol=1e-3 + 1e-9)
     assert np.isnan(u[hole]).all() and fin.sum() > 100
+
+
+def test_lean_kernel_matches_general(ctx, oracle):
+    """The lean kernel (windows without masked pixels, the default form of
+    such bins) against the general wavefront kernel on the same bins with the
+    same number of wavefronts: the same operations in the same order, so
+    bit-identical results wherever the lean kernel kept the subset; subsets it
+    handed to the fallback launch (a zero of g under the quadtree rules)
+    differ at most by the fallback's summation order.  6-DOF and 2-DOF."""
+    from staq_dic_amd import Batch, params, synth
+    from staq_dic_amd._lib import IN_FOLD, IN_U0, IN_UOLD
+    fr, coords, win = synth.config_hole(H=384, W=384, n_speckles=1700, max_disp=3.0, radius=60,
+                                        margin=24, band=(96, 40))
+    truth = synth.truth_at(fr.field, coords)
+    U0 = np.round(truth[:, 4:6]).reshape(-1)
+    ctx.set_frame(fr.f, fr.g, fr.mask)
+    b = Batch(ctx, coords, win)
+    clear = [x["bs6"] in (1, 2, 4) for x in b.bins()]
+    assert any(clear) and not all(clear), b.bins()
+    b.set_input(IN_U0, U0)
+    p6 = params(tol=1e-3)
+    p2 = params(tol=1e-3, mu=1e-3)
+
+    def run(mode, forms):
+        b.set_forms(forms, mode)
+        (b.icgn6 if mode == 6 else b.icgn2)(p6 if mode == 6 else p2)
+        return b.outputs()
+
+    def compare(out_l, out_g, with_f):
+        Ul, Fl, _, sl = out_l
+        Ug, Fg, _, sg = out_g
+        same = (sl == sg) & (np.all(Ul.reshape(-1, 2) == Ug.reshape(-1, 2), axis=1) |
+                             np.isnan(Ul.reshape(-1, 2)).all(axis=1))
+        if with_f:
+            same &= np.all(Fl.reshape(-1, 4) == Fg.reshape(-1, 4), axis=1) | \
+                np.isnan(Fl.reshape(-1, 4)).all(axis=1)
+        assert np.mean(same) > 0.97, np.mean(same)
+        _cmp_icgn(sl, sg, Ul, Ug, Fl if with_f else None, Fg if with_f else None,
+                  max_status_mismatch=0.01)
+
+    for nw in (1, 2, 4):
+        general = 0 if nw == 1 else -nw
+        _print("6-DOF: lean kernel with", nw, "wavefronts against the general kernel")
+        out_l = run(6, [nw if c else general for c in clear])
+        out_g = run(6, [general] * len(clear))
+        compare(out_l, out_g, True)
+    UOld = np.where(np.isfinite(out_g[0]), out_g[0], U0)
+    FOld = np.where(np.isfinite(out_g[1]), out_g[1], 0.0)
+    b.set_input(IN_UOLD, UOld)
+    b.set_input(IN_FOLD, FOld)
+    for nw in (1, 2):
+        general = 0 if nw == 1 else -nw
+        _print("2-DOF: lean kernel with", nw, "wavefronts against the general kernel")
+        out_l = run(2, [nw if c else general for c in clear])
+        out_g = run(2, [general] * len(clear))
+        compare(out_l, out_g, False)
+    b.close()
+
+
+def test_handed_back_subsets_move_to_the_general_kernel(ctx, plain_cfg):
+    """Subsets the lean kernel hands to the fallback launch (here: windows
+    whose warped support meets a patch of exact zeros planted in g, which the
+    quadtree rules treat as masked) move to the general kernel's bins at the
+    next reorder; the results stay the same up to the fallback's summation
+    order, and nothing is handed back afterwards."""
+    from staq_dic_amd import Batch, params, synth
+    from staq_dic_amd._lib import IN_U0
+    fr, coords, win = plain_cfg
+    truth = synth.truth_at(fr.field, coords)
+    U0 = np.round(truth[:, 4:6]).reshape(-1)
+    g = fr.g.copy()
+    g[240:246, 240:246] = 0.0
+    g[100:104, 380:384] = 0.0
+    ctx.set_frame(fr.f, g, fr.mask)
+    b = Batch(ctx, coords, win)
+    b.set_input(IN_U0, U0)
+    p = params(tol=1e-3)
+
+    def general_count():
+        return sum(x["count"] for x in b.bins() if x["bs6"] not in (1, 2, 4))
+
+    n0 = general_count()
+    b.icgn6(p)
+    first = b.outputs()
+    b.reorder()
+    n1 = general_count()
+    _print("subsets in general-kernel bins:", n0, "->", n1)
+    assert n1 > n0
+    b.icgn6(p)
+    second = b.outputs()
+    _cmp_icgn(first[3], second[3], first[0], second[0], first[1], second[1],
+              max_status_mismatch=0.01)
+    assert np.mean(first[3] == second[3]) > 0.99
+    b.reorder()
+    assert general_count() == n1
+    b.close()

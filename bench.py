@@ -3,15 +3,22 @@
 Workload (the quadtree-with-hole configuration): a 1024 x 1024 8-bit speckle
 pair (12 000 Gaussian speckles, seed 1002) with a centred circular hole of
 radius 150 px, deformed by the Kirsch plate-with-hole field scaled to 8 px;
-a quadtree subset list refined at the hole edge (leaves 8-32 px, windows
-16-64 px, ~6 000 subsets).  A step is one 6-DOF IC-GN local step
-(LocalICGNQuadtree) over every subset: one hipGraph holding the per-bin
-launches, inputs already resident in HBM, tol 1e-3, at most 100 iterations.
+a quadtree subset list refined at the hole edge (leaves 8-32 px, 6016 nodes).
+Every node gets the one window the reference's quadtree drivers give it
+(DICpara.winsize = 32), grown by the IC-GN step where it meets the hole.  A
+step is one 6-DOF IC-GN local step (LocalICGNQuadtree) over every subset: one
+hipGraph holding the per-bin launches, inputs already resident in HBM, tol
+1e-3, at most 100 iterations.
 
-The JSON line also carries the 2-DOF augmented-Lagrangian step (Subpb1Quadtree),
-the one-launch-per-subset form's summed device time (its overhead-free floor),
-the integer search, the accuracy against the analytic field, the roofline of the
-IC-GN step and the CPU baseline (the FP64 oracle on the host cores).
+The JSON line also carries: the same nodes with windows tied to their leaf
+size (16-64 px); the 2-DOF augmented-Lagrangian step (Subpb1Quadtree); the
+one-launch-per-subset form's summed device time (its overhead-free floor);
+the integer search; the adjoint ZNSSD forward+backward on the 2048^2
+sinusoidal pair; the multi-instance batch (16 pairs of 512^2-2048^2 in one
+graph, per-pair latency percentiles); the large-deformation incremental pair
+(4096^2 16-bit, ~100 k subsets, integer search w 64 r 48); the accuracy
+against the analytic fields; the roofline of the step's dominant launch and
+the CPU baseline (the FP64 oracle on the host cores).
 """
 from __future__ import annotations
 
@@ -58,6 +65,212 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
+def kernel_name(bin_desc, mode=6):
+    f = bin_desc["bs6" if mode == 6 else "bs2"]
+    if f in (1, 2, 4, 8, 16):
+        return f"icgn_lean_kernel<{bin_desc['ppt']},{mode},{f}>"
+    if f <= 0:
+        return f"icgn_wave_kernel<{bin_desc['ppt']},{mode},{1 if f == 0 else -f}>"
+    return f"icgn_kernel<{f},{mode}>"
+
+
+def accuracy(U, st, truth):
+    """RMS (px) of (u, v) against the analytic field over converged subsets
+    (status 1-100), and over the subsets the reference keeps after its
+    bad-subset rule (LocalICGNQuadtree.m:151-169: status > max(mean + std, 6)
+    is refilled)."""
+    from staq_dic_amd.api import classify_local
+    conv = (st >= 1) & (st <= 100)
+    err = np.hypot(U[0::2] - truth[:, 4], U[1::2] - truth[:, 5])
+    bad, _ = classify_local(st)
+    kept = conv & ~bad
+    rms = float(np.sqrt(np.mean(err[conv] ** 2))) if conv.any() else float("nan")
+    rms_kept = float(np.sqrt(np.mean(err[kept] ** 2))) if kept.any() else float("nan")
+    return conv, err, rms, rms_kept, kept
+
+
+class Timer:
+    """HIP events on the context's stream (the stream every launch of a step
+    is enqueued on, or forks from and joins)."""
+
+    def __init__(self, stream):
+        import torch
+        self.torch = torch
+        self.stream = stream
+        self.e0 = torch.cuda.Event(enable_timing=True)
+        self.e1 = torch.cuda.Event(enable_timing=True)
+
+    def graph_ms(self, b, reps, warmup=2):
+        for _ in range(warmup):
+            b.graph_launch()
+        self.torch.cuda.synchronize()
+        self.e0.record(self.stream)
+        for _ in range(reps):
+            b.graph_launch()
+        self.e1.record(self.stream)
+        self.torch.cuda.synchronize()
+        return self.e0.elapsed_time(self.e1) / reps
+
+
+def schedule(b, p, mode, tm, tune=True):
+    """Order each bin longest-first from a first pass, capture the step with
+    the batch's default kernel forms, then (``tune``) with the autotuned
+    forms, and keep whichever graph replays faster (the measured schedule)."""
+    key = "bs6" if mode == 6 else "bs2"
+    (b.icgn6 if mode == 6 else b.icgn2)(p)
+    b.reorder()
+    default_forms = [x[key] for x in b.bins()]
+    b.graph_capture(p, mode)
+    t_default = tm.graph_ms(b, 10)
+    out = {"default_forms": default_forms, "default_graph_ms": t_default, "measured": "default"}
+    if tune:
+        b.autotune(p, mode)
+        tuned = [x[key] for x in b.bins()]
+        out["autotuned_forms"] = tuned
+        b.graph_capture(p, mode)
+        out["autotuned_graph_ms"] = tm.graph_ms(b, 10)
+        if out["autotuned_graph_ms"] < t_default:
+            out["measured"] = "autotuned"
+        else:
+            b.set_forms(default_forms, mode)
+            b.graph_capture(p, mode)
+    log(f"[bench] mode {mode} schedule: {out}")
+    return out
+
+
+def roofline(b, p, mode, st, win_x, win_y, steps):
+    """Dominant launch of the step (the launch that carries the most
+    algorithmic bytes), timed by events on the stream it runs on over
+    ``steps`` eager steps."""
+    bins = b.bins()
+    kms = np.zeros(len(bins) + 1)
+    for _ in range(steps):
+        kms += b.step_timed(p, mode)
+    kms /= steps
+    bin_of = b.bin_of()
+    conv = (st >= 1) & (st <= 100)
+    iters = np.where(conv, st, 0).astype(float)
+    bytes_bin = np.array([algorithmic_bytes(win_x[bin_of == k], win_y[bin_of == k],
+                                            iters[bin_of == k]) for k in range(len(bins))])
+    kdom = int(np.argmax(bytes_bin))
+    kname = kernel_name(bins[kdom], mode)
+    bytes_launch = float(bytes_bin[kdom])
+    achieved = bytes_launch / (kms[kdom] / 1e3) / 1e9
+    log(f"[bench] dominant launch {kname}: {kms[kdom]:.3f} ms, {bytes_launch / 1e6:.1f} MB, "
+        f"{achieved:.1f} GB/s; launches {[round(float(x), 3) for x in kms]} ms")
+    return {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": achieved / HBM_PEAK_GBS, "traffic": _profiled_traffic(kname),
+            "kernel": kname, "subsets_per_launch": int((bin_of == kdom).sum()),
+            "algorithmic_bytes_per_launch": bytes_launch, "launch_ms": float(kms[kdom]),
+            "launch_ms_all": [float(x) for x in kms],
+            "step_algorithmic_bytes": float(bytes_bin.sum())}
+
+
+def local_step(ctx, tm, coords, win, truth, steps, warmup, slot=None, tune=True, roof=True):
+    """Build a batch, schedule it, replay the captured 6-DOF step ``steps``
+    times; returns (result dict, batch)."""
+    from staq_dic_amd import Batch, params
+    from staq_dic_amd._lib import IN_U0
+    b = Batch(ctx, coords, win, slot=slot)
+    U0 = np.round(truth[:, 4:6]).reshape(-1)
+    b.set_input(IN_U0, U0)
+    p6 = params(tol=1e-3)
+    sched = schedule(b, p6, 6, tm, tune=tune)
+    ms = tm.graph_ms(b, steps, warmup)
+    U, F, Hp, st = b.outputs()
+    conv, err, rms, rms_kept, kept = accuracy(U, st, truth)
+    n_conv = int(conv.sum())
+    out = {"subsets": int(len(coords)), "ms_per_step": ms, "converged": n_conv,
+           "converged_subsets_per_s": n_conv / (ms / 1e3), "rms_px": rms,
+           "rms_px_kept": rms_kept, "kept": int(kept.sum()),
+           "windows": {str(int(w)): int((win == w).sum()) for w in np.unique(win)},
+           "schedule": sched}
+    if roof:
+        out["roofline"] = roofline(b, p6, 6, st, win.astype(float), win.astype(float), steps)
+        out["roofline"]["step_achieved_gbs"] = out["roofline"]["step_algorithmic_bytes"] / (
+            ms / 1e3) / 1e9
+    log(f"[bench] icgn6 step {ms:.3f} ms, {n_conv}/{len(coords)} converged, rms {rms:.4f} px "
+        f"(kept {rms_kept:.4f} px)")
+    return out, b
+
+
+def multi_pair_leg(ctx, tm, steps, warmup):
+    """16 frame pairs resident at once (slots 0-15), all subsets in one batch
+    and one graph; each pair's own step replayed alone gives its latency."""
+    from staq_dic_amd import Batch, params, synth
+    from staq_dic_amd._lib import IN_U0
+    t0 = time.time()
+    pairs = synth.config_multi()
+    log(f"[bench] multi-pair workload ready: {sum(len(c) for _, c, _ in pairs)} subsets, "
+        f"{time.time() - t0:.1f}s")
+    for j, (fr, _, _) in enumerate(pairs):
+        ctx.set_frame(fr.f, fr.g, fr.mask, slot=j)
+    coords = np.concatenate([c for _, c, _ in pairs])
+    win = np.concatenate([w for _, _, w in pairs])
+    slot = np.concatenate([np.full(len(c), j, dtype=np.int32) for j, (_, c, _) in enumerate(pairs)])
+    truth = np.concatenate([synth.truth_at(fr.field, c) for fr, c, _ in pairs])
+    agg, b = local_step(ctx, tm, coords, win, truth, steps, warmup, slot=slot, tune=False)
+    b.close()
+    p6 = params(tol=1e-3)
+    lat = []
+    for j, (fr, c, w) in enumerate(pairs):
+        bj = Batch(ctx, c, w, slot=np.full(len(c), j, dtype=np.int32))
+        bj.set_input(IN_U0, np.round(synth.truth_at(fr.field, c)[:, 4:6]).reshape(-1))
+        bj.icgn6(p6)
+        bj.reorder()
+        bj.graph_capture(p6, 6)
+        lat.append(tm.graph_ms(bj, max(3, steps // 4)))
+        bj.close()
+    lat = np.asarray(lat)
+    agg.update({"workload": "16 pairs (seeds 5000-5015), sizes from {512,1024,1536,2048}^2, "
+                            "random affine fields (|t| <= 10 px, |grad u| <= 0.05), quadtree "
+                            "leaves 8-32 px, windows 16-64 px, 6-DOF local step, tol 1e-3",
+                "pair_sizes": [int(fr.f.shape[0]) for fr, _, _ in pairs],
+                "pair_subsets": [int(len(c)) for _, c, _ in pairs],
+                "pair_latency_ms": [float(x) for x in lat],
+                "pair_latency_ms_p50": float(np.percentile(lat, 50)),
+                "pair_latency_ms_p99": float(np.percentile(lat, 99)),
+                "pairs_in_sequence_ms": float(lat.sum())})
+    log(f"[bench] multi-pair: one graph {agg['ms_per_step']:.3f} ms, per-pair p50 "
+        f"{agg['pair_latency_ms_p50']:.3f} ms p99 {agg['pair_latency_ms_p99']:.3f} ms")
+    return agg
+
+
+def large_leg(ctx, tm, steps, warmup):
+    """The large-deformation incremental pair (frames 39 -> 40 of the 4096^2
+    16-bit sequence): the integer search (w 64, r 48) and the 6-DOF local step
+    over ~100 k quadtree subsets."""
+    from staq_dic_amd import synth
+    from staq_dic_amd.api import funIntegerSearch
+    t0 = time.time()
+    fr, coords, win = synth.config_large()
+    truth = synth.truth_at(fr.field, coords)
+    log(f"[bench] large-deformation workload ready: {len(coords)} subsets, "
+        f"{time.time() - t0:.1f}s")
+    H, W = fr.f.shape
+    ctx.set_frame(fr.f, fr.g, fr.mask, slot=0)
+    import torch
+    torch.cuda.synchronize()
+    t = time.perf_counter()
+    x, y, u, v, cc = funIntegerSearch(fr.f, fr.g, 48, [1, H], [1, W], 64, 32, ctx=ctx, slot=0)
+    dt = time.perf_counter() - t
+    tr = synth.truth_at(fr.field, np.stack([x.ravel(order="F"), y.ravel(order="F")], axis=1))
+    ok = np.isfinite(u.ravel(order="F"))
+    du = np.abs(u.ravel(order="F")[ok] - tr[ok, 4])
+    dv = np.abs(v.ravel(order="F")[ok] - tr[ok, 5])
+    search = {"points": int(x.size), "seconds_host_to_host": dt, "points_per_s": x.size / dt,
+              "within_1px": float(np.mean((du <= 1) & (dv <= 1)))}
+    log(f"[bench] large integer search: {x.size} points in {dt * 1e3:.1f} ms")
+    res, b = local_step(ctx, tm, coords, win, truth, steps, warmup, tune=False)
+    b.close()
+    res.update({"workload": "4096x4096 16-bit speckle (192 000 speckles, seed 1004), frames 39 "
+                            "-> 40 of the stretch (0.2 k/40) + bump (40 k/40 px, sigma 300) "
+                            "sequence, quadtree leaves 4-64 px, windows 16-64 px, 6-DOF local "
+                            "step, tol 1e-3",
+                "integer_search": search})
+    return res
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -66,6 +279,10 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-extras", action="store_true")
+    ap.add_argument("--no-large", action="store_true")
+    ap.add_argument("--windows", choices=["reference", "leaf"], default="reference",
+                    help="window rule of the measured step: the reference's one window for "
+                         "every node (32 px), or windows tied to the leaf size")
     ap.add_argument("--copy-calib", action="store_true",
                     help="first run a 256 MiB elementwise copy (read 256 MiB, write 256 MiB) "
                          "three times, the known-size kernel that scales the HBM counters")
@@ -77,7 +294,7 @@ def main():
 
     import torch
 
-    from staq_dic_amd import Batch, Context, params, synth
+    from staq_dic_amd import Context, params, synth
     from staq_dic_amd._lib import IN_U0
     from staq_dic_amd.api import funIntegerSearch
 
@@ -90,141 +307,58 @@ def main():
         del src, dst
 
     t0 = time.time()
-    fr, coords, win = synth.config_hole()
+    rule = args.windows
+    fr, coords, win_leaf = synth.config_hole()
+    win = np.full(len(coords), 32, dtype=np.int64) if rule == "reference" else win_leaf
     truth = synth.truth_at(fr.field, coords)
     N = coords.shape[0]
-    U0 = np.round(truth[:, 4:6]).reshape(-1)
     log(f"[bench] workload ready: {N} subsets, {time.time() - t0:.1f}s")
 
     ctx = Context(0)
     ctx.set_frame(fr.f, fr.g, fr.mask)
-    b = Batch(ctx, coords, win)
-    b.set_input(IN_U0, U0)
-    p6 = params(tol=1e-3)
     stream = torch.cuda.ExternalStream(ctx.stream)
+    tm = Timer(stream)
 
-    # schedule: order each bin longest-first from a first pass, autotune the
-    # kernel form per bin, keep whichever of the default and the tuned
-    # schedule replays faster as a graph, then capture the step in a graph
-    ev0 = torch.cuda.Event(enable_timing=True)
-    ev1 = torch.cuda.Event(enable_timing=True)
-
-    def graph_ms(p, mode, reps=10):
-        b.graph_capture(p, mode)
-        for _ in range(2):
-            b.graph_launch()
-        torch.cuda.synchronize()
-        ev0.record(stream)
-        for _ in range(reps):
-            b.graph_launch()
-        ev1.record(stream)
-        torch.cuda.synchronize()
-        return ev0.elapsed_time(ev1) / reps
-
-    def schedule(p, mode):
-        """Order each bin longest-first from a first pass, capture the step
-        with the batch's default kernel forms (the measured schedule), and
-        time the autotuned forms beside it for the record."""
-        key = "bs6" if mode == 6 else "bs2"
-        (b.icgn6 if mode == 6 else b.icgn2)(p)
-        b.reorder()
-        default_forms = [x[key] for x in b.bins()]
-        t_default = graph_ms(p, mode)
-        b.autotune(p, mode)
-        tuned_forms = [x[key] for x in b.bins()]
-        t_tuned = graph_ms(p, mode)
-        b.set_forms(default_forms, mode)
-        log(f"[bench] mode {mode} schedule: default {t_default:.3f} ms {default_forms}, "
-            f"autotuned {t_tuned:.3f} ms {tuned_forms}")
-        b.graph_capture(p, mode)
-        return {"default_forms": default_forms, "default_graph_ms": t_default,
-                "autotuned_forms": tuned_forms, "autotuned_graph_ms": t_tuned}
-
-    sched6 = schedule(p6, 6)
-    for _ in range(args.warmup):
-        b.graph_launch()
-    torch.cuda.synchronize()
-    ev0.record(stream)
-    for _ in range(args.steps):
-        b.graph_launch()
-    ev1.record(stream)
-    torch.cuda.synchronize()
-    ms = ev0.elapsed_time(ev1) / args.steps
-    U, F, Hp, st = b.outputs()
-    conv = (st >= 1) & (st <= 100)
-    n_conv = int(conv.sum())
-    err = np.hypot(U[0::2] - truth[:, 4], U[1::2] - truth[:, 5])
-    rms = float(np.sqrt(np.mean(err[conv] ** 2)))
-    e_sorted = np.sort(err[conv])
-    rms99 = float(np.sqrt(np.mean(e_sorted[: int(0.99 * e_sorted.size)] ** 2)))
-    value = n_conv / (ms / 1e3)
-    log(f"[bench] icgn6 step {ms:.3f} ms, {n_conv}/{N} converged, rms {rms:.4f} px")
-
-    # roofline of the dominant kernel (the launch that carries the most
-    # algorithmic bytes): every launch of the step timed with events on the
-    # stream it runs on, over the same number of eager steps
-    bins = b.bins()
-    kms = np.zeros(len(bins) + 1)
-    for _ in range(args.steps):
-        kms += b.step_timed(p6, 6)
-    kms /= args.steps
-    bin_of = b.bin_of()
-    iters = np.where(conv, st, 0).astype(float)
-    bytes_bin = np.array([algorithmic_bytes(win[bin_of == k], win[bin_of == k], iters[bin_of == k])
-                          for k in range(len(bins))])
-    kdom = int(np.argmax(bytes_bin))
-    bdom = bins[kdom]
-    bytes_launch = float(bytes_bin[kdom])
-    achieved = bytes_launch / (kms[kdom] / 1e3) / 1e9
-    f6 = bdom["bs6"]
-    if f6 in (1, 2, 4):
-        kname = f"icgn_lean_kernel<{bdom['ppt']},6,{f6}>"
-    elif f6 <= 0:
-        kname = f"icgn_wave_kernel<{bdom['ppt']},6,{1 if f6 == 0 else -f6}>"
-    else:
-        kname = f"icgn_kernel<{f6},6>"
-    step_bytes = float(bytes_bin.sum())
-    log(f"[bench] dominant launch {kname}: {kms[kdom]:.3f} ms, {bytes_launch / 1e6:.1f} MB, "
-        f"{achieved:.1f} GB/s; launches {[round(float(x), 3) for x in kms]} ms")
-    traffic = _profiled_traffic(kname)
-
+    head, b = local_step(ctx, tm, coords, win, truth, args.steps, args.warmup)
+    ms = head["ms_per_step"]
+    value = head["converged_subsets_per_s"]
+    rms = head["rms_px"]
+    roof = head.pop("roofline")
+    wl = ("reference window rule: every node 32 px (DICpara.winsize), grown at the hole"
+          if rule == "reference" else "windows tied to the leaf size, clamp(2 leaf, 16, 64)")
     result = {
         "metric": METRIC, "value": value, "unit": "subsets/s", "n_gpus": 1,
         "steps": args.steps, "warmup": args.warmup, "ms_per_step": ms,
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
         "data": "synthetic",
         "config": {"workload": "hole-quadtree-1024: 1024x1024 speckle pair, r=150 hole, Kirsch "
-                               "field 8 px, quadtree leaves 8-32 px, windows 16-64 px, 6-DOF "
-                               "IC-GN local step, tol 1e-3",
+                               "field 8 px, quadtree leaves 8-32 px, " + wl + ", 6-DOF IC-GN "
+                               "local step, tol 1e-3",
                    "subsets": N, "parallelism": "1 GPU, one graph per local step"},
-        "converged": n_conv, "rms_px": rms, "rms_px_best99": rms99, "schedule": sched6,
-        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
-                     "kernel": kname, "subsets_per_launch": int((bin_of == kdom).sum()),
-                     "algorithmic_bytes_per_launch": bytes_launch,
-                     "launch_ms": float(kms[kdom]),
-                     "launch_ms_all": [float(x) for x in kms],
-                     "step_algorithmic_bytes": step_bytes,
-                     "step_achieved_gbs": step_bytes / (ms / 1e3) / 1e9},
+        "converged": head["converged"], "rms_px": rms, "rms_px_kept": head["rms_px_kept"],
+        "accuracy_gate_0p01px": {"converged": bool(rms <= 0.01),
+                                 "kept_by_reference_rule": bool(head["rms_px_kept"] <= 0.01)},
+        "schedule": head["schedule"], "roofline": roof,
     }
 
     if not args.no_extras:
+        # the same nodes with windows tied to their leaf size
+        other = win_leaf if rule == "reference" else np.full(len(coords), 32, dtype=np.int64)
+        alt, balt = local_step(ctx, tm, coords, other, truth, args.steps, args.warmup, tune=False,
+                               roof=False)
+        balt.close()
+        result["leaf_window_rule" if rule == "reference" else "reference_window_rule"] = alt
+
         # 2-DOF AL step on the 6-DOF result (identity global step, vdual = 0),
         # scheduled and replayed the same way
+        U0 = np.round(truth[:, 4:6]).reshape(-1)
+        p6 = params(tol=1e-3)
         b.set_input(IN_U0, U0)
         b.icgn6(p6)
         b.chain()
         p2 = params(tol=1e-3, mu=1e-3)
-        sched2 = schedule(p2, 2)
-        for _ in range(args.warmup):
-            b.graph_launch()
-        torch.cuda.synchronize()
-        ev0.record(stream)
-        for _ in range(args.steps):
-            b.graph_launch()
-        ev1.record(stream)
-        torch.cuda.synchronize()
-        ms2 = ev0.elapsed_time(ev1) / args.steps
+        sched2 = schedule(b, p2, 2, tm)
+        ms2 = tm.graph_ms(b, args.steps, args.warmup)
         _, _, _, st2 = b.outputs()
         c2 = int(((st2 >= 1) & (st2 <= 100)).sum())
         result["icgn2_al_step"] = {"ms": ms2, "converged": c2, "subsets_per_s": c2 / (ms2 / 1e3),
@@ -260,28 +394,47 @@ def main():
             P = torch.tensor(ta, dtype=torch.float64, device=dev, requires_grad=True)
             cc_ = torch.tensor(ca, device=dev)
             ww = torch.tensor(wa, dtype=torch.int32, device=dev)
+            # the batch above is finished with frame slot 0; the adjoint reads slot 0
             ctx.set_frame(fa.f, fa.g, fa.mask, slot=0)
             for _ in range(2):
                 cost, _ = znssd(P, cc_, ww, ctx)
                 cost.sum().backward()
             torch.cuda.synchronize()
-            ev0.record(stream)
+            tm.e0.record(stream)
             for _ in range(args.steps):
                 P.grad = None
                 cost, _ = znssd(P, cc_, ww, ctx)
                 cost.sum().backward()
-            ev1.record(stream)
+            tm.e1.record(stream)
             torch.cuda.synchronize()
-        msa = ev0.elapsed_time(ev1) / args.steps
+        msa = tm.e0.elapsed_time(tm.e1) / args.steps
         result["adjoint_znssd"] = {"workload": "2048x2048 sinusoidal field, 16-px grid, window 16",
                                    "subsets": int(len(ca)), "ms_fwd_bwd": msa,
                                    "subsets_per_s": len(ca) / (msa / 1e3)}
         log(f"[bench] adjoint fwd+bwd {msa:.3f} ms for {len(ca)} subsets")
-        ctx.set_frame(fr.f, fr.g, fr.mask, slot=0)
+        b.close()
+        ctx.close()
+
+        # multi-instance batch and large-deformation pair, each in a fresh context
+        ctx = Context(0)
+        result["multi_pair"] = multi_pair_leg(ctx, Timer(torch.cuda.ExternalStream(ctx.stream)),
+                                              args.steps, args.warmup)
+        ctx.close()
+        if not args.no_large:
+            ctx = Context(0)
+            result["large_deformation"] = large_leg(
+                ctx, Timer(torch.cuda.ExternalStream(ctx.stream)), max(3, args.steps // 4), 1)
+            ctx.close()
+        ctx = None
+    else:
+        b.close()
+        ctx.close()
+        ctx = None
 
     if not args.no_cpu:
         from oracle import pyoracle
         of = pyoracle.Frame(fr.f, fr.g, fr.mask)
+        U0 = np.round(truth[:, 4:6]).reshape(-1)
         cores = min(os.cpu_count() or 1, 16)
         reps, t_cpu, cconv = 0, 0.0, 0
         while t_cpu < args.cpu_seconds:
@@ -296,8 +449,6 @@ def main():
                                             f"(FP64 oracle, OpenMP dynamic), {t_cpu:.1f} s"}
         log(f"[bench] cpu {cconv / t_cpu:.0f} subsets/s on {cores} threads")
 
-    b.close()
-    ctx.close()
     print(json.dumps(result))
 
 

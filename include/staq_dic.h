@@ -51,9 +51,9 @@ typedef struct staq_params {
                         whole-window statistics, DfDx==0 mask rule)                      */
     int block_size;  /* kernel form for every bin: 0 = the batch's (tuned) schedule;
                         64/128/256/512 = workgroup kernel of that size; -1/-2/-4 = that
-                        many wavefronts per subset; 1/2/4 = the lean kernel for windows
-                        without masked pixels with that many wavefronts (a subset it
-                        cannot finish goes to the fallback launch)                       */
+                        many wavefronts per subset; 1/2/4/8/16 = the lean kernel for
+                        windows without masked pixels with that many wavefronts (a subset
+                        it cannot finish goes to the fallback launch)                    */
     int persistent;  /* 1: persistent grid pulling subsets from a device work queue      */
 } staq_params;
 
@@ -203,8 +203,8 @@ int staq_batch_bins(staq_batch *b, int *nbins, int *wcap, long *count, int *bs6,
                     int *ppt, int maxbins);
 
 /* Set the kernel form of bin `bin` for mode 6 or 2 (0 = one wavefront per
- * subset, -1/-2/-4 = that many wavefronts, 1/2/4 = the lean kernel for windows
- * without masked pixels with that many wavefronts, 64..512 = workgroup
+ * subset, -1/-2/-4 = that many wavefronts, 1/2/4/8/16 = the lean kernel for
+ * windows without masked pixels with that many wavefronts, 64..512 = workgroup
  * kernel), as staq_batch_autotune would; drops a captured graph. */
 int staq_batch_set_form(staq_batch *b, int bin, int mode, int form);
 /* Bin index of every subset (N entries, in the caller's subset order).  Bins

@@ -514,10 +514,16 @@ static int default_bs(int npix)
 // of that size per subset (icgn.hip).
 static bool valid_form(int f)
 {
-    return f == 0 || f == -1 || f == -2 || f == -4 || f == 1 || f == 2 || f == 4 || f == 64 ||
-           f == 128 || f == 256 || f == 512;
+    return f == 0 || f == -1 || f == -2 || f == -4 || f == 1 || f == 2 || f == 4 || f == 8 ||
+           f == 16 || f == 64 || f == 128 || f == 256 || f == 512;
 }
-static bool lean_form(int f) { return f == 1 || f == 2 || f == 4; }
+static bool lean_form(int f) { return f == 1 || f == 2 || f == 4 || f == 8 || f == 16; }
+// pixel slots per lane of a wavefront form for windows of npix pixels (0: too large)
+static int form_ppt(int f, int npix)
+{
+    if (lean_form(f)) return icgn_lean_ppt(npix, f);
+    return f <= 0 ? icgn_wave_ppt(npix, f == 0 ? 1 : -f) : 0;
+}
 // wavefronts per subset of a wavefront form, 0 for a workgroup form
 static int form_nw(int f)
 {
@@ -578,7 +584,7 @@ static int build_bins(staq_batch *b)
         // pixel slots hold the window, else four (two) wavefronts; bins for
         // the general kernel take four (two, one) wavefronts; windows beyond
         // that take the workgroup kernel
-        int form = default_bs(bin.npix_cap);
+        int form = default_bs(bin.wave_npix);
         const int g = bin.masked ? -1 : 1;
         if (!bin.masked && bin.ppt && bin.ppt <= 5)
             form = 1;
@@ -588,6 +594,8 @@ static int build_bins(staq_batch *b)
             form = 2 * g;
         else if (bin.ppt)
             form = bin.masked ? 0 : 1;
+        else if (!bin.masked && icgn_lean_ppt(bin.wave_npix, 16))
+            form = 16;
         bin.bs[0] = bin.bs[1] = form;
         b->bins.push_back(bin);
         fbx = std::max(fbx, grown_window(bin.wxcap));
@@ -815,8 +823,7 @@ int staq_batch_bins(staq_batch *b, int *nbins, int *wcap, long *count, int *bs6,
         if (bs6) bs6[k] = b->bins[k].bs[0];
         if (bs2) bs2[k] = b->bins[k].bs[1];
         if (ppt) {
-            const int nw = form_nw(b->bins[k].bs[0]);
-            ppt[k] = nw ? icgn_wave_ppt(b->bins[k].wave_npix, nw) : 0;
+            ppt[k] = form_ppt(b->bins[k].bs[0], b->bins[k].wave_npix);
         }
     }
     return STAQ_OK;
@@ -830,7 +837,7 @@ extern "C" int staq_batch_set_form(staq_batch *b, int bin, int mode, int form)
         return STAQ_EINVAL;
     if (!valid_form(form))
         return fail(b->ctx, STAQ_EINVAL,
-                    "kernel form must be 0, -1, -2, -4, 1, 2, 4, 64, 128, 256 or 512");
+                    "kernel form must be 0, -1, -2, -4, 1, 2, 4, 8, 16, 64, 128, 256 or 512");
     b->bins[bin].bs[mode == 6 ? 0 : 1] = form;
     if (b->gexec) {  // a captured graph holds the old schedule
         hipGraphExecDestroy(b->gexec);
@@ -927,11 +934,11 @@ static int enqueue_step(staq_batch *b, const staq_params *p, int mode, hipStream
                                              : (p->block_size ? p->block_size : bin.bs[mi]);
         if (!valid_form(bs))
             return fail(ctx, STAQ_EINVAL,
-                        "block_size must be 0, -1, -2, -4, 1, 2, 4, 64, 128, 256 or 512");
+                        "block_size must be 0, -1, -2, -4, 1, 2, 4, 8, 16, 64, 128, 256 or 512");
         const int nw = form_nw(bs);
         const bool lean = lean_form(bs);
-        const int wppt = nw ? icgn_wave_ppt(bin.wave_npix, nw) : 0;
-        if (nw && !wppt) bs = default_bs(bin.npix_cap);
+        const int wppt = nw ? form_ppt(bs, bin.wave_npix) : 0;
+        if (nw && !wppt) bs = default_bs(bin.wave_npix);
         int grid = (int)bin.n;
         if (timed) HIPCHK(ctx, hipEventRecord(b->tev[2 * k], sk));
         if (nw && wppt) {
@@ -948,6 +955,11 @@ static int enqueue_step(staq_batch *b, const staq_params *p, int mode, hipStream
             else
                 HIPCHK(ctx, icgn_wave_launch(a, mode, wppt, nw, grid, sk));
         } else {
+            // the workgroup kernel holds the windows its subsets grow to at
+            // set-up, so none of them defers to the fallback launch
+            const int gside = tile_side((int)std::ceil(std::sqrt((double)bin.wave_npix)));
+            a.npix_cap = std::max(bin.npix_cap, bin.wave_npix);
+            a.tile_cap = std::max(bin.tile_cap, gside * gside);
             if (p->persistent) {
                 a.work_counter = ints + k;
                 grid = occupancy_grid(bs, icgn_lds_bytes(a.npix_cap, a.tile_cap), bin.n);
@@ -1132,7 +1144,7 @@ int staq_batch_autotune(staq_batch *b, const staq_params *p, int mode)
         HIPCHK(ctx, hipEventCreate(&e));
         b->tev.push_back(e);
     }
-    const int cand[10] = {0, -2, -4, 1, 2, 4, 64, 128, 256, 512};
+    const int cand[12] = {0, -2, -4, 1, 2, 4, 8, 16, 64, 128, 256, 512};
     const int mi = mode == 6 ? 0 : 1;
     staq_params q = *p;
     q.block_size = 0;
@@ -1161,7 +1173,7 @@ int staq_batch_autotune(staq_batch *b, const staq_params *p, int mode)
         int best_bs = cur;
         for (int c : cand) {
             if (c == cur) continue;
-            if (form_nw(c) && !icgn_wave_ppt(bin.wave_npix, form_nw(c))) continue;
+            if (form_nw(c) && !form_ppt(c, bin.wave_npix)) continue;
             if (lean_form(c) && bin.masked) continue;  // its subsets would all defer
             if (c == 512 && bin.npix_cap < 1500) continue;
             if (c == 64 && bin.npix_cap > 4500) continue;

@@ -7,8 +7,9 @@
 //   * no bwselect regions, no window growth, no FP64 residual pass: a subset
 //     that meets any of them (a zero of f*mask in its window, a zero of the
 //     warped g under the quadtree rules, degenerate statistics, a Hessian that
-//     is not positive definite, a window beyond 64 * PPT * NW pixels) is left
-//     for the fallback launch (status -2), which solves it from the start;
+//     is not positive definite, a window beyond 64 * PPT * NW pixels, a warped
+//     support beyond the g tile) is left for the fallback launch (status -2),
+//     which solves it from the start;
 //   * per pixel slot the lane keeps df/dx, df/dy and the window offsets
 //     (x - x0, y - y0) in registers, so an iteration is the warp, the Keys
 //     gather from the resident g tile, eight sums, and the FP64 step;
@@ -25,15 +26,16 @@ namespace staq {
 
 namespace {
 
+template <int NW>
 struct LeanLds {
     double Lp[21];   // packed Cholesky factor of H (column-major lower triangle)
     double Ld[6];    // 1 / diag(Lp)
     double H2i[4];   // 2-DOF: inverse of the damped 2x2 system
     double FJ[6], SJ[6];
     // exchange between the wavefronts of one subset (double-buffered)
-    float xf[2][4][24];
-    double xd[2][4][16];
-    int xi[2][4][8];
+    float xf[2][NW][24];
+    double xd[2][NW][16];
+    int xi[2][NW][8];
 };
 
 template <int PPT, int MODE, int NW>
@@ -42,7 +44,7 @@ __global__ __launch_bounds__(64 * NW) void icgn_lean_kernel(IcgnArgs a)
     constexpr int NJ = MODE;
     constexpr int CAP = 64 * PPT * NW;
     extern __shared__ __align__(16) unsigned char smem[];
-    __shared__ LeanLds L;
+    __shared__ LeanLds<NW> L;
     __shared__ long item_sh;
     float *tile = reinterpret_cast<float *>(smem);
     const int tid = threadIdx.x;
@@ -91,7 +93,7 @@ __global__ __launch_bounds__(64 * NW) void icgn_lean_kernel(IcgnArgs a)
             if (!fits) {
                 state = 2;
                 status = ST_ERROR;
-            } else if ((wx + 1) * (wy + 1) > CAP || wx + 1 > 64 || wy + 1 > 64) {
+            } else if ((wx + 1) * (wy + 1) > CAP) {
                 state = 3;
             }
         }
@@ -309,7 +311,6 @@ __global__ __launch_bounds__(64 * NW) void icgn_lean_kernel(IcgnArgs a)
             // FP32 local coordinates; the tile keeps a margin of up to 4
             const int nu0 = (int)floor(umin) - 2, nv0 = (int)floor(vmin) - 2;
             const int nu1 = (int)floor(umax) + 3, nv1 = (int)floor(vmax) + 3;
-            bool use_tile = true;
             const bool reload = !have_tile || nu0 < tu0 || nv0 < tv0 || nu1 >= tu0 + tnx ||
                                 nv1 >= tv0 + tny;
             if (reload) {
@@ -319,19 +320,19 @@ __global__ __launch_bounds__(64 * NW) void icgn_lean_kernel(IcgnArgs a)
                 tv0 = nv0 - m;
                 tnx = nu1 - nu0 + 1 + 2 * m;
                 tny = nv1 - nv0 + 1 + 2 * m;
-                use_tile = tnx * tny <= a.tile_cap;
-                have_tile = use_tile;
-                if (!use_tile) {  // gather from HBM; origin as for a tile
-                    tu0 = nu0;
-                    tv0 = nv0;
-                    tnx = nu1 - nu0 + 1;
-                    tny = nv1 - nv0 + 1;
+                if (tnx * tny > a.tile_cap) {
+                    // a warped support beyond the tile is left to the general
+                    // kernel, which gathers from HBM; without a second gather
+                    // path this kernel needs a third fewer registers
+                    state = 3;
+                    break;
                 }
+                have_tile = true;
             }
             const float a11 = (float)(1 + P[0]), a12 = (float)P[2], a21 = (float)P[1],
                         a22 = (float)(1 + P[3]);
             const float cu = (float)((x0 + P[4]) - tu0), cv = (float)((y0 + P[5]) - tv0);
-            if (use_tile && reload) {
+            if (reload) {
                 __syncthreads();  // the last iteration's gathers are done with the old tile
                 const int total = tnx * tny;
                 const float rcpn = 1.0f / tnx;
@@ -355,9 +356,8 @@ __global__ __launch_bounds__(64 * NW) void icgn_lean_kernel(IcgnArgs a)
                 }
                 __syncthreads();
             }
-            // Keys bicubic of the window pixel at offset (dx, dy), from the
-            // resident tile or, if the warped support outgrew it, from HBM
-            auto gather = [&](auto from_tile, float dx, float dy) {
+            // Keys bicubic of the window pixel at offset (dx, dy) from the resident tile
+            auto gather = [&](float dx, float dy) {
                 const float lu = fmaf(a11, dx, fmaf(a12, dy, cu));
                 const float lv = fmaf(a21, dx, fmaf(a22, dy, cv));
                 const float fu = floorf(lu), fvv = floorf(lv);
@@ -366,23 +366,11 @@ __global__ __launch_bounds__(64 * NW) void icgn_lean_kernel(IcgnArgs a)
                 keys_w(lv - fvv, wv);
                 const int iu = (int)fu - 1, iv = (int)fvv - 1;
                 float g = 0.f;
-                if constexpr (decltype(from_tile)::value) {
-                    const float *b = tile + iu + iv * tnx;
+                const float *b = tile + iu + iv * tnx;
 #pragma unroll
-                    for (int q = 0; q < 4; ++q) {
-                        const float *c4 = b + q * tnx;
-                        g = fmaf(wv[q], fmaf(wu[0], c4[0], fmaf(wu[1], c4[1], fmaf(wu[2], c4[2], wu[3] * c4[3]))), g);
-                    }
-                } else {
-#pragma unroll
-                    for (int q = 0; q < 4; ++q) {
-                        const int y = min(max(tv0 + iv + q, 1), fr.W);
-                        const float *col = fr.g + (size_t)(y - 1) * fr.H;
-                        float s = 0.f;
-#pragma unroll
-                        for (int r = 0; r < 4; ++r) s = fmaf(wu[r], gld(col, (size_t)(min(max(tu0 + iu + r, 1), fr.H) - 1)), s);
-                        g = fmaf(wv[q], s, g);
-                    }
+                for (int q = 0; q < 4; ++q) {
+                    const float *c4 = b + q * tnx;
+                    g = fmaf(wv[q], fmaf(wu[0], c4[0], fmaf(wu[1], c4[1], fmaf(wu[2], c4[2], wu[3] * c4[3]))), g);
                 }
                 return g;
             };
@@ -394,11 +382,11 @@ __global__ __launch_bounds__(64 * NW) void icgn_lean_kernel(IcgnArgs a)
             for (int q = 0; q < 2 + NJ; ++q) acc[q] = 0.f;
             int n_nz = 0;
             bool any_zero = false, any_nan = false;
-            auto fused_pass = [&](auto from_tile) {
+            {
 #pragma unroll
                 for (int k = 0; k < PPT; ++k) {
                     const bool valid = (vbits >> k) & 1u;
-                    const float g = gather(from_tile, dxr[k], dyr[k]);
+                    const float g = gather(dxr[k], dyr[k]);
                     any_zero |= valid && g == 0.f;
                     any_nan |= valid && isnan(g);
                     const float ge = valid ? g : 0.f;
@@ -420,11 +408,7 @@ __global__ __launch_bounds__(64 * NW) void icgn_lean_kernel(IcgnArgs a)
                     }
                     n_nz += (int)__popcll(__ballot(nz));
                 }
-            };
-            if (use_tile)
-                fused_pass(std::true_type{});
-            else
-                fused_pass(std::false_type{});
+            }
             int xv[4] = {n_nz, wave_any(any_zero) ? 1 : 0, wave_any(any_nan) ? 1 : 0, 0};
             if constexpr (NW == 1)
                 group_reduce<NW, 0, 0, 4>(L, par, nullptr, nullptr, xv);
@@ -559,17 +543,34 @@ hipError_t launch_lean_ppt(const IcgnArgs &a, int ppt, int grid, hipStream_t s)
         case 7: return launch_lean<7, MODE, 2>(a, grid, s);
         case 9: return launch_lean<9, MODE, 2>(a, grid, s);
         }
-    } else {
+    } else if constexpr (NW == 4) {
         switch (ppt) {
         case 2: return launch_lean<2, MODE, 4>(a, grid, s);
         case 3: return launch_lean<3, MODE, 4>(a, grid, s);
         case 5: return launch_lean<5, MODE, 4>(a, grid, s);
+        case 7: return launch_lean<7, MODE, 4>(a, grid, s);
+        case 9: return launch_lean<9, MODE, 4>(a, grid, s);
         }
+    } else if constexpr (NW == 8) {
+        if (ppt == 9) return launch_lean<9, MODE, 8>(a, grid, s);
+    } else if constexpr (NW == 16) {
+        if (ppt == 5) return launch_lean<5, MODE, 16>(a, grid, s);
     }
     return hipErrorInvalidValue;
 }
 
 }  // namespace
+
+int icgn_lean_ppt(int npix_cap, int nw)
+{
+    // up to four wavefronts: the slot counts of the general kernel; eight and
+    // sixteen wavefronts (windows of up to 4608 and 5120 pixels, the 64-px
+    // windows): nine and five slots per lane
+    if (nw <= 4) return icgn_wave_ppt(npix_cap, nw);
+    if (nw == 8) return 64 * 8 * 9 >= npix_cap ? 9 : 0;
+    if (nw == 16) return 64 * 16 * 5 >= npix_cap ? 5 : 0;
+    return 0;
+}
 
 hipError_t icgn_lean_launch(const IcgnArgs &a, int mode, int ppt, int nw, int grid, hipStream_t s)
 {
@@ -577,10 +578,14 @@ hipError_t icgn_lean_launch(const IcgnArgs &a, int mode, int ppt, int nw, int gr
         if (nw == 1) return launch_lean_ppt<6, 1>(a, ppt, grid, s);
         if (nw == 2) return launch_lean_ppt<6, 2>(a, ppt, grid, s);
         if (nw == 4) return launch_lean_ppt<6, 4>(a, ppt, grid, s);
+        if (nw == 8) return launch_lean_ppt<6, 8>(a, ppt, grid, s);
+        if (nw == 16) return launch_lean_ppt<6, 16>(a, ppt, grid, s);
     } else {
         if (nw == 1) return launch_lean_ppt<2, 1>(a, ppt, grid, s);
         if (nw == 2) return launch_lean_ppt<2, 2>(a, ppt, grid, s);
         if (nw == 4) return launch_lean_ppt<2, 4>(a, ppt, grid, s);
+        if (nw == 8) return launch_lean_ppt<2, 8>(a, ppt, grid, s);
+        if (nw == 16) return launch_lean_ppt<2, 16>(a, ppt, grid, s);
     }
     return hipErrorInvalidValue;
 }

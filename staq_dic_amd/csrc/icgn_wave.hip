@@ -1131,6 +1131,8 @@ hipError_t launch_wave_ppt(const IcgnArgs &a, int ppt, int grid, hipStream_t s)
         case 2: return launch_wave<2, MODE, 4>(a, grid, s);
         case 3: return launch_wave<3, MODE, 4>(a, grid, s);
         case 5: return launch_wave<5, MODE, 4>(a, grid, s);
+        case 7: return launch_wave<7, MODE, 4>(a, grid, s);
+        case 9: return launch_wave<9, MODE, 4>(a, grid, s);
         }
     }
     return hipErrorInvalidValue;
@@ -1142,9 +1144,9 @@ int icgn_wave_ppt(int npix_cap, int nw)
 {
     // smallest instantiated pixels-per-lane count that holds npix_cap pixels
     // over nw wavefronts
-    static const int s1[] = {5, 6, 7, 9, 18}, s2[] = {3, 4, 5, 7, 9}, s4[] = {2, 3, 5};
+    static const int s1[] = {5, 6, 7, 9, 18}, s2[] = {3, 4, 5, 7, 9}, s4[] = {2, 3, 5, 7, 9};
     const int *set = nw == 1 ? s1 : (nw == 2 ? s2 : s4);
-    const int n = nw == 1 ? 5 : (nw == 2 ? 5 : 3);
+    const int n = 5;
     if (nw != 1 && nw != 2 && nw != 4) return 0;
     for (int i = 0; i < n; ++i)
         if (64 * nw * set[i] >= npix_cap) return set[i];

@@ -62,8 +62,10 @@ hipError_t icgn_launch(const IcgnArgs &a, int mode, int bs, int grid, hipStream_
 // (0: too large for the wavefront kernel)
 int icgn_wave_ppt(int npix_cap, int nw);
 hipError_t icgn_wave_launch(const IcgnArgs &a, int mode, int ppt, int nw, int grid, hipStream_t s);
-// lean wavefront kernel for windows without masked pixels (icgn_lean.hip);
-// the pixel-slot counts are those of icgn_wave_ppt
+// lean wavefront kernel for windows without masked pixels (icgn_lean.hip),
+// with 1, 2, 4, 8 or 16 wavefronts per subset; pixels per lane as
+// icgn_wave_ppt up to four wavefronts (0: too large)
+int icgn_lean_ppt(int npix_cap, int nw);
 hipError_t icgn_lean_launch(const IcgnArgs &a, int mode, int ppt, int nw, int grid, hipStream_t s);
 size_t icgn_lds_bytes(int npix_cap, int tile_cap);
 hipError_t ba_interp2_launch(const void *F, long M, long N, long P, const void *X, const void *Y,

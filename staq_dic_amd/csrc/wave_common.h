@@ -59,6 +59,7 @@ __device__ __forceinline__ void solve_chol6(const double *Lp, const double *Ld, 
 #pragma unroll
         for (int k = 0; k < i; ++k) s -= LP(i, k) * y[k];
         y[i] = s * Ld[i];
+        __builtin_amdgcn_sched_barrier(0);
     }
 #pragma unroll
     for (int i = 5; i >= 0; --i) {
@@ -66,6 +67,7 @@ __device__ __forceinline__ void solve_chol6(const double *Lp, const double *Ld, 
 #pragma unroll
         for (int k = i + 1; k < 6; ++k) s -= LP(k, i) * d[k];
         d[i] = s * Ld[i];
+        __builtin_amdgcn_sched_barrier(0);
     }
 #undef LP
 }
@@ -93,6 +95,41 @@ __device__ __forceinline__ void group_reduce(Lds &L, int &par, float *f, double 
             for (int q = 0; q < NI; ++q) L.xi[par][w][q] = iv[q];
         }
         __syncthreads();
+        if constexpr (ND > 0 || NW > 4) {
+            // many values (the set-up sums) or many wavefronts: lane q adds
+            // the wavefronts' partials of value q (floats in lanes 0..,
+            // doubles in lanes 32.., ints in lanes 56..), then each total is
+            // read from its lane; no lane holds every partial at once
+            static_assert(NF <= 24 && ND <= 16 && NI <= 8, "exchange layout");
+            const int ln = threadIdx.x & 63;
+            float vf = 0.f;
+            double vd = 0.0;
+            int vi = 0;
+            if (ln < NF) {
+                float t = L.xf[par][0][ln];
+#pragma unroll
+                for (int ww = 1; ww < NW; ++ww) t += L.xf[par][ww][ln];
+                vf = t;
+            } else if (ln >= 32 && ln < 32 + ND) {
+                double t = L.xd[par][0][ln - 32];
+#pragma unroll
+                for (int ww = 1; ww < NW; ++ww) t += L.xd[par][ww][ln - 32];
+                vd = t;
+            } else if (ln >= 56 && ln < 56 + NI) {
+                int t = L.xi[par][0][ln - 56];
+#pragma unroll
+                for (int ww = 1; ww < NW; ++ww) t += L.xi[par][ww][ln - 56];
+                vi = t;
+            }
+#pragma unroll
+            for (int q = 0; q < NF; ++q) f[q] = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(vf), q));
+#pragma unroll
+            for (int q = 0; q < ND; ++q) d[q] = readlane_f64(vd, 32 + q);
+#pragma unroll
+            for (int q = 0; q < NI; ++q) iv[q] = __builtin_amdgcn_readlane(vi, 56 + q);
+            par ^= 1;
+            return;
+        }
 #pragma unroll
         for (int q = 0; q < NF; ++q) {
             float t = L.xf[par][0][q];

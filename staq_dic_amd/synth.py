@@ -21,7 +21,8 @@ import numpy as np
 __all__ = [
     "speckle_image", "bicubic", "warp_image", "normalize", "AffineField", "KirschField",
     "SinusoidField", "StretchBumpField", "Frame", "make_frame", "plain_grid_nodes",
-    "quadtree_nodes", "config_plain", "config_hole", "config_adjoint",
+    "quadtree_nodes", "quadtree_radial", "IncrementalField", "config_plain", "config_hole",
+    "config_adjoint", "config_large", "config_multi",
 ]
 
 
@@ -181,6 +182,38 @@ class StretchBumpField:
                 -self.amp * g * (Y - self.yc) / self.sig ** 2, z, z)
 
 
+@dataclasses.dataclass
+class IncrementalField:
+    """Displacement between two frames of a sequence, in the earlier frame's
+    coordinates: frame j shows the pattern moved by phi_j(X) = X + u_j(X), so a
+    point x of frame ``a`` sits at X = phi_a^-1(x) in the pattern and at
+    phi_b(X) in frame ``b``; u(x) = phi_b(phi_a^-1(x)) - x and
+    grad u = (I + grad u_b)(I + grad u_a)^-1 - I at X."""
+    a: object
+    b: object
+
+    def _pre(self, X, Y):
+        X = np.asarray(X, dtype=float)
+        Y = np.asarray(Y, dtype=float)
+        return invert_map(self.a, X, Y)
+
+    def __call__(self, X, Y):
+        PX, PY = self._pre(X, Y)
+        ub, vb = self.b(PX, PY)
+        return PX + ub - X, PY + vb - Y
+
+    def jac(self, X, Y):
+        PX, PY = self._pre(X, Y)
+        a11, a12, a21, a22 = self.a.jac(PX, PY)
+        b11, b12, b21, b22 = self.b.jac(PX, PY)
+        A11, A12, A21, A22 = 1 + a11, a12, a21, 1 + a22
+        B11, B12, B21, B22 = 1 + b11, b12, b21, 1 + b22
+        det = A11 * A22 - A12 * A21
+        I11, I12, I21, I22 = A22 / det, -A12 / det, -A21 / det, A11 / det
+        return (B11 * I11 + B12 * I21 - 1, B11 * I12 + B12 * I22,
+                B21 * I11 + B22 * I21, B21 * I12 + B22 * I22 - 1)
+
+
 def invert_map(field, x, y, iters: int = 30):
     """Solve X + u(X) = x per pixel by Newton's method."""
     X = np.array(x, dtype=float)
@@ -330,6 +363,45 @@ def quadtree_nodes(H, W, hole, leaf_max=32, leaf_min=8, band=(450, 150), margin=
     return coords, win
 
 
+def quadtree_radial(H, W, centre, radii, leaves, margin=64, wmin=8, wmax=64):
+    """Quadtree subset list refined towards a point.
+
+    Cells of ``leaves[-1]`` px tile [margin, size - margin]; a cell whose
+    nearest point lies within ``radii[i]`` px of ``centre`` is split down to
+    ``leaves[i]`` (``radii`` increasing, ``leaves`` increasing, one more leaf
+    size than radii).  Nodes are the leaf corners; each node gets window
+    w = clamp(2 * smallest adjacent leaf, wmin, wmax).
+    Returns (coords N x 2, winsize N)."""
+    xc, yc = centre
+    top = leaves[-1]
+    best = {}
+
+    def target(x0, y0, s):
+        nx = min(max(xc, x0), x0 + s)
+        ny = min(max(yc, y0), y0 + s)
+        d = math.hypot(nx - xc, ny - yc)
+        for r, leaf in zip(radii, leaves):
+            if d <= r:
+                return leaf
+        return top
+
+    stack = [(x0, y0, top) for x0 in range(margin, H - margin - top + 1, top)
+             for y0 in range(margin, W - margin - top + 1, top)]
+    while stack:
+        x0, y0, s = stack.pop()
+        if s > target(x0, y0, s):
+            h = s // 2
+            stack.extend([(x0, y0, h), (x0 + h, y0, h), (x0, y0 + h, h), (x0 + h, y0 + h, h)])
+            continue
+        for cx in (x0, x0 + s):
+            for cy in (y0, y0 + s):
+                best[(cx, cy)] = min(best.get((cx, cy), 1 << 30), s)
+    keys = sorted(best.keys(), key=lambda k: (k[1], k[0]))
+    coords = np.array(keys, dtype=float)
+    win = np.array([min(wmax, max(wmin, 2 * best[k])) for k in keys], dtype=np.int64)
+    return coords, win
+
+
 def truth_at(field, coords):
     """Analytic (u, v) and displacement gradient at nodes: P = [u_x v_x u_y v_y u v]."""
     X = coords[:, 0]
@@ -348,9 +420,17 @@ def config_plain(seed=1001, H=512, W=512, n_speckles=3000, winsize=32, step=16):
     return fr, coords, np.full(len(coords), winsize, dtype=np.int64)
 
 
-def config_hole(seed=1002, H=1024, W=1024, n_speckles=12000, max_disp=8.0, radius=150.0, **qt):
+def config_hole(seed=1002, H=1024, W=1024, n_speckles=12000, max_disp=8.0, radius=150.0,
+                window_rule="leaf", winsize=32, **qt):
     """Quadtree with a centred hole of radius ``radius`` px, Kirsch field
-    scaled to ``max_disp`` px, leaves 8-32 px refined at the hole edge."""
+    scaled to ``max_disp`` px, leaves 8-32 px refined at the hole edge.
+
+    ``window_rule`` "leaf": each node's window is tied to its smallest
+    adjacent leaf, clamp(2 leaf, 16, 64).  "reference": every node gets the
+    one window ``winsize`` as the reference's quadtree drivers do
+    (main_ALDIC_Quadtree_v2.m:361, LocalICGNQuadtree.m:85 passes
+    DICpara.winsize); windows then differ only where the IC-GN step grows
+    them at the mask (funICGNQuadtree.m:66-88,139-145)."""
     hole = (H / 2 + 0.5, W / 2 + 0.5, float(radius))
     k = KirschField(hole[0], hole[1], hole[2])
     xx, yy = np.meshgrid(np.arange(1, H + 1, 4.0), np.arange(1, W + 1, 4.0), indexing="ij")
@@ -359,6 +439,10 @@ def config_hole(seed=1002, H=1024, W=1024, n_speckles=12000, max_disp=8.0, radiu
     k.scale = max_disp / float(np.hypot(u, v).max())
     fr = make_frame(H, W, n_speckles, seed, k, hole=hole)
     coords, win = quadtree_nodes(H, W, hole, **qt)
+    if window_rule == "reference":
+        win = np.full(len(coords), winsize, dtype=np.int64)
+    elif window_rule != "leaf":
+        raise ValueError(f"unknown window rule {window_rule!r}")
     return fr, coords, win
 
 
@@ -372,3 +456,61 @@ def config_adjoint(seed=1003, H=2048, W=2048, n_speckles=48000, step=16, winsize
     X, Y = np.meshgrid(xs, xs, indexing="ij")
     coords = np.stack([X.ravel(order="F"), Y.ravel(order="F")], axis=1).astype(float)
     return fr, coords, np.full(len(coords), winsize, dtype=np.int64)
+
+
+def stretch_bump_frame_field(k, H, W, n_frames=40):
+    """Cumulative field of frame k of the large-deformation sequence: affine
+    x-stretch 0.2 k/n_frames about the centre plus a centred Gaussian bump of
+    40 k/n_frames px (sigma 300 px) in u."""
+    return StretchBumpField(H / 2 + 0.5, W / 2 + 0.5, 0.2 * k / n_frames, 40.0 * k / n_frames, 300.0)
+
+
+def config_large(k=40, seed=1004, H=4096, W=4096, n_speckles=192000, bits=16,
+                 radii=(460, 900, 1500, 2100), leaves=(4, 8, 16, 32, 64), margin=64, wmin=16):
+    """Large deformation: 16-bit speckle pattern (``n_speckles`` per 4096^2,
+    seed 1004) shown in frames 0..40 under the cumulative stretch-and-bump
+    field; returns the incremental pair (frame k-1, frame k) as a Frame whose
+    ``field`` is the incremental displacement, and a quadtree refined towards
+    the bump (leaves 4-64 px, windows clamp(2 leaf, wmin, 64))."""
+    f0 = speckle_image(H, W, n_speckles, seed, bits)
+    fa = stretch_bump_frame_field(k - 1, H, W)
+    fb = stretch_bump_frame_field(k, H, W)
+    ga, _ = warp_image(f0, fa, bits)
+    gb, _ = warp_image(f0, fb, bits)
+    del f0
+    mask = np.ones((H, W))
+    fr = Frame(f=_f32(normalize(ga)), g=_f32(normalize(gb)), mask=mask,
+               field=IncrementalField(fa, fb))
+    coords, win = quadtree_radial(H, W, (H / 2 + 0.5, W / 2 + 0.5), radii, leaves, margin=margin,
+                                  wmin=wmin)
+    return fr, coords, win
+
+
+MULTI_SIZES = (512, 1024, 1536, 2048)
+
+
+def config_multi(n_pairs=16, seed0=5000, sizes=MULTI_SIZES, density=3000 / 512 ** 2):
+    """Multi-instance batch: pair j (seed seed0 + j) has a square size drawn
+    from ``sizes``, a random affine field about its centre (translation
+    |t| <= 10 px, each gradient component in [-0.05, 0.05]) and its own
+    quadtree refined towards a random point (leaves 8-32 px, windows
+    16-64 px).  Returns a list of (Frame, coords, winsize)."""
+    out = []
+    for j in range(n_pairs):
+        rng = np.random.Generator(np.random.PCG64(seed0 + j))
+        n = int(rng.choice(np.asarray(sizes)))
+        ang = rng.uniform(0, 2 * np.pi)
+        tr = 10.0 * math.sqrt(rng.uniform())
+        a11, a12, a21, a22 = rng.uniform(-0.05, 0.05, 4)
+        c = n / 2 + 0.5
+        t = (tr * math.cos(ang), tr * math.sin(ang))
+        field = AffineField(t[0] - a11 * c - a12 * c, t[1] - a21 * c - a22 * c, a11, a12, a21, a22)
+        fr = make_frame(n, n, int(round(density * n * n)), seed0 + j, field)
+        # the pattern's largest displacement decides the margin (window 64 + bicubic support)
+        dmax = 10.0 + 0.1 * n / 2 * math.sqrt(2)
+        margin = int(math.ceil(dmax)) + 40
+        p = rng.uniform(0.25 * n, 0.75 * n, 2)
+        coords, win = quadtree_radial(n, n, (p[0], p[1]), (0.2 * n, 0.45 * n), (8, 16, 32),
+                                      margin=margin, wmin=16)
+        out.append((fr, coords, win))
+    return out

@@ -554,7 +554,7 @@ def test_lean_kernel_matches_general(ctx, oracle):
     U0 = np.round(truth[:, 4:6]).reshape(-1)
     ctx.set_frame(fr.f, fr.g, fr.mask)
     b = Batch(ctx, coords, win)
-    clear = [x["bs6"] in (1, 2, 4) for x in b.bins()]
+    clear = [x["bs6"] in (1, 2, 4, 8, 16) for x in b.bins()]
     assert any(clear) and not all(clear), b.bins()
     b.set_input(IN_U0, U0)
     p6 = params(tol=1e-3)
@@ -616,7 +616,7 @@ def test_handed_back_subsets_move_to_the_general_kernel(ctx, plain_cfg):
     p = params(tol=1e-3)
 
     def general_count():
-        return sum(x["count"] for x in b.bins() if x["bs6"] not in (1, 2, 4))
+        return sum(x["count"] for x in b.bins() if x["bs6"] not in (1, 2, 4, 8, 16))
 
     n0 = general_count()
     b.icgn6(p)
@@ -633,3 +633,45 @@ def test_handed_back_subsets_move_to_the_general_kernel(ctx, plain_cfg):
     b.reorder()
     assert general_count() == n1
     b.close()
+
+
+@pytest.mark.parametrize("form", [8, 16, 256])
+def test_wide_forms_on_64px_windows(ctx, oracle, plain_cfg, form):
+    """64-px windows (4225 pixels): the lean kernel with eight and sixteen
+    wavefronts per subset (the default form of such bins) and the workgroup
+    kernel, each against the oracle, 6-DOF then 2-DOF on its result."""
+    from staq_dic_amd import Batch, params, synth
+    from staq_dic_amd._lib import IN_FOLD, IN_U0, IN_UOLD
+    fr, _, _ = plain_cfg
+    H, W = fr.f.shape
+    xs = np.arange(40, 441, 12)  # warped windows stay inside the 512^2 frame
+    X, Y = np.meshgrid(xs, xs, indexing="ij")
+    coords = np.stack([X.ravel(order="F"), Y.ravel(order="F")], axis=1).astype(float)
+    win = np.full(len(coords), 64, dtype=np.int64)
+    truth = synth.truth_at(fr.field, coords)
+    U0 = np.round(truth[:, 4:6]).reshape(-1)
+    ctx.set_frame(fr.f, fr.g, fr.mask)
+    _print("64-px windows, form", form, "N =", len(coords))
+    b = Batch(ctx, coords, win)
+    assert len(b.bins()) == 1 and b.bins()[0]["bs6"] == 16, b.bins()
+    b.set_forms([form], 6)
+    b.set_forms([form], 2)
+    b.set_input(IN_U0, U0)
+    b.icgn6(params(tol=1e-3))
+    Ug, Fg, _, sg = b.outputs()
+    b.reorder()
+    # the lean forms kept every subset: nothing was handed to the fallback
+    assert len(b.bins()) == 1, b.bins()
+    of = oracle.Frame(fr.f, fr.g, fr.mask)
+    Uo, Fo, _, so = oracle.local_icgn(of, U0, coords, win, tol=1e-3)
+    assert _cmp_icgn(sg, so, Ug, Uo, Fg, Fo) > 0.95 * len(coords)
+    UOld = np.where(np.isfinite(Ug), Ug, U0)
+    FOld = np.where(np.isfinite(Fg), Fg, 0.0)
+    b.set_input(IN_UOLD, UOld)
+    b.set_input(IN_FOLD, FOld)
+    b.icgn2(params(tol=1e-3, mu=1e-3))
+    Ug2, _, _, sg2 = b.outputs()
+    b.close()
+    Uo2, so2 = oracle.subpb1(of, UOld, FOld, np.zeros_like(UOld), coords, win, win, 1e-3,
+                             tol=1e-3)
+    _cmp_icgn(sg2, so2, Ug2, Uo2)

@@ -209,7 +209,7 @@ def multi_pair_leg(ctx, tm, steps, warmup):
     win = np.concatenate([w for _, _, w in pairs])
     slot = np.concatenate([np.full(len(c), j, dtype=np.int32) for j, (_, c, _) in enumerate(pairs)])
     truth = np.concatenate([synth.truth_at(fr.field, c) for fr, c, _ in pairs])
-    agg, b = local_step(ctx, tm, coords, win, truth, steps, warmup, slot=slot, tune=False)
+    agg, b = local_step(ctx, tm, coords, win, truth, steps, warmup, slot=slot)
     b.close()
     p6 = params(tol=1e-3)
     lat = []
@@ -241,7 +241,7 @@ def large_leg(ctx, tm, steps, warmup):
     16-bit sequence): the integer search (w 64, r 48) and the 6-DOF local step
     over ~100 k quadtree subsets."""
     from staq_dic_amd import synth
-    from staq_dic_amd.api import funIntegerSearch
+    from staq_dic_amd.api import integer_search_slot
     t0 = time.time()
     fr, coords, win = synth.config_large()
     truth = synth.truth_at(fr.field, coords)
@@ -249,19 +249,22 @@ def large_leg(ctx, tm, steps, warmup):
         f"{time.time() - t0:.1f}s")
     H, W = fr.f.shape
     ctx.set_frame(fr.f, fr.g, fr.mask, slot=0)
-    import torch
-    torch.cuda.synchronize()
+    # the search on the resident pair (its outputs come back to the host)
+    x, y, u, v, cc = integer_search_slot(ctx, 0, 48, [1, H], [1, W], 64, 32)
     t = time.perf_counter()
-    x, y, u, v, cc = funIntegerSearch(fr.f, fr.g, 48, [1, H], [1, W], 64, 32, ctx=ctx, slot=0)
+    x, y, u, v, cc = integer_search_slot(ctx, 0, 48, [1, H], [1, W], 64, 32)
     dt = time.perf_counter() - t
     tr = synth.truth_at(fr.field, np.stack([x.ravel(order="F"), y.ravel(order="F")], axis=1))
     ok = np.isfinite(u.ravel(order="F"))
     du = np.abs(u.ravel(order="F")[ok] - tr[ok, 4])
     dv = np.abs(v.ravel(order="F")[ok] - tr[ok, 5])
-    search = {"points": int(x.size), "seconds_host_to_host": dt, "points_per_s": x.size / dt,
+    m, k = 65.0, 99.0
+    search = {"points": int(x.size), "seconds": dt, "points_per_s": x.size / dt,
+              "fp64_tflops": x.size * k * k * m * m * 2 / dt / 1e12,
+              "fp64_peak_tflops": 78.6,
               "within_1px": float(np.mean((du <= 1) & (dv <= 1)))}
     log(f"[bench] large integer search: {x.size} points in {dt * 1e3:.1f} ms")
-    res, b = local_step(ctx, tm, coords, win, truth, steps, warmup, tune=False)
+    res, b = local_step(ctx, tm, coords, win, truth, steps, warmup)
     b.close()
     res.update({"workload": "4096x4096 16-bit speckle (192 000 speckles, seed 1004), frames 39 "
                             "-> 40 of the stretch (0.2 k/40) + bump (40 k/40 px, sigma 300) "
@@ -296,7 +299,6 @@ def main():
 
     from staq_dic_amd import Context, params, synth
     from staq_dic_amd._lib import IN_U0
-    from staq_dic_amd.api import funIntegerSearch
 
     if args.copy_calib:
         src = torch.ones(64 << 20, dtype=torch.float32, device="cuda")
@@ -375,12 +377,13 @@ def main():
                                 "batched_vs_plain": value / (cp / (floor_ms / 1e3))}
         log(f"[bench] per-subset form floor {floor_ms:.1f} ms")
 
-        # integer search on the same pair (w 32, step 16, r 10)
+        # integer search on the same pair (w 32, step 16, r 10), frames resident
+        from staq_dic_amd.api import integer_search_slot
+        x, y, u, v, cc = integer_search_slot(ctx, 0, 10, [1, 1024], [1, 1024], 32, 16)
         t = time.perf_counter()
-        x, y, u, v, cc = funIntegerSearch(fr.f, fr.g, 10, [1, 1024], [1, 1024], 32, 16, ctx=ctx,
-                                          slot=1)
+        x, y, u, v, cc = integer_search_slot(ctx, 0, 10, [1, 1024], [1, 1024], 32, 16)
         dt = time.perf_counter() - t
-        result["integer_search"] = {"points": int(x.size), "seconds_host_to_host": dt,
+        result["integer_search"] = {"points": int(x.size), "seconds": dt,
                                     "points_per_s": x.size / dt}
 
         # adjoint ZNSSD (forward + vector-Jacobian product through the bicubic

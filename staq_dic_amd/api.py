@@ -107,11 +107,18 @@ def funIntegerSearch(f, g, tempSizeOfSearchRegion, gridx, gridy, winsize, winste
     if tempNoOfInitPt != 0:
         raise NotImplementedError("seed-point mode needs interactive input (funIntegerSearch.m:37-74)")
     c = _c(ctx)
+    c.set_frame(f, g, None, slot=slot)
+    return integer_search_slot(c, slot, tempSizeOfSearchRegion, gridx, gridy, winsize,
+                               winstepsize)
+
+
+def integer_search_slot(c, slot, tempSizeOfSearchRegion, gridx, gridy, winsize, winstepsize):
+    """funIntegerSearch on the frame pair already resident in context slot
+    ``slot`` (f = its reference image, g = its deformed image)."""
     r = np.broadcast_to(np.asarray(tempSizeOfSearchRegion, dtype=np.int64), (2,))
     w = int(np.asarray(winsize).ravel()[0])
     step = int(np.asarray(winstepsize).ravel()[0])
-    c.set_frame(f, g, None, slot=slot)
-    H, W = np.shape(f)
+    H, W = c.frames[slot]
     mx = (H // step + 2) * (W // step + 2)
     gx = np.asarray([int(gridx[0]), int(gridx[-1])], dtype=np.int64)
     gy = np.asarray([int(gridy[0]), int(gridy[-1])], dtype=np.int64)

@@ -431,7 +431,8 @@ int staq_integer_search(staq_ctx *ctx, int slot, long r1, long r2, const long gr
     if (slot < 0 || (size_t)slot >= ctx->frames.size() || !ctx->frames[slot].set)
         return fail(ctx, STAQ_ENOFRAME, "frame slot not set");
     if (r1 < 0 || r2 < 0 || w < 2 || step < 1) return fail(ctx, STAQ_EINVAL, "bad search geometry");
-    if (search_lds_bytes((int)w, (int)r1, (int)r2) > 150 * 1024)
+    if (!search_uses_strip((int)w, (int)r1, (int)r2) &&
+        search_lds_bytes((int)w, (int)r1, (int)r2) > 150 * 1024)
         return fail(ctx, STAQ_EINVAL, "template plus search block exceed LDS");
     hipSetDevice(ctx->device);
     const Frame &fr = ctx->frames[slot];
@@ -450,9 +451,13 @@ int staq_integer_search(staq_ctx *ctx, int slot, long r1, long r2, const long gr
     const long np = nx * ny;
     if (np == 0) return STAQ_OK;
     const long kk = (2 * r1 + 3) * (2 * r2 + 3);
-    const long chunk = std::max(1L, std::min(np, (256L << 20) / (kk * 16)));
+    // per point: the (num, den) map and, for the strip kernel, the column box sums
+    const long kcs = search_uses_strip((int)w, (int)r1, (int)r2)
+                         ? (2 * r1 + 3) * (w + 2 * r2 + 3) : 0;
+    const long chunk = std::max(1L, std::min(np, (512L << 20) / ((kk + kcs) * 16)));
     hipStream_t s = ctx->stream;
     HIPCHK(ctx, ctx->s[0].ensure((size_t)chunk * kk * 16));
+    HIPCHK(ctx, ctx->s[6].ensure((size_t)chunk * std::max(kcs, 1L) * 16));
     for (int k = 0; k < 5; ++k) HIPCHK(ctx, ctx->s[1 + k].ensure((size_t)np * 8));
     SearchArgs a;
     a.f = fr.f.as<float>();
@@ -468,6 +473,7 @@ int staq_integer_search(staq_ctx *ctx, int slot, long r1, long r2, const long gr
     a.nx = (int)nx;
     a.ny = (int)ny;
     a.scratch = ctx->s[0].as<double2>();
+    a.colsum = ctx->s[6].as<double2>();
     a.x0 = ctx->s[1].as<double>();
     a.y0 = ctx->s[2].as<double>();
     a.u = ctx->s[3].as<double>();
@@ -594,8 +600,8 @@ static int build_bins(staq_batch *b)
             form = 2 * g;
         else if (bin.ppt)
             form = bin.masked ? 0 : 1;
-        else if (!bin.masked && icgn_lean_ppt(bin.wave_npix, 16))
-            form = 16;
+        else if (!bin.masked && icgn_lean_ppt(bin.wave_npix, 8))
+            form = 8;  // sixteen wavefronts spill registers (1024 threads cap them at 128)
         bin.bs[0] = bin.bs[1] = form;
         b->bins.push_back(bin);
         fbx = std::max(fbx, grown_window(bin.wxcap));

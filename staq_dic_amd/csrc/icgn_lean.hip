@@ -36,6 +36,9 @@ struct LeanLds {
     float xf[2][NW][24];
     double xd[2][NW][16];
     int xi[2][NW][8];
+    // eight or more wavefronts: the Gauss-Newton step of wavefront 0
+    double Pn[6], rel2, abs2, inv_init2;
+    int stop, init_ok;
 };
 
 template <int PPT, int MODE, int NW>
@@ -436,6 +439,80 @@ __global__ __launch_bounds__(64 * NW) void icgn_lean_kernel(IcgnArgs a)
                 state = 3;  // degenerate statistics: the reference's FP64 residual
                 break;
             }
+            // steepest-descent vector, norm test, Gauss-Newton step and
+            // inverse-compositional update (the same operations as below);
+            // false: the loop ends (state 1)
+            auto gn_step = [&]() -> bool {
+                double b[6] = {0, 0, 0, 0, 0, 0};
+                {
+                    const double ratio = bf * __builtin_amdgcn_rsq(var);
+#pragma unroll
+                    for (int q = 0; q < NJ; ++q)
+                        b[q] = L.FJ[q] - ratio * ((double)acc[2 + q] - dg * L.SJ[q]);
+                }
+                double nb2 = 0.0;
+                if (MODE == 6) {
+#pragma unroll
+                    for (int q = 0; q < 6; ++q) nb2 += b[q] * b[q];
+                } else {
+                    // :379 tempb = 2 b / bottomf^2 + mu (P - UOld - vdual)
+                    const double s2 = 2.0 * __builtin_amdgcn_rcp(bottomf * bottomf);
+                    b[0] = b[0] * s2 + a.mu * ((P[4] - uo_u) - vd_u);
+                    b[1] = b[1] * s2 + a.mu * ((P[5] - uo_v) - vd_v);
+                    nb2 = b[0] * b[0] + b[1] * b[1];
+                }
+                abs2 = nb2;
+                if (it == 1) {
+                    init_ok = nb2 > tol2;  // normOfWNewInit > tol
+                    inv_init2 = __builtin_amdgcn_rcp(nb2);
+                }
+                rel2 = init_ok ? nb2 * inv_init2 : 0.0;
+                if (rel2 < tol2 || abs2 < tol2) return false;
+                double d[6] = {0, 0, 0, 0, 0, 0};
+                if (MODE == 6) {
+                    solve_chol6(L.Lp, L.Ld, b, d);
+                } else {
+                    d[4] = -(L.H2i[0] * b[0] + L.H2i[2] * b[1]);
+                    d[5] = -(L.H2i[1] * b[0] + L.H2i[3] * b[1]);
+                }
+                return ic_compose(P, d);
+            };
+            if constexpr (NW >= 8) {
+                bool go = false;
+                // with eight or more wavefronts on one CU the redundant FP64
+                // step would queue behind itself on every SIMD: wavefront 0
+                // takes it and hands the new state to the others
+                if (wv == 0) {
+                    go = gn_step();
+                    if (lane == 0) {
+#pragma unroll
+                        for (int q = 0; q < 6; ++q) L.Pn[q] = P[q];
+                        L.rel2 = rel2;
+                        L.abs2 = abs2;
+                        L.inv_init2 = inv_init2;
+                        L.init_ok = init_ok;
+                        L.stop = !go;
+                    }
+                }
+                __syncthreads();
+                if (wv != 0) {
+#pragma unroll
+                    for (int q = 0; q < 6; ++q) P[q] = L.Pn[q];
+                    rel2 = L.rel2;
+                    abs2 = L.abs2;
+                    inv_init2 = L.inv_init2;
+                    init_ok = L.init_ok;
+                    go = !L.stop;
+                }
+                go = __builtin_amdgcn_readfirstlane((int)go);
+                if (!go) {
+                    state = 1;
+                    break;
+                }
+                continue;
+            }
+            // up to four wavefronts (one per SIMD): every wavefront takes the
+            // same step, with no barrier on the iteration's critical path
             double b[6] = {0, 0, 0, 0, 0, 0};
             {
                 const double ratio = bf * __builtin_amdgcn_rsq(var);

@@ -52,6 +52,7 @@ struct SearchArgs {
     int r1, r2, w;
     int gx0, gy0, step, nx, ny;  // grid: ii = gx0 + ix*step, jj = gy0 + iy*step
     double2 *scratch;            // (num, den) per kept shift, per point of this launch
+    double2 *colsum;             // strip kernel: column box sums, k1 x (w+2r2+3) per point
     long point0;                 // first point of this launch
     double *x0, *y0, *u, *v, *ccmax;  // ny x nx column-major outputs
 };
@@ -77,5 +78,8 @@ hipError_t mask_class_launch(const FrameDev *frames, const int *slot, const doub
                              const int *wx, const int *wy, long N, int *out, hipStream_t s);
 hipError_t search_launch(const SearchArgs &a, long npoints, hipStream_t s);
 size_t search_lds_bytes(int w, int r1, int r2);
+// the register-blocked strip kernel takes geometries whose padded blocks fit its LDS budget
+size_t search_strip_lds_bytes(int w, int r1, int r2);
+bool search_uses_strip(int w, int r1, int r2);
 
 }  // namespace staq

@@ -637,8 +637,8 @@ def test_handed_back_subsets_move_to_the_general_kernel(ctx, plain_cfg):
 
 @pytest.mark.parametrize("form", [8, 16, 256])
 def test_wide_forms_on_64px_windows(ctx, oracle, plain_cfg, form):
-    """64-px windows (4225 pixels): the lean kernel with eight and sixteen
-    wavefronts per subset (the default form of such bins) and the workgroup
+    """64-px windows (4225 pixels): the lean kernel with eight (the default
+    form of such bins) and sixteen wavefronts per subset and the workgroup
     kernel, each against the oracle, 6-DOF then 2-DOF on its result."""
     from staq_dic_amd import Batch, params, synth
     from staq_dic_amd._lib import IN_FOLD, IN_U0, IN_UOLD
@@ -653,7 +653,7 @@ def test_wide_forms_on_64px_windows(ctx, oracle, plain_cfg, form):
     ctx.set_frame(fr.f, fr.g, fr.mask)
     _print("64-px windows, form", form, "N =", len(coords))
     b = Batch(ctx, coords, win)
-    assert len(b.bins()) == 1 and b.bins()[0]["bs6"] == 16, b.bins()
+    assert len(b.bins()) == 1 and b.bins()[0]["bs6"] == 8, b.bins()
     b.set_forms([form], 6)
     b.set_forms([form], 2)
     b.set_input(IN_U0, U0)

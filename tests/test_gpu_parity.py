@@ -675,3 +675,82 @@ def test_wide_forms_on_64px_windows(ctx, oracle, plain_cfg, form):
     Uo2, so2 = oracle.subpb1(of, UOld, FOld, np.zeros_like(UOld), coords, win, win, 1e-3,
                              tol=1e-3)
     _cmp_icgn(sg2, so2, Ug2, Uo2)
+
+
+def test_large_deformation_incremental_pair(ctx, oracle):
+    """The large-deformation configuration at 768^2: a 16-bit pattern shown
+    in frames 39 and 40 of the stretch-and-bump sequence, the incremental pair
+    on a quadtree refined towards the bump (leaves 4-64 px, windows 16-64 px),
+    all in one batch: 6-DOF step against the oracle subset by subset, the
+    incremental displacement against the analytic field, and the integer
+    search (window 64, radius 48) against the oracle point by point."""
+    from staq_dic_amd import Batch, params, synth
+    from staq_dic_amd.api import integer_search_slot
+    from staq_dic_amd._lib import IN_U0
+    fr, coords, win = synth.config_large(H=768, W=768, n_speckles=6750, radii=(90, 170, 280, 400),
+                                         margin=48)
+    assert set(np.unique(win)) == {16, 32, 64}, np.unique(win)
+    truth = synth.truth_at(fr.field, coords)
+    U0 = np.round(truth[:, 4:6]).reshape(-1)
+    ctx.set_frame(fr.f, fr.g, fr.mask, slot=0)
+    _print("large-deformation pair 768^2, N =", len(coords))
+    b = Batch(ctx, coords, win)
+    b.set_input(IN_U0, U0)
+    b.icgn6(params(tol=1e-3))
+    Ug, Fg, _, sg = b.outputs()
+    b.close()
+    of = oracle.Frame(fr.f, fr.g, fr.mask)
+    Uo, Fo, _, so = oracle.local_icgn(of, U0, coords, win, tol=1e-3)
+    assert _cmp_icgn(sg, so, Ug, Uo, Fg, Fo) > 0.9 * len(coords)
+    ok = (sg >= 1) & (sg <= 100) & (win >= 32)
+    err = np.hypot(Ug[0::2] - truth[:, 4], Ug[1::2] - truth[:, 5])[ok]
+    assert np.sqrt(np.mean(err ** 2)) < 0.01
+    H, W = fr.f.shape
+    x, y, u, v, cc = integer_search_slot(ctx, 0, 48, [1, H], [1, W], 64, 64)
+    xo, yo, uo, vo, co = oracle.integer_search(fr.f, fr.g, 48, [1, H], [1, W], 64, 64)
+    np.testing.assert_array_equal(x, xo)
+    np.testing.assert_array_equal(u, uo)
+    np.testing.assert_array_equal(v, vo)
+
+
+def test_znssd_sinusoid_field(ctx, oracle):
+    """The adjoint configuration's field (u = 4 sin(2 pi x/512), v = 3 cos(2 pi
+    y/768)) at 768^2 on its 16-px node grid: forward cost and vector-Jacobian
+    product against the FP64 oracle, and the gradient against central finite
+    differences of the oracle's cost (h = 1e-3 px) on u and v."""
+    from staq_dic_amd import synth
+    from staq_dic_amd.autograd import znssd
+    import torch
+    fr, coords, win = synth.config_adjoint(H=768, W=768, n_speckles=6750)
+    truth = synth.truth_at(fr.field, coords)
+    rng = np.random.default_rng(7)
+    P0 = truth + rng.normal(0, [0.002, 0.002, 0.002, 0.002, 0.1, 0.1], truth.shape)
+    ctx.set_frame(fr.f, fr.g, fr.mask, slot=0)
+    _print("znssd sinusoid 768^2, N =", len(coords))
+    dev = torch.device("cuda", 0)
+    P = torch.tensor(P0, dtype=torch.float64, device=dev, requires_grad=True)
+    cost, status = znssd(P, torch.tensor(coords, device=dev),
+                         torch.tensor(win, dtype=torch.int32, device=dev), ctx)
+    cost.sum().backward()
+    torch.cuda.synchronize()
+    of = oracle.Frame(fr.f, fr.g, fr.mask)
+    tot, co, go, so = oracle.znssd(of, P0, coords, win)
+    st = status.cpu().numpy()
+    np.testing.assert_array_equal(st, so)
+    ok = so == 0
+    assert ok.mean() > 0.95
+    np.testing.assert_allclose(cost.detach().cpu().numpy()[ok], co[ok], rtol=1e-4, atol=1e-6)
+    g = P.grad.cpu().numpy()
+    scale = np.abs(go[ok]).max(axis=0) + 1e-12
+    assert np.max(np.abs(g[ok] - go[ok]) / scale) < 2e-3
+    # central differences of the FP64 cost on the translation parameters
+    h = 1e-3
+    sel = np.flatnonzero(ok)[:: max(1, ok.sum() // 40)]
+    for q in (4, 5):
+        Pp, Pm = P0[sel].copy(), P0[sel].copy()
+        Pp[:, q] += h
+        Pm[:, q] -= h
+        _, cp, _, _ = oracle.znssd(of, Pp, coords[sel], win[sel], want_grad=False)
+        _, cm, _, _ = oracle.znssd(of, Pm, coords[sel], win[sel], want_grad=False)
+        fd = (cp - cm) / (2 * h)
+        np.testing.assert_allclose(g[sel, q], fd, rtol=0.02, atol=2e-3 * np.abs(fd).max())

@@ -109,6 +109,25 @@ int staq_integer_search(staq_ctx *ctx, int slot, long r1, long r2, const long gr
                         long *nx, long *ny, double *x0, double *y0, double *u, double *v,
                         double *ccmax);
 
+/* The two correlations of the multigrid integer search funIntegerSearchMg
+ * (func/funIntegerSearchMg.m:3-285, IntegerSearch.m:122-125 with
+ * InitFFTSearchMethod = 0), on frame `slot`; boxes are [x0 x1 y0 y1], 1-based,
+ * inclusive.  The level bookkeeping and the final gridfit are the caller's.
+ *
+ * staq_xcorr_full_peak: [xpeak, ypeak, max_f] = findpeak(normxcorr2(C, D), 1)
+ * with C = f(box_f), D = g(box_g), the whole normxcorr2 map (:17-27); NaN
+ * for a constant template (the reference's error). */
+int staq_xcorr_full_peak(staq_ctx *ctx, int slot, const long box_f[4], const long box_g[4],
+                         double out[3]);
+/* staq_xcorr_boxes: for each of nbox boxes (4 longs each), C = f(box_f),
+ * D = g(box_g +- pad), normxcorr2(C, D) cropped as :165 (2 pad + 3 rows,
+ * 2 pad + 4 columns of placements), findpeak: out = [xpeak ypeak max_f] per box
+ * (3 doubles), ok = 1, or ok = 0 and NaN where the reference's try fails
+ * (D outside the image, template and g box of different sizes, constant
+ * template). */
+int staq_xcorr_boxes(staq_ctx *ctx, int slot, long nbox, const long *box_f, const long *box_g,
+                     long pad, double *out, int *ok);
+
 /* Raw subset outputs of LocalICGNQuadtree (func_quadtree/LocalICGNQuadtree.m:63-147)
  * or, with params->plain, LocalICGN (func/LocalICGN.m:60-115), before the
  * bad-subset classification: U (2N), F (4N), Hpar (N x 21), status (N).

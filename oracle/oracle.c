@@ -272,6 +272,53 @@ void orc_normxcorr2_kept(const double *C, long m1, long m2, const double *D, lon
     free(den);
 }
 
+/* normxcorr2(C, D), the whole map ((m1+d1-1) x (m2+d2-1), column-major):     */
+/* every placement of the template over the zero-padded image, by direct    */
+/* FP64 sums over the overlap (the public definition; no FFT, no running     */
+/* sums).  Map entry (k, l), 0-based, puts the template's top-left at D(k -  */
+/* m1 + 1, l - m2 + 1).  Used by funIntegerSearchMg.m:19,170.                */
+void orc_normxcorr2_full(const double *C, long m1, long m2, const double *D, long d1, long d2,
+                         double *out)
+{
+    const long k1 = m1 + d1 - 1, k2 = m2 + d2 - 1;
+    const double mn = (double)(m1 * m2);
+    double sT = 0.0;
+    for (long i = 0; i < m1 * m2; ++i) sT += C[i];
+    const double mT = sT / mn;
+    double ssT = 0.0;
+    for (long i = 0; i < m1 * m2; ++i) ssT += (C[i] - mT) * (C[i] - mT);
+    const double denT = sqrt(ssT);
+    double *den = (double *)malloc(sizeof(double) * (size_t)(k1 * k2));
+    double dmax = 0.0;
+    for (long l = 0; l < k2; ++l)
+        for (long k = 0; k < k1; ++k) {
+            const long s1 = k - m1 + 1, s2 = l - m2 + 1;
+            double sTA = 0.0, sA = 0.0, sAA = 0.0;
+            for (long j = 0; j < m2; ++j) {
+                const long dj = s2 + j;
+                if (dj < 0 || dj >= d2) continue;
+                for (long i = 0; i < m1; ++i) {
+                    const long di = s1 + i;
+                    if (di < 0 || di >= d1) continue;
+                    const double v = D[di + dj * d1];
+                    sTA += C[i + j * m1] * v;
+                    sA += v;
+                    sAA += v * v;
+                }
+            }
+            const double num = sTA - sA * sT / mn;
+            double dA = sAA - sA * sA / mn;
+            if (dA < 0) dA = 0;
+            const double dn = denT * sqrt(dA);
+            out[k + l * k1] = num;
+            den[k + l * k1] = dn;
+            if (dn > dmax) dmax = dn;
+        }
+    const double tol = sqrt(nextafter(dmax, INFINITY) - dmax);
+    for (long i = 0; i < k1 * k2; ++i) out[i] = den[i] > tol ? out[i] / den[i] : 0.0;
+    free(den);
+}
+
 /* ------------------------------------------------------------------------- */
 /* funIntegerSearch.m:14-27 (ROI shrink) and :82-203 (whole-field loop).      */
 /* ------------------------------------------------------------------------- */

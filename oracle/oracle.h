@@ -54,6 +54,8 @@ void orc_findpeak(const double *f, long nr, long nc, double *xpeak, double *ypea
 /* normxcorr2(C, D) restricted to the block funIntegerSearch.m:134 keeps:
  * shifts s1, s2 in [-1, 2r+1], output (2r+3) x (2r+3) column-major.
  * C is m1 x m2, D is (m1+2r) x (m2+2r). */
+void orc_normxcorr2_full(const double *C, long m1, long m2, const double *D, long d1, long d2,
+                         double *out);
 void orc_normxcorr2_kept(const double *C, long m1, long m2, const double *D, long r1,
                          long r2, double *out);
 

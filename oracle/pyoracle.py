@@ -47,6 +47,7 @@ def lib():
                                    C.c_int, C.POINTER(C.c_ubyte)]
         L.orc_findpeak.argtypes = [_dp, C.c_long, C.c_long, _dp, _dp, _dp]
         L.orc_normxcorr2_kept.argtypes = [_dp, C.c_long, C.c_long, _dp, C.c_long, C.c_long, _dp]
+        L.orc_normxcorr2_full.argtypes = [_dp, C.c_long, C.c_long, _dp, C.c_long, C.c_long, _dp]
         L.orc_integer_search.argtypes = [_dp, _dp, C.c_long, C.c_long, C.c_long, C.c_long, _lp, _lp,
                                          C.c_long, C.c_long, C.c_long, _lp, _lp, _dp, _dp, _dp,
                                          _dp, _dp]
@@ -131,6 +132,159 @@ def normxcorr2_kept(Cm, D, r1, r2):
     out = np.zeros((2 * r1 + 3, 2 * r2 + 3), order="F")
     lib().orc_normxcorr2_kept(_d(Cm), Cm.shape[0], Cm.shape[1], _d(D), r1, r2, _d(out))
     return out
+
+
+def normxcorr2_full(Cm, D):
+    """normxcorr2(C, D): the whole (m1+d1-1) x (m2+d2-1) map, direct FP64."""
+    Cm = _F(Cm)
+    D = _F(D)
+    out = np.zeros((Cm.shape[0] + D.shape[0] - 1, Cm.shape[1] + D.shape[1] - 1), order="F")
+    lib().orc_normxcorr2_full(_d(Cm), Cm.shape[0], Cm.shape[1], _d(D), D.shape[0], D.shape[1],
+                              _d(out))
+    return out
+
+
+def integer_search_mg(f, g, gridx, gridy, winsize, winstepsize):
+    """funIntegerSearchMg (func/funIntegerSearchMg.m:3-285) with direct FP64
+    normxcorr2 maps: returns the scattered box centres and shifts before the
+    gridfit (tempx, tempy, tempu, tempv, tempPhi) and the final grid limits
+    (gridx, gridy after :256-261).  Small images only (the level-1 map is
+    computed by direct sums)."""
+    f = np.asarray(f, dtype=np.float64)
+    g = np.asarray(g, dtype=np.float64)
+    H, W = f.shape
+    gx = [int(gridx[0]), int(gridx[-1])]
+    gy = [int(gridy[0]), int(gridy[-1])]
+    bgap = 1 + int(np.floor(0.75 * winsize + 0.5))  # :6 (MATLAB round: half away from zero)
+    gx[0] = max(gx[0], bgap)
+    gy[0] = max(gy[0], bgap)
+    gx[1] = min(gx[1], H - bgap)
+    gy[1] = min(gy[1], W - bgap)
+
+    def fpk(X):
+        xp, yp, mf = findpeak(X)
+        return xp, yp, mf
+
+    # level 1 (:17-27)
+    Cm = f[gx[0] - 1:gx[1], gy[0] - 1:gy[1]]
+    Dm = g[gx[0] - 1:gx[1], gy[0] - 1:gy[1]]
+    X = normxcorr2_full(Cm, Dm)
+    v1, u1, _ = fpk(X)
+    u = u1 - np.ceil((X.shape[0] + 1) / 2.0)
+    v = v1 - np.ceil((X.shape[1] + 1) / 2.0)
+
+    def clamp_boxes(xf, yf, xg, yg):
+        # :30-37 / :221-245: move g boxes inside the border gap, f boxes alike
+        for b in range(len(xf)):
+            if xg[b][0] < bgap:
+                t = xg[b][0]; xg[b][0] = bgap; xf[b][0] = xf[b][0] + bgap - t
+            if yg[b][0] < bgap:
+                t = yg[b][0]; yg[b][0] = bgap; yf[b][0] = yf[b][0] + bgap - t
+            if xg[b][1] > H - bgap:
+                t = xg[b][1]; xg[b][1] = H - bgap; xf[b][1] = xf[b][1] + (H - bgap) - t
+            if yg[b][1] > W - bgap:
+                t = yg[b][1]; yg[b][1] = W - bgap; yf[b][1] = yf[b][1] + (W - bgap) - t
+        for b in range(len(xf)):  # :40-41 / :247-254: even widths
+            if (xf[b][1] - xf[b][0]) % 2 == 1:
+                xf[b][1] -= 1; xg[b][1] -= 1
+            if (yf[b][1] - yf[b][0]) % 2 == 1:
+                yf[b][1] -= 1; yg[b][1] -= 1
+
+    xf = [list(map(float, gx))]
+    yf = [list(map(float, gy))]
+    xg = [[gx[0] + np.ceil(u), gx[1] + np.ceil(u)]]
+    yg = [[gy[0] + np.ceil(v), gy[1] + np.ceil(v)]]
+    clamp_boxes(xf, yf, xg, yg)
+    wx = xf[0][1] - xf[0][0]
+    wy = yf[0][1] - yf[0][0]
+    uc = [np.ceil(u)]
+    vc = [np.ceil(v)]
+    while True:
+        ratio = wx / wy
+        nxf, nyf, nxg, nyg, un, vn = [], [], [], [], [], []
+        for b in range(len(xf)):
+            mx = 0.5 * (xf[b][0] + xf[b][1]); my = 0.5 * (yf[b][0] + yf[b][1])
+            mgx = 0.5 * (xg[b][0] + xg[b][1]); mgy = 0.5 * (yg[b][0] + yg[b][1])
+            if ratio > 2:
+                if xf[b][1] - xf[b][0] > winsize:
+                    parts = [([xf[b][0], mx], yf[b], [xg[b][0], mgx], yg[b]),
+                             ([mx, xf[b][1]], yf[b], [mgx, xg[b][1]], yg[b])]
+                else:
+                    parts = [(xf[b], yf[b], xg[b], yg[b])] * 2
+            elif ratio < 0.5:
+                if yf[b][1] - yf[b][0] > winsize:
+                    parts = [(xf[b], [yf[b][0], my], xg[b], [yg[b][0], mgy]),
+                             (xf[b], [my, yf[b][1]], xg[b], [mgy, yg[b][1]])]
+                else:
+                    parts = [(xf[b], yf[b], xg[b], yg[b])] * 2
+            else:
+                if xf[b][1] - xf[b][0] > winsize and yf[b][1] - yf[b][0] > winsize:
+                    lo_x, hi_x = [xf[b][0], mx], [mx, xf[b][1]]
+                    lo_gx, hi_gx = [xg[b][0], mgx], [mgx, xg[b][1]]
+                    lo_y, hi_y = [yf[b][0], my], [my, yf[b][1]]
+                    lo_gy, hi_gy = [yg[b][0], mgy], [mgy, yg[b][1]]
+                    parts = [(lo_x, lo_y, lo_gx, lo_gy), (hi_x, lo_y, hi_gx, lo_gy),
+                             (lo_x, hi_y, lo_gx, hi_gy), (hi_x, hi_y, hi_gx, hi_gy)]
+                else:
+                    parts = [(xf[b], yf[b], xg[b], yg[b])] * 4
+            for (a1, a2, a3, a4) in parts:
+                nxf.append(list(a1)); nyf.append(list(a2)); nxg.append(list(a3)); nyg.append(list(a4))
+                un.append(uc[b]); vn.append(vc[b])
+        if ratio > 2:
+            nwx, nwy = wx / 2, wy
+        elif ratio < 0.5:
+            nwx, nwy = wx, wy / 2
+        else:
+            nwx, nwy = wx / 2, wy / 2
+        phi = np.zeros(len(nxf))
+        for b in range(len(nxf)):  # :151-170
+            try:
+                vals = nxf[b] + nyf[b] + nxg[b] + nyg[b]
+                if not all(np.isfinite(vals)):
+                    raise ValueError
+                a0, a1_ = int(nxf[b][0]), int(nxf[b][1])
+                b0, b1 = int(nyf[b][0]), int(nyf[b][1])
+                c0, c1 = int(nxg[b][0]) - 5, int(nxg[b][1]) + 5
+                e0, e1 = int(nyg[b][0]) - 5, int(nyg[b][1]) + 5
+                if min(a0, b0, c0, e0) < 1 or a1_ > H or c1 > H or b1 > W or e1 > W:
+                    raise ValueError
+                Cb = f[a0 - 1:a1_, b0 - 1:b1]
+                Db = g[c0 - 1:c1, e0 - 1:e1]
+                if Cb.shape[0] > Db.shape[0] or Cb.shape[1] > Db.shape[1] or Cb.max() == Cb.min():
+                    raise ValueError
+                Xb = normxcorr2_full(Cb, Db)
+                w1, w2 = Cb.shape[0] - 1, Cb.shape[1] - 1
+                crop = Xb[w1 - 1:Xb.shape[0] - w1 + 1, w2 - 1:Xb.shape[1] - w2 + 2]
+                zshape = (Xb.shape[0] - 2 * w1 + 2, Xb.shape[1] - 2 * w2 + 2)
+                xp, yp, mf = fpk(crop)
+                un[b] = un[b] + (yp - np.ceil(zshape[0] / 2.0))
+                vn[b] = vn[b] + (xp - np.ceil(zshape[1] / 2.0))
+                phi[b] = mf
+            except ValueError:
+                un[b] = np.nan; vn[b] = np.nan; phi[b] = 0.0
+        if nwx < winsize and nwy < winsize:
+            break
+        xf, yf = nxf, nyf
+        xg = [[nxf[b][0] + np.ceil(un[b]), nxf[b][1] + np.ceil(un[b])] for b in range(len(nxf))]
+        yg = [[nyf[b][0] + np.ceil(vn[b]), nyf[b][1] + np.ceil(vn[b])] for b in range(len(nyf))]
+        xf = [list(r) for r in xf]; yf = [list(r) for r in yf]
+        clamp_boxes(xf, yf, xg, yg)
+        wx, wy = nwx, nwy
+        uc = [np.ceil(x) for x in un]
+        vc = [np.ceil(x) for x in vn]
+    tx = np.array([0.5 * (a[0] + a[1]) for a in nxf])
+    ty = np.array([0.5 * (a[0] + a[1]) for a in nyf])
+    tu = np.array(un, dtype=float)
+    tv = np.array(vn, dtype=float)
+    # :262-270 final grid limits
+    ix1, ix2 = int(np.argmin(tx)), int(np.argmax(tx))
+    iy1, iy2 = int(np.argmin(ty)), int(np.argmax(ty))
+    gap = lambda d: np.ceil(1 + 1.25 * winsize + d)
+    if gx[0] < gap(tu[ix1]): gx[0] = gap(tu[ix1])
+    if gy[0] < gap(tv[iy1]): gy[0] = gap(tv[iy1])
+    if gx[1] > H - gap(tu[ix2]) + 1: gx[1] = H - gap(tu[ix2]) + 1
+    if gy[1] > W - gap(tv[iy2]) + 1: gy[1] = W - gap(tv[iy2]) + 1
+    return tx, ty, tu, tv, phi, gx, gy
 
 
 def integer_search(f, g, r, gridx, gridy, winsize, step):

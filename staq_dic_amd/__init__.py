@@ -11,10 +11,11 @@ from .api import (  # noqa: F401
     local_icgn_raw, subpb1_raw,
 )
 from .global_step import Subpb2Quadtree  # noqa: F401
+from .mgsearch import funIntegerSearchMg, gridfit  # noqa: F401
 
 __all__ = [
     "Batch", "Context", "StaqError", "load", "params", "IntegerSearch", "LocalICGN",
     "LocalICGNQuadtree", "Subpb1", "Subpb1Quadtree", "ba_interp2", "classify_local",
     "classify_subpb1", "default_context", "funImgGradient", "funIntegerSearch",
-    "local_icgn_raw", "subpb1_raw", "Subpb2Quadtree",
+    "local_icgn_raw", "subpb1_raw", "Subpb2Quadtree", "funIntegerSearchMg", "gridfit",
 ]

@@ -134,27 +134,42 @@ def integer_search_slot(c, slot, tempSizeOfSearchRegion, gridx, gridy, winsize, 
 
 
 def IntegerSearch(ImgRef, ImgDef, file_name=None, DICpara=None, ctx=None):
-    """[DICpara,x0,y0,u,v,cc] = IntegerSearch(ImgRef, ImgDef, file_name, DICpara),
-    whole-field mode (InitFFTSearchMethod = 1).  The search radius is
-    DICpara['SizeOfFFTSearchRegion'] (the reference hard-codes 6,
-    IntegerSearch.m:49).  Grid points on a hole of DICpara['ImgRefMask'] are
-    NaN (:69-77); the median-test outlier filter follows (:116-119)."""
+    """[DICpara,x0,y0,u,v,cc] = IntegerSearch(ImgRef, ImgDef, file_name, DICpara).
+
+    DICpara['InitFFTSearchMethod'] 1 (the default, IntegerSearch.m:29-35):
+    whole-field search with radius DICpara['SizeOfFFTSearchRegion'] (the
+    reference hard-codes 6, :49), then the median-test outlier filter
+    (:116-119).  0: the multigrid search funIntegerSearchMg (:122-125, as
+    main_ALDIC_Quadtree_inc.m:91 sets it), no outlier filter.  2 (seed
+    points) needs interactive input and is not provided.  Either way grid
+    points on a hole of DICpara['ImgRefMask'] are NaN (:69-77, :128-135)."""
     P = dict(DICpara or {})
     H, W = np.shape(ImgRef)
     roi = P.get("gridxyROIRange", {"gridx": [1, H], "gridy": [1, W]})
-    r = P.get("SizeOfFFTSearchRegion", 6)
-    x0, y0, u, v, cc = funIntegerSearch(ImgRef, ImgDef, r, roi["gridx"], roi["gridy"],
-                                        P["winsize"], P["winstepsize"], 0, ctx=ctx)
+    method = P.get("InitFFTSearchMethod", 1)
+    if method == 0:
+        from .mgsearch import funIntegerSearchMg
+        r = 0
+        x0, y0, u, v, cc = funIntegerSearchMg(ImgRef, ImgDef, roi["gridx"], roi["gridy"],
+                                              P["winsize"], P["winstepsize"], ctx=ctx)
+    elif method == 1:
+        r = P.get("SizeOfFFTSearchRegion", 6)
+        x0, y0, u, v, cc = funIntegerSearch(ImgRef, ImgDef, r, roi["gridx"], roi["gridy"],
+                                            P["winsize"], P["winstepsize"], 0, ctx=ctx)
+    else:
+        raise NotImplementedError("seed-point search needs interactive input (IntegerSearch.m:58-64)")
     mask = P.get("ImgRefMask")
     if mask is not None and x0.size:
-        m = np.asarray(mask)[x0.astype(np.int64).ravel() - 1, y0.astype(np.int64).ravel() - 1]
+        xi = np.round(x0).astype(np.int64).ravel()
+        yi = np.round(y0).astype(np.int64).ravel()
+        m = np.asarray(mask)[xi - 1, yi - 1]
         hole = (m == 0).reshape(x0.shape)
         u = np.where(hole, np.nan, u)
         v = np.where(hole, np.nan, v)
     # median-test outlier filter, threshold 100 (IntegerSearch.m:116-119)
-    if np.isfinite(u).sum() >= 4 and min(np.shape(u)) >= 2:
+    if method == 1 and np.isfinite(u).sum() >= 4 and min(np.shape(u)) >= 2:
         u, v, cc, _, _, _ = funRemoveOutliers(u, v, cc, 0, 100)
-    P["InitFFTSearchMethod"] = 1
+    P["InitFFTSearchMethod"] = method
     P["SizeOfFFTSearchRegion"] = r
     return P, x0, y0, u, v, cc
 

@@ -9,7 +9,8 @@ import sys
 HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libstaqdic.so")
-SOURCES = ["capi.hip", "icgn.hip", "icgn_wave.hip", "icgn_lean.hip", "gather.hip", "search.hip"]
+SOURCES = ["capi.hip", "icgn.hip", "icgn_wave.hip", "icgn_lean.hip", "gather.hip", "search.hip",
+           "mgsearch.hip"]
 ARCH = os.environ.get("STAQ_OFFLOAD_ARCH", "gfx950")
 
 HIPCC_FLAGS = [
@@ -49,7 +50,8 @@ def build(verbose: bool = False, jobs: int | None = None) -> str:
     with ThreadPoolExecutor(jobs) as ex:
         list(ex.map(lambda c: _run(c, verbose), cmds))
     tmp = LIB + ".tmp"
-    _run([hipcc(), f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", tmp, *objs], verbose)
+    _run([hipcc(), f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", tmp, *objs,
+          "-L/opt/rocm/lib", "-lhipfft", "-Wl,-rpath,/opt/rocm/lib"], verbose)
     os.replace(tmp, LIB)
     return LIB
 

@@ -490,6 +490,104 @@ int staq_integer_search(staq_ctx *ctx, int slot, long r1, long r2, const long gr
     return STAQ_OK;
 }
 
+static bool frame_desc(staq_ctx *ctx, int slot, FrameDev &d)
+{
+    if (slot < 0 || (size_t)slot >= ctx->frames.size() || !ctx->frames[slot].set) return false;
+    const Frame &f = ctx->frames[slot];
+    d.f = f.f.as<float>();
+    d.g = f.g.as<float>();
+    d.mask = f.mask.as<float>();
+    d.dfdx = f.dfdx.as<float>();
+    d.dfdy = f.dfdy.as<float>();
+    d.H = (int)f.H;
+    d.W = (int)f.W;
+    return true;
+}
+
+int staq_xcorr_full_peak(staq_ctx *ctx, int slot, const long box_f[4], const long box_g[4],
+                         double out[3])
+{
+    if (!ctx || !box_f || !box_g || !out) return fail(ctx, STAQ_EINVAL, "null argument");
+    FrameDev fr;
+    if (!frame_desc(ctx, slot, fr)) return fail(ctx, STAQ_ENOFRAME, "frame slot not set");
+    const long n1 = box_f[1] - box_f[0] + 1, n2 = box_f[3] - box_f[2] + 1;
+    const long a1 = box_g[1] - box_g[0] + 1, a2 = box_g[3] - box_g[2] + 1;
+    if (n1 < 2 || n2 < 2 || a1 < n1 || a2 < n2 || box_f[0] < 1 || box_f[2] < 1 ||
+        box_f[1] > fr.H || box_f[3] > fr.W || box_g[0] < 1 || box_g[2] < 1 || box_g[1] > fr.H ||
+        box_g[3] > fr.W)
+        return fail(ctx, STAQ_EINVAL, "box outside the image, or template larger than the image");
+    hipSetDevice(ctx->device);
+    size_t bytes[4];
+    xcorr_full_sizes(n1, n2, a1, a2, bytes);
+    void *buf[4];
+    for (int k = 0; k < 4; ++k) {
+        HIPCHK(ctx, ctx->s[6 + k].ensure(bytes[k]));
+        buf[k] = ctx->s[6 + k].p;
+    }
+    char err[256] = {0};
+    const int rc = xcorr_full(fr, box_f, box_g, buf, ctx->stream, out, err, sizeof err);
+    if (rc) return fail(ctx, STAQ_EHIP, err);
+    return STAQ_OK;
+}
+
+int staq_xcorr_boxes(staq_ctx *ctx, int slot, long nbox, const long *box_f, const long *box_g,
+                     long pad, double *out, int *ok)
+{
+    if (!ctx || nbox < 0 || (nbox > 0 && (!box_f || !box_g || !out || !ok)))
+        return fail(ctx, STAQ_EINVAL, "null argument");
+    if (pad < 0 || pad > 6) return fail(ctx, STAQ_EINVAL, "pad must be 0..6");
+    FrameDev fr;
+    if (!frame_desc(ctx, slot, fr)) return fail(ctx, STAQ_ENOFRAME, "frame slot not set");
+    if (nbox == 0) return STAQ_OK;
+    hipSetDevice(ctx->device);
+    // boxes whose g support leaves the image, or whose template and g box
+    // differ in size, are the reference's catch (NaN); the others are computed
+    std::vector<long> bf, bg, idx;
+    std::vector<int> tile0(1, 0);
+    for (long b = 0; b < nbox; ++b) {
+        const long *F = box_f + 4 * b, *G = box_g + 4 * b;
+        const long n1 = F[1] - F[0] + 1, n2 = F[3] - F[2] + 1;
+        ok[b] = 0;
+        out[3 * b] = out[3 * b + 1] = out[3 * b + 2] = NAN;
+        const bool good = n1 >= 2 && n2 >= 2 && G[1] - G[0] + 1 == n1 && G[3] - G[2] + 1 == n2 &&
+                          F[0] >= 1 && F[2] >= 1 && F[1] <= fr.H && F[3] <= fr.W &&
+                          G[0] - pad >= 1 && G[2] - pad >= 1 && G[1] + pad <= fr.H &&
+                          G[3] + pad <= fr.W;
+        if (!good) continue;
+        idx.push_back(b);
+        bf.insert(bf.end(), F, F + 4);
+        bg.insert(bg.end(), G, G + 4);
+        tile0.push_back(tile0.back() + xcorr_box_tiles(n1, n2));
+    }
+    const long nb = (long)idx.size();
+    if (nb == 0) return STAQ_OK;
+    const long ntile = tile0.back();
+    hipStream_t s = ctx->stream;
+    HIPCHK(ctx, ctx->s[6].ensure((size_t)nb * 64 + (size_t)(nb + 1) * 4));
+    HIPCHK(ctx, ctx->s[7].ensure((size_t)ntile * xcorr_box_part_doubles((int)pad) * 8));
+    HIPCHK(ctx, ctx->s[8].ensure((size_t)nb * 24));
+    HIPCHK(ctx, ctx->s[9].ensure((size_t)nb * 4));
+    long *dbf = ctx->s[6].as<long>(), *dbg = dbf + 4 * nb;
+    int *dt0 = reinterpret_cast<int *>(dbg + 4 * nb);
+    HIPCHK(ctx, hipMemcpyAsync(dbf, bf.data(), nb * 32, hipMemcpyHostToDevice, s));
+    HIPCHK(ctx, hipMemcpyAsync(dbg, bg.data(), nb * 32, hipMemcpyHostToDevice, s));
+    HIPCHK(ctx, hipMemcpyAsync(dt0, tile0.data(), (nb + 1) * 4, hipMemcpyHostToDevice, s));
+    HIPCHK(ctx, xcorr_boxes_launch(fr, (int)pad, nb, ntile, dbf, dbg, dt0, ctx->s[7].as<double>(),
+                                   ctx->s[8].as<double>(), ctx->s[9].as<int>(), s));
+    std::vector<double> ho(3 * nb);
+    std::vector<int> hk(nb);
+    HIPCHK(ctx, hipMemcpyAsync(ho.data(), ctx->s[8].p, nb * 24, hipMemcpyDeviceToHost, s));
+    HIPCHK(ctx, hipMemcpyAsync(hk.data(), ctx->s[9].p, nb * 4, hipMemcpyDeviceToHost, s));
+    HIPCHK(ctx, hipStreamSynchronize(s));
+    for (long k = 0; k < nb; ++k) {
+        const long b = idx[k];
+        ok[b] = hk[k];
+        if (hk[k])
+            for (int q = 0; q < 3; ++q) out[3 * b + q] = ho[3 * k + q];
+    }
+    return STAQ_OK;
+}
+
 }  // extern "C"
 
 // ---------------------------------------------------------------------------

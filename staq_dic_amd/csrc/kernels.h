@@ -82,4 +82,19 @@ size_t search_lds_bytes(int w, int r1, int r2);
 size_t search_strip_lds_bytes(int w, int r1, int r2);
 bool search_uses_strip(int w, int r1, int r2);
 
+// multigrid search (mgsearch.hip).  Level 1: the full normxcorr2 map of
+// f(box_f) against g(box_g) and findpeak on it; four device buffers of the
+// sizes xcorr_full_sizes gives; out = xpeak (dim 2), ypeak (dim 1), max_f
+// (NaN for a constant template).  Returns 0 or a negative error with text.
+void xcorr_full_sizes(long n1, long n2, long a1, long a2, size_t bytes[4]);
+int xcorr_full(const FrameDev &fr, const long box_f[4], const long box_g[4], void *const buf[4],
+               hipStream_t s, double out[3], char *err, size_t errlen);
+// Levels >= 2: nb boxes (device arrays: bf, bg 4 per box; tile0 prefix sums of
+// xcorr_box_tiles), part = ntile * xcorr_box_part_doubles(pad) doubles
+size_t xcorr_box_part_doubles(int pad);
+int xcorr_box_tiles(long n1, long n2);
+hipError_t xcorr_boxes_launch(const FrameDev &fr, int pad, long nb, long ntile, const long *bf,
+                              const long *bg, const int *tile0, double *part, double *out, int *ok,
+                              hipStream_t s);
+
 }  // namespace staq

@@ -264,6 +264,22 @@ def large_leg(ctx, tm, steps, warmup):
               "fp64_peak_tflops": 78.6,
               "within_1px": float(np.mean((du <= 1) & (dv <= 1)))}
     log(f"[bench] large integer search: {x.size} points in {dt * 1e3:.1f} ms")
+    # the multigrid search of the incremental driver (InitFFTSearchMethod = 0):
+    # whole-region FFT normxcorr2, then the box levels, then gridfit on the host
+    from staq_dic_amd.mgsearch import mg_scatter, mg_search_slot
+    t = time.perf_counter()
+    tx, ty, tu, tv, phi, gx, gy = mg_scatter(ctx, 0, H, W, [1, H], [1, W], 64)
+    t_lv = time.perf_counter() - t
+    t = time.perf_counter()
+    xg, yg, ug, vg, ccg = mg_search_slot(ctx, 0, H, W, [1, H], [1, W], 64, 32)
+    t_all = time.perf_counter() - t
+    trg = synth.truth_at(fr.field, np.stack([xg.ravel(), yg.ravel()], axis=1))
+    okg = np.isfinite(ug.ravel())
+    search["multigrid"] = {"boxes_last_level": int(tx.size), "seconds_levels": t_lv,
+                           "seconds_with_gridfit": t_all, "grid_points": int(ug.size),
+                           "within_1px": float(np.mean((np.abs(ug.ravel()[okg] - trg[okg, 4]) <= 1)
+                                                       & (np.abs(vg.ravel()[okg] - trg[okg, 5]) <= 1)))}
+    log(f"[bench] multigrid search: levels {t_lv * 1e3:.1f} ms, with gridfit {t_all * 1e3:.1f} ms")
     res, b = local_step(ctx, tm, coords, win, truth, steps, warmup)
     b.close()
     res.update({"workload": "4096x4096 16-bit speckle (192 000 speckles, seed 1004), frames 39 "
@@ -323,7 +339,11 @@ def main():
 
     head, b = local_step(ctx, tm, coords, win, truth, args.steps, args.warmup)
     ms = head["ms_per_step"]
-    value = head["converged_subsets_per_s"]
+    # value: subsets per second that the reference keeps as converged local
+    # results (status 1-100 and not beyond its iteration-count threshold,
+    # LocalICGNQuadtree.m:151-169; the rest it refills), the set whose RMS
+    # error is the accuracy gate
+    value = head["kept"] / (ms / 1e3)
     rms = head["rms_px"]
     roof = head.pop("roofline")
     wl = ("reference window rule: every node 32 px (DICpara.winsize), grown at the hole"
@@ -337,9 +357,13 @@ def main():
                                "field 8 px, quadtree leaves 8-32 px, " + wl + ", 6-DOF IC-GN "
                                "local step, tol 1e-3",
                    "subsets": N, "parallelism": "1 GPU, one graph per local step"},
-        "converged": head["converged"], "rms_px": rms, "rms_px_kept": head["rms_px_kept"],
-        "accuracy_gate_0p01px": {"converged": bool(rms <= 0.01),
-                                 "kept_by_reference_rule": bool(head["rms_px_kept"] <= 0.01)},
+        "value_counts": "subsets kept by the reference's bad-subset rule (status 1-100, "
+                        "iterations <= max(mean + std, 6)) per second",
+        "kept": head["kept"], "rms_px_kept": head["rms_px_kept"],
+        "converged": head["converged"], "converged_subsets_per_s": head["converged_subsets_per_s"],
+        "rms_px": rms,
+        "accuracy_gate_0p01px": {"kept_by_reference_rule": bool(head["rms_px_kept"] <= 0.01),
+                                 "all_converged": bool(rms <= 0.01)},
         "schedule": head["schedule"], "roofline": roof,
     }
 
@@ -374,7 +398,8 @@ def main():
         cp = int(((stp >= 1) & (stp <= 100)).sum())
         result["plain_form"] = {"kernel_ms_sum": floor_ms, "converged": cp,
                                 "subsets_per_s": cp / (floor_ms / 1e3),
-                                "batched_vs_plain": value / (cp / (floor_ms / 1e3))}
+                                "batched_vs_plain": head["converged_subsets_per_s"] /
+                                                    (cp / (floor_ms / 1e3))}
         log(f"[bench] per-subset form floor {floor_ms:.1f} ms")
 
         # integer search on the same pair (w 32, step 16, r 10), frames resident

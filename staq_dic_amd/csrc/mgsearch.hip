@@ -170,19 +170,28 @@ __device__ __forceinline__ double ncc_at(const double2 *nd, long q, double tol)
 }
 
 // first maximum in column-major order: per-block (value, index), then one block
-__global__ void full_argmax_kernel(const double2 *nd, long n, const double *dmax, long nb_d,
+// largest of n values by one workgroup (the block maxima of den)
+__global__ void max_reduce_kernel(const double *v, long n, double *out)
+{
+    __shared__ double red[1024];
+    double m = 0.0;
+    for (long q = threadIdx.x; q < n; q += blockDim.x) m = fmax(m, v[q]);
+    red[threadIdx.x] = m;
+    __syncthreads();
+    for (int o = 512; o > 0; o >>= 1) {
+        if ((int)threadIdx.x < o) red[threadIdx.x] = fmax(red[threadIdx.x], red[threadIdx.x + o]);
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) out[0] = red[0];
+}
+
+__global__ void full_argmax_kernel(const double2 *nd, long n, const double *dmax_all,
                                    double *bv_out, long *bi_out)
 {
     __shared__ double sv[256];
     __shared__ long si[256];
-    __shared__ double tol_s;
-    if (threadIdx.x == 0) {
-        double mx = 0.0;
-        for (long b = 0; b < nb_d; ++b) mx = fmax(mx, dmax[b]);
-        tol_s = sqrt(nextafter(mx, INFINITY) - mx);  // sqrt(eps(max den))
-    }
-    __syncthreads();
-    const double tol = tol_s;
+    const double mx = dmax_all[0];
+    const double tol = sqrt(nextafter(mx, INFINITY) - mx);  // sqrt(eps(max den))
     double bv = -INFINITY;
     long bi = 0x7fffffffffffffffL;
     const long per = (n + gridDim.x - 1) / gridDim.x;
@@ -214,13 +223,11 @@ __global__ void full_argmax_kernel(const double2 *nd, long n, const double *dmax
     }
 }
 
-__global__ void full_peak_kernel(const double2 *nd, long k1, long k2, const double *dmax,
-                                 long nb_d, const double *bv, const long *bi, long nb,
-                                 double *out)
+__global__ void full_peak_kernel(const double2 *nd, long k1, long k2, const double *dmax_all,
+                                 const double *bv, const long *bi, long nb, double *out)
 {
     if (threadIdx.x != 0 || blockIdx.x != 0) return;
-    double mx = 0.0;
-    for (long b = 0; b < nb_d; ++b) mx = fmax(mx, dmax[b]);
+    const double mx = dmax_all[0];
     const double tol = sqrt(nextafter(mx, INFINITY) - mx);
     double v = bv[0];
     long i = bi[0];
@@ -233,15 +240,19 @@ __global__ void full_peak_kernel(const double2 *nd, long k1, long k2, const doub
     findpeak_at(C, k1, k2, i, out);
 }
 
-// template statistics of the region (sum T, sum (T - mean)^2, any two differ)
+// template statistics of the region (sum T, sum T^2, count of values other
+// than the first): per-block partials, then one block adds them in order
+constexpr int kStatBlocks = 512;
+
 __global__ void tstat_kernel(const float *f, int H, long fx0, long fy0, long n1, long n2,
-                             double *acc)
+                             double *part)
 {
     __shared__ double r0[256], r1[256], r2[256];
     const long n = n1 * n2;
     const float t0 = f[(fx0 - 1) + (size_t)(fy0 - 1) * H];
     double s = 0.0, ss = 0.0, dif = 0.0;
-    for (long q = threadIdx.x; q < n; q += blockDim.x) {
+    for (long q = (long)blockIdx.x * blockDim.x + threadIdx.x; q < n;
+         q += (long)gridDim.x * blockDim.x) {
         const long i = q % n1, j = q / n1;
         const double v = f[(fx0 + i - 1) + (size_t)(fy0 + j - 1) * H];
         s += v;
@@ -261,10 +272,24 @@ __global__ void tstat_kernel(const float *f, int H, long fx0, long fy0, long n1,
         __syncthreads();
     }
     if (threadIdx.x == 0) {
-        acc[0] = r0[0];
-        acc[1] = r1[0];
-        acc[2] = r2[0];
+        part[3 * blockIdx.x] = r0[0];
+        part[3 * blockIdx.x + 1] = r1[0];
+        part[3 * blockIdx.x + 2] = r2[0];
     }
+}
+
+__global__ void tstat_sum_kernel(const double *part, int nb, double *acc)
+{
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    double a = 0.0, b = 0.0, c = 0.0;
+    for (int k = 0; k < nb; ++k) {
+        a += part[3 * k];
+        b += part[3 * k + 1];
+        c += part[3 * k + 2];
+    }
+    acc[0] = a;
+    acc[1] = b;
+    acc[2] = c;
 }
 
 // ------------------------------------------------------------- levels >= 2
@@ -453,7 +478,7 @@ void xcorr_full_sizes(long n1, long n2, long a1, long a2, size_t bytes[4])
     bytes[0] = (size_t)P1 * P2 * 16;                            // FFT of rot180(T), then the product
     bytes[1] = (size_t)P1 * P2 * 16;                            // FFT of A
     bytes[2] = (size_t)k1 * (size_t)(k2 > a2 ? k2 : a2) * 32;  // column sums, then local sums
-    bytes[3] = (size_t)(nb_d + 2 * 1024 + 8) * 8;               // reductions
+    bytes[3] = (size_t)(nb_d + 2 * 1024 + 16 + 3 * kStatBlocks) * 8;  // reductions
 }
 
 int xcorr_full(const FrameDev &fr, const long box_f[4], const long box_g[4], void *const buf[4],
@@ -470,6 +495,7 @@ int xcorr_full(const FrameDev &fr, const long box_f[4], const long box_g[4], voi
     double *red = static_cast<double *>(buf[3]);
     double *dmax = red, *bv = red + nb_d, *stt = red + nb_d + 2 * nb_a;
     long *bi = reinterpret_cast<long *>(red + nb_d + nb_a);
+    double *dmax_all = stt + 8, *spart = stt + 16;
     auto hip_err = [&](hipError_t e, const char *what) {
         snprintf(err, errlen, "%s: %s", what, hipGetErrorString(e));
         return -5;
@@ -479,8 +505,9 @@ int xcorr_full(const FrameDev &fr, const long box_f[4], const long box_g[4], voi
     hipLaunchKernelGGL(full_stage_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, fr.f,
                        fr.g, (int)H, box_f[0], box_f[2], n1, n2, box_g[0], box_g[2], a1, a2, P1,
                        P2, ft, fa);
-    hipLaunchKernelGGL(tstat_kernel, dim3(1), dim3(256), 0, s, fr.f, (int)H, box_f[0], box_f[2], n1,
-                       n2, stt);
+    hipLaunchKernelGGL(tstat_kernel, dim3(kStatBlocks), dim3(256), 0, s, fr.f, (int)H, box_f[0],
+                       box_f[2], n1, n2, spart);
+    hipLaunchKernelGGL(tstat_sum_kernel, dim3(1), dim3(64), 0, s, spart, kStatBlocks, stt);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return hip_err(e, "xcorr stage");
     hipfftHandle plan;
@@ -520,9 +547,10 @@ int xcorr_full(const FrameDev &fr, const long box_f[4], const long box_g[4], voi
     const double denT = sqrt(fmax(st[1] - st[0] * st[0] / mn, 0.0));
     hipLaunchKernelGGL(full_numden_kernel, dim3((unsigned)nb_d), dim3(256), 0, s, ft, P1, ls, k1, k2,
                        st[0], mn, denT, (double)n, dmax);
-    hipLaunchKernelGGL(full_argmax_kernel, dim3((unsigned)nb_a), dim3(256), 0, s, ls, k1 * k2, dmax,
-                       nb_d, bv, bi);
-    hipLaunchKernelGGL(full_peak_kernel, dim3(1), dim3(64), 0, s, ls, k1, k2, dmax, nb_d, bv, bi,
+    hipLaunchKernelGGL(max_reduce_kernel, dim3(1), dim3(1024), 0, s, dmax, nb_d, dmax_all);
+    hipLaunchKernelGGL(full_argmax_kernel, dim3((unsigned)nb_a), dim3(256), 0, s, ls, k1 * k2,
+                       dmax_all, bv, bi);
+    hipLaunchKernelGGL(full_peak_kernel, dim3(1), dim3(64), 0, s, ls, k1, k2, dmax_all, bv, bi,
                        nb_a, stt);
     e = hipGetLastError();
     if (e == hipSuccess) e = hipMemcpyAsync(out, stt, 24, hipMemcpyDeviceToHost, s);

@@ -439,12 +439,20 @@ hipError_t search_launch(const SearchArgs &a, long npoints, hipStream_t s)
                             hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024 - 8192);
         hipFuncSetAttribute((const void *)search_strip_kernel<256, kStripP>,
                             hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024 - 8192);
+        hipFuncSetAttribute((const void *)search_strip_kernel<512, kStripP>,
+                            hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024 - 8192);
         attr = true;
     }
     if (search_uses_strip(a.w, a.r1, a.r2)) {
+        // a block that needs more than half the LDS runs alone on its CU:
+        // eight wavefronts then keep two per SIMD to hide the FP64 latency
         const size_t lds = search_strip_lds_bytes(a.w, a.r1, a.r2);
-        hipLaunchKernelGGL((search_strip_kernel<256, kStripP>), dim3((unsigned)npoints), dim3(256),
-                           lds, s, a);
+        if (lds > 64 * 1024)
+            hipLaunchKernelGGL((search_strip_kernel<512, kStripP>), dim3((unsigned)npoints),
+                               dim3(512), lds, s, a);
+        else
+            hipLaunchKernelGGL((search_strip_kernel<256, kStripP>), dim3((unsigned)npoints),
+                               dim3(256), lds, s, a);
     } else {
         const size_t lds = search_lds_bytes(a.w, a.r1, a.r2);
         hipLaunchKernelGGL(search_kernel<256>, dim3((unsigned)npoints), dim3(256), lds, s, a);

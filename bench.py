@@ -74,7 +74,7 @@ def kernel_name(bin_desc, mode=6):
     return f"icgn_kernel<{f},{mode}>"
 
 
-def accuracy(U, st, truth):
+def accuracy(U, st, truth, plain=False):
     """RMS (px) of (u, v) against the analytic field over converged subsets
     (status 1-100), and over the subsets the reference keeps after its
     bad-subset rule (LocalICGNQuadtree.m:151-169: status > max(mean + std, 6)
@@ -82,7 +82,7 @@ def accuracy(U, st, truth):
     from staq_dic_amd.api import classify_local
     conv = (st >= 1) & (st <= 100)
     err = np.hypot(U[0::2] - truth[:, 4], U[1::2] - truth[:, 5])
-    bad, _ = classify_local(st)
+    bad, _ = classify_local(st, plain=plain)
     kept = conv & ~bad
     rms = float(np.sqrt(np.mean(err[conv] ** 2))) if conv.any() else float("nan")
     rms_kept = float(np.sqrt(np.mean(err[kept] ** 2))) if kept.any() else float("nan")
@@ -168,19 +168,20 @@ def roofline(b, p, mode, st, win_x, win_y, steps, traffic=False):
 
 
 def local_step(ctx, tm, coords, win, truth, steps, warmup, slot=None, tune=True, roof=True,
-               traffic=False):
+               traffic=False, plain=False):
     """Build a batch, schedule it, replay the captured 6-DOF step ``steps``
-    times; returns (result dict, batch)."""
+    times; returns (result dict, batch).  ``plain``: the uniform-grid twin
+    (LocalICGN / funICGN semantics)."""
     from staq_dic_amd import Batch, params
     from staq_dic_amd._lib import IN_U0
     b = Batch(ctx, coords, win, slot=slot)
     U0 = np.round(truth[:, 4:6]).reshape(-1)
     b.set_input(IN_U0, U0)
-    p6 = params(tol=1e-3)
+    p6 = params(tol=1e-3, plain=plain)
     sched = schedule(b, p6, 6, tm, tune=tune)
     ms = tm.graph_ms(b, steps, warmup)
     U, F, Hp, st = b.outputs()
-    conv, err, rms, rms_kept, kept = accuracy(U, st, truth)
+    conv, err, rms, rms_kept, kept = accuracy(U, st, truth, plain)
     n_conv = int(conv.sum())
     out = {"subsets": int(len(coords)), "ms_per_step": ms, "converged": n_conv,
            "converged_subsets_per_s": n_conv / (ms / 1e3), "rms_px": rms,
@@ -379,6 +380,17 @@ def main():
                                roof=False)
         balt.close()
         result["leaf_window_rule" if rule == "reference" else "reference_window_rule"] = alt
+
+        # the plain method on its uniform grid (512^2 affine pair, 900 windows of 32 px)
+        fp, cp_, wp = synth.config_plain()
+        ctx.set_frame(fp.f, fp.g, fp.mask, slot=3)
+        plain_res, bp = local_step(ctx, tm, cp_, wp, synth.truth_at(fp.field, cp_), args.steps,
+                                   args.warmup, slot=np.full(len(cp_), 3, dtype=np.int32),
+                                   tune=False, roof=False, plain=True)
+        bp.close()
+        plain_res["workload"] = ("plain-512: 512x512 affine pair (seed 1001), uniform grid step 16, "
+                                 "window 32, LocalICGN semantics, tol 1e-3")
+        result["plain_method_512"] = plain_res
 
         # 2-DOF AL step on the 6-DOF result (identity global step, vdual = 0),
         # scheduled and replayed the same way

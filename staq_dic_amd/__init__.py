@@ -11,6 +11,7 @@ from .api import (  # noqa: F401
     local_icgn_raw, subpb1_raw,
 )
 from .global_step import Subpb2Quadtree  # noqa: F401
+from .initguess import Init, regularizeNd, uniform_initial_guess  # noqa: F401
 from .mesh import GenerateQuadtreeMesh, MeshSetUp  # noqa: F401
 from .mgsearch import funIntegerSearchMg, gridfit  # noqa: F401
 
@@ -19,5 +20,5 @@ __all__ = [
     "LocalICGNQuadtree", "Subpb1", "Subpb1Quadtree", "ba_interp2", "classify_local",
     "classify_subpb1", "default_context", "funImgGradient", "funIntegerSearch",
     "local_icgn_raw", "subpb1_raw", "Subpb2Quadtree", "funIntegerSearchMg", "gridfit",
-    "GenerateQuadtreeMesh", "MeshSetUp",
+    "GenerateQuadtreeMesh", "MeshSetUp", "Init", "regularizeNd", "uniform_initial_guess",
 ]

@@ -25,7 +25,8 @@ import numpy as np
 import scipy.sparse as sp
 import scipy.sparse.linalg as spla
 
-__all__ = ["Subpb2Quadtree", "gauss_points", "assemble_subpb2"]
+__all__ = ["Subpb2Quadtree", "gauss_points", "assemble_subpb2", "gauss_point_strain",
+           "funGlobalNodalStrainQuadtree"]
 
 
 def gauss_points(order):
@@ -179,3 +180,75 @@ def Subpb2Quadtree(DICmesh, GaussPtOrder, beta, mu, U, F, udual, vdual, alpha, w
         Aff = A[free][:, free].tocsc()
         Uhat[free] = spla.spsolve(Aff, b[free])
     return Uhat[:2 * N]
+
+
+def gauss_point_strain(coordinatesFEM, elementsFEM, U, GaussPtOrder):
+    """StrainGaussPt and CoordGaussPt of funGlobalNodalStrainQuadtree.m:
+    43-253: the displacement gradient [u_x, u_y, v_x, v_y] of the finite-
+    element field U at the GaussPtOrder^2 Gauss points of every element.
+
+    As the reference does (:35), the mid-node columns of elementsFEM are
+    cleared first, so every element is a plain bilinear quadrilateral.  The
+    element's (ksi, eta) are the bilinear maps that take the corners 1..4 to
+    ksi = (-1, 1, 1, -1), eta = (-1, -1, 1, 1) (:106-113); the Gauss points
+    lie on the box between corners 1 and 3 (:128-133), row
+    o^2 (e - 1) + o (i - 1) + j holding x point i and y point j (:202-216)."""
+    coords = np.asarray(coordinatesFEM, dtype=np.float64)
+    el = np.asarray(elementsFEM, dtype=np.int64)[:, :4]
+    E = el.shape[0]
+    U = np.asarray(U, dtype=np.float64).ravel()
+    pts, _ = gauss_points(GaussPtOrder)
+    o = pts.size
+    xy = coords[el - 1]  # (E, 4, 2)
+    Lm = np.stack([xy[..., 0] * xy[..., 1], xy[..., 0], xy[..., 1], np.ones((E, 4))], axis=2)
+    lk = np.linalg.solve(Lm, np.broadcast_to(np.array([-1.0, 1.0, 1.0, -1.0])[:, None], (E, 4, 1)))[..., 0]
+    le = np.linalg.solve(Lm, np.broadcast_to(np.array([-1.0, -1.0, 1.0, 1.0])[:, None], (E, 4, 1)))[..., 0]
+    p1, p3 = xy[:, 0], xy[:, 2]
+    gx = pts[None, :] * 0.5 * (p3[:, :1] - p1[:, :1]) + 0.5 * (p3[:, :1] + p1[:, :1])  # (E, o)
+    gy = pts[None, :] * 0.5 * (p3[:, 1:] - p1[:, 1:]) + 0.5 * (p3[:, 1:] + p1[:, 1:])
+    Ue = np.stack([U[2 * el - 2], U[2 * el - 1]], axis=2)  # (E, 4, 2)
+    zero = np.zeros((E, 4))
+    strain = np.zeros((E, o, o, 4))
+    cgp = np.zeros((E, o, o, 2))
+    for i in range(o):
+        for j in range(o):
+            X, Y = gx[:, i], gy[:, j]
+            ksi = lk[:, 0] * X * Y + lk[:, 1] * X + lk[:, 2] * Y + lk[:, 3]
+            eta = le[:, 0] * X * Y + le[:, 1] * X + le[:, 2] * Y + le[:, 3]
+            _, Dk, De = _shape(ksi, eta, zero)
+            Dk, De = Dk[:, :4], De[:, :4]
+            J11 = np.einsum("ea,ea->e", Dk, xy[..., 0])
+            J12 = np.einsum("ea,ea->e", Dk, xy[..., 1])
+            J21 = np.einsum("ea,ea->e", De, xy[..., 0])
+            J22 = np.einsum("ea,ea->e", De, xy[..., 1])
+            det = J11 * J22 - J12 * J21
+            Dx = (J22 / det)[:, None] * Dk + (-J12 / det)[:, None] * De
+            Dy = (-J21 / det)[:, None] * Dk + (J11 / det)[:, None] * De
+            strain[:, i, j, 0] = np.einsum("ea,ea->e", Dx, Ue[..., 0])
+            strain[:, i, j, 1] = np.einsum("ea,ea->e", Dy, Ue[..., 0])
+            strain[:, i, j, 2] = np.einsum("ea,ea->e", Dx, Ue[..., 1])
+            strain[:, i, j, 3] = np.einsum("ea,ea->e", Dy, Ue[..., 1])
+            cgp[:, i, j, 0] = X
+            cgp[:, i, j, 1] = Y
+    return strain.reshape(E * o * o, 4), cgp.reshape(E * o * o, 2)
+
+
+def funGlobalNodalStrainQuadtree(DICmesh, U, GaussPtOrder, waitBarDisplayOrNot=1):
+    """[StrainNodalPt, StrainGaussPt, CoordGaussPt] =
+    funGlobalNodalStrainQuadtree(DICmesh, U, GaussPtOrder, waitBarDisplayOrNot)
+    (func_quadtree/funGlobalNodalStrainQuadtree.m:1-334).
+
+    The Gauss-point gradients (``gauss_point_strain``) are carried to the
+    nodes by natural-neighbour interpolation with nearest-point extrapolation,
+    one column at a time (:285-292): nodal column k interpolates Gauss column
+    k.  StrainNodalPt is 4N, node-major; its four entries per node are the
+    Gauss columns in their order, [u_x, u_y, v_x, v_y], although the reference
+    names them F11, F21, F12, F22 (:294) - so entries 2 and 3 are in the
+    opposite order to the F of the local steps ([u_x, v_x, u_y, v_y]); kept
+    as the reference returns it.  The smoothing branch (:296-330) is not
+    reached: the reference fixes smoothness = 0 (:40)."""
+    from .natural import natural_nearest
+    coords = np.asarray(DICmesh["coordinatesFEM"], dtype=np.float64)
+    sg, cg = gauss_point_strain(coords, DICmesh["elementsFEM"], U, GaussPtOrder)
+    nodal = natural_nearest(cg, sg, coords)
+    return nodal.reshape(-1), sg, cg

@@ -1,0 +1,191 @@
+"""Natural-neighbour (Sibson) interpolation with nearest-site extrapolation,
+on the host (FP64): MATLAB's scatteredInterpolant(x, y, v, 'natural',
+'nearest') as the reference calls it (func_quadtree/
+funGlobalNodalStrainQuadtree.m:285-292, func_quadtree/
+LocalICGNQuadtree.m:313-330, func_quadtree/Subpb1Quadtree.m:247-253).
+
+Sibson's coordinates of a query q inside the convex hull of the sites: insert
+q into the Voronoi diagram of the sites; the weight of site i is the area
+that q's new cell takes from i's old cell, over the area of q's new cell.
+Here that is computed without building either diagram whole:
+
+1. the natural neighbours of q are the vertices of the Delaunay triangles
+   whose circumcircle contains q (the Bowyer-Watson cavity, found by a walk
+   from the triangle that contains q);
+2. q's new cell is the intersection of the half-planes nearer to q than to
+   each natural neighbour;
+3. the part of that cell that site i loses is the cell cut to the half-planes
+   nearer to i than to every other natural neighbour (inside q's cell, the
+   nearest old site is always a natural neighbour).
+
+The weights do not depend on how the triangulation breaks ties between
+co-circular sites (the Voronoi diagram is unique): such a tie only adds
+candidates whose share is empty.  A query on a site takes the site's value; a
+query on the hull boundary takes the linear interpolation between the ends
+of its hull edge (the limit of Sibson's coordinates there); a query outside
+the hull takes the value of its nearest site, the lowest-numbered one where
+several are equally near (MATLAB documents no tie rule).
+"""
+from __future__ import annotations
+
+import numpy as np
+from scipy.spatial import Delaunay, cKDTree
+
+__all__ = ["natural_nearest", "sibson_weights"]
+
+
+def _clip(poly, a, b):
+    """Convex polygon (k, 2) cut to the half-plane a . x <= b."""
+    s = poly @ a - b
+    inside = s <= 0.0
+    if inside.all():
+        return poly
+    if not inside.any():
+        return poly[:0]
+    out = []
+    k = poly.shape[0]
+    for i in range(k):
+        j = i + 1 if i + 1 < k else 0
+        if inside[i]:
+            out.append(poly[i])
+        if inside[i] != inside[j]:
+            t = s[i] / (s[i] - s[j])
+            out.append(poly[i] + t * (poly[j] - poly[i]))
+    return np.array(out)
+
+
+def _area(poly):
+    if poly.shape[0] < 3:
+        return 0.0
+    x, y = poly[:, 0], poly[:, 1]
+    return 0.5 * abs(np.dot(x, np.roll(y, -1)) - np.dot(y, np.roll(x, -1)))
+
+
+class _Triangulation:
+    def __init__(self, sites):
+        self.p = np.asarray(sites, dtype=np.float64)
+        self.tri = Delaunay(self.p)
+        s = self.tri.simplices
+        a, b, c = self.p[s[:, 0]], self.p[s[:, 1]], self.p[s[:, 2]]
+        # circumcentres, relative to vertex a for accuracy
+        bx, by = b[:, 0] - a[:, 0], b[:, 1] - a[:, 1]
+        cx, cy = c[:, 0] - a[:, 0], c[:, 1] - a[:, 1]
+        d = 2.0 * (bx * cy - by * cx)
+        b2, c2 = bx * bx + by * by, cx * cx + cy * cy
+        ux = (cy * b2 - by * c2) / d
+        uy = (bx * c2 - cx * b2) / d
+        self.cc = np.stack([a[:, 0] + ux, a[:, 1] + uy], axis=1)
+        self.r2 = ux * ux + uy * uy
+        span = np.ptp(self.p, axis=0).max()
+        self.span = span if span > 0 else 1.0
+        self.kd = cKDTree(self.p)
+
+    def cavity(self, q, s0):
+        """Vertices of the triangles whose circumcircle holds q, by a walk
+        over triangle neighbours from s0."""
+        tol = 1e-10
+        seen = {s0}
+        stack = [s0]
+        verts = set(self.tri.simplices[s0])
+        nb = self.tri.neighbors
+        while stack:
+            t = stack.pop()
+            for u in nb[t]:
+                if u < 0 or u in seen:
+                    continue
+                seen.add(u)
+                d = q - self.cc[u]
+                if d @ d <= self.r2[u] * (1.0 + tol):
+                    stack.append(u)
+                    verts.update(self.tri.simplices[u])
+        return np.array(sorted(verts))
+
+    def on_hull(self, q, s):
+        """(i, j, t) when q lies on a hull edge of triangle s, at
+        q = (1-t) p_i + t p_j; a hull edge is one without a neighbour."""
+        p = self.p
+        eps = 1e-12 * self.span
+        verts = self.tri.simplices[s]
+        for k in range(3):
+            if self.tri.neighbors[s][k] >= 0:
+                continue
+            i, j = verts[(k + 1) % 3], verts[(k + 2) % 3]
+            e = p[j] - p[i]
+            L2 = e @ e
+            w = q - p[i]
+            if abs(e[0] * w[1] - e[1] * w[0]) <= eps * np.sqrt(L2):
+                t = (w @ e) / L2
+                return i, j, min(max(t, 0.0), 1.0)
+        return None
+
+
+def _sibson(T, q, s0):
+    """Indices and Sibson weights of the natural neighbours of interior q."""
+    p = T.p
+    cand = T.cavity(q, s0)
+    big = 1e4 * T.span
+    cell = np.array([[q[0] - big, q[1] - big], [q[0] + big, q[1] - big],
+                     [q[0] + big, q[1] + big], [q[0] - big, q[1] + big]])
+    for j in cand:
+        a = p[j] - q
+        cell = _clip(cell, a, a @ (0.5 * (p[j] + q)))
+    areas = np.zeros(cand.size)
+    for k, i in enumerate(cand):
+        poly = cell
+        for j in cand:
+            if j == i:
+                continue
+            a = p[j] - p[i]
+            poly = _clip(poly, a, a @ (0.5 * (p[j] + p[i])))
+            if poly.shape[0] < 3:
+                break
+        areas[k] = _area(poly)
+    tot = areas.sum()
+    keep = areas > 0
+    return cand[keep], areas[keep] / tot
+
+
+def sibson_weights(sites, q):
+    """(indices, weights) of Sibson's coordinates of one interior query q."""
+    T = _Triangulation(sites)
+    q = np.asarray(q, dtype=np.float64)
+    s0 = int(T.tri.find_simplex(q[None, :])[0])
+    if s0 < 0:
+        raise ValueError("query outside the convex hull of the sites")
+    return _sibson(T, q, s0)
+
+
+def natural_nearest(sites, values, queries):
+    """scatteredInterpolant(sites, values, 'natural', 'nearest') evaluated at
+    queries: sites (n, 2), values (n,) or (n, m), queries (k, 2)."""
+    sites = np.asarray(sites, dtype=np.float64)
+    V = np.asarray(values, dtype=np.float64)
+    squeeze = V.ndim == 1
+    if squeeze:
+        V = V[:, None]
+    Q = np.asarray(queries, dtype=np.float64)
+    out = np.empty((Q.shape[0], V.shape[1]))
+    if sites.shape[0] < 3:
+        raise ValueError("natural-neighbour interpolation needs at least three sites")
+    T = _Triangulation(sites)
+    simp = T.tri.find_simplex(Q)
+    dist, near = T.kd.query(Q, k=1)
+    for k in range(Q.shape[0]):
+        q = Q[k]
+        if dist[k] <= 1e-12 * T.span:
+            out[k] = V[near[k]]
+            continue
+        if simp[k] < 0:
+            # outside the hull: the nearest site, the lowest index on a tie
+            cand = T.kd.query_ball_point(q, dist[k] * (1 + 1e-12) + 1e-300)
+            out[k] = V[min(cand)] if cand else V[near[k]]
+            continue
+        edge = T.on_hull(q, int(simp[k]))
+        if edge is not None:
+            i, j, t = edge
+            out[k] = (1.0 - t) * V[i] + t * V[j]
+            continue
+        idx, w = _sibson(T, q, int(simp[k]))
+        out[k] = w @ V[idx]
+    return out[:, 0] if squeeze else out
+

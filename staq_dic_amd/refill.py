@@ -15,18 +15,17 @@ bad subsets (NaN after the bad-subset classification) in two stages:
 
 Stage 1 solves the reference's dense (n+3) x (n+3) system [A P; P' 0]; the
 matrix is the same for every field of a component, so it is factored once and
-solved for all fields together.  Stage 2 uses linear interpolation on the
-Delaunay triangulation with nearest-node extrapolation: SciPy has no
-natural-neighbour interpolant, so this stage is an approximation of the
-reference (exact at the data nodes, different inside triangles).
+solved for all fields together.  Stage 2 is Sibson's natural-neighbour
+interpolation with nearest-node extrapolation (``natural.natural_nearest``).
 """
 from __future__ import annotations
 
 import numpy as np
 import scipy.linalg as sla
 from scipy import ndimage
-from scipy.interpolate import LinearNDInterpolator, NearestNDInterpolator
 from scipy.spatial.distance import cdist
+
+from .natural import natural_nearest
 
 __all__ = ["thinplate_fit", "thinplate_eval", "refill_bad"]
 
@@ -59,15 +58,6 @@ def thinplate_eval(centers, coeff, points):
     n = c.shape[0]
     r = cdist(p, c)
     return _phi(r) @ coeff[:n] + coeff[n] + p[:, :1] * coeff[n + 1] + p[:, 1:2] * coeff[n + 2]
-
-
-def _natural_nearest(xy_known, vals, xy_all):
-    lin = LinearNDInterpolator(xy_known, vals)
-    out = lin(xy_all)
-    bad = ~np.isfinite(out).all(axis=1) if out.ndim > 1 else ~np.isfinite(out)
-    if bad.any():
-        out[bad] = NearestNDInterpolator(xy_known, vals)(xy_all[bad])
-    return out
 
 
 def refill_bad(U, F, coordinatesFEM, mask):
@@ -108,7 +98,7 @@ def refill_bad(U, F, coordinatesFEM, mask):
     nan_u = ~np.isfinite(U[:, 0])
     nan_f = np.zeros(N, dtype=bool) if Fm is None else ~np.isfinite(Fm[:, 0])
     if nan_u.any() or nan_f.any():
-        U = _natural_nearest(coords[~nan_u], U[~nan_u], coords)
+        U = natural_nearest(coords[~nan_u], U[~nan_u], coords)
         if Fm is not None:
-            Fm = _natural_nearest(coords[~nan_f], Fm[~nan_f], coords)
+            Fm = natural_nearest(coords[~nan_f], Fm[~nan_f], coords)
     return U.reshape(-1), (None if Fm is None else Fm.reshape(-1))

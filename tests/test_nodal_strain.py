@@ -1,0 +1,136 @@
+"""Natural-neighbour interpolation and the FE nodal strain
+(funGlobalNodalStrainQuadtree).
+
+Sibson's coordinates are pinned against a brute-force count of the area each
+site loses when the query is inserted (a raster of nearest-site labels before
+and after), against symmetry (the centre of a square of sites takes a
+quarter from each) and against linear precision, which every natural-
+neighbour interpolant has.  Outside the hull the nearest site is used.
+
+The Gauss-point gradients are pinned on the reference's own quadtree mesh
+(tests/golden/circular_hole_000_001.npz, with its hanging nodes) by fields a
+bilinear element reproduces: a linear field gives its constant gradient at
+every Gauss point and node, in the column order [u_x, u_y, v_x, v_y]; the
+field u = c x y gives u_x = c y and u_y = c x at the Gauss points' own
+coordinates, which pins where they lie; at a node inside a uniform patch the
+four nearest Gauss points sit symmetrically, so the nodal value is their mean.
+"""
+import os
+
+import numpy as np
+import pytest
+from scipy.spatial import cKDTree
+
+from staq_dic_amd.global_step import funGlobalNodalStrainQuadtree, gauss_point_strain
+from staq_dic_amd.natural import natural_nearest, sibson_weights
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def test_sibson_weights_brute_force():
+    rng = np.random.default_rng(0)
+    P = rng.uniform(0, 10, (40, 2))
+    xs = np.linspace(0, 10, 1200)
+    X, Y = np.meshgrid(xs, xs)
+    pix = np.stack([X.ravel(), Y.ravel()], axis=1)
+    d_old, lab_old = cKDTree(P).query(pix)
+    for q in (np.array([5.1, 4.7]), np.array([3.3, 6.2])):
+        idx, w = sibson_weights(P, q)
+        assert abs(w.sum() - 1) < 1e-12 and (w > 0).all()
+        stolen = np.hypot(pix[:, 0] - q[0], pix[:, 1] - q[1]) < d_old
+        cnt = np.bincount(lab_old[stolen], minlength=P.shape[0]).astype(float)
+        wb = cnt / cnt.sum()
+        np.testing.assert_allclose(w, wb[idx], atol=2e-3)
+        assert wb[np.setdiff1d(np.arange(P.shape[0]), idx)].sum() < 1e-3
+
+
+def test_sibson_symmetry_and_linear_precision():
+    g = np.arange(6.0)
+    X, Y = np.meshgrid(g, g, indexing="ij")
+    P = np.stack([X.ravel(), Y.ravel()], axis=1)
+    idx, w = sibson_weights(P, np.array([2.5, 2.5]))
+    assert sorted(idx.tolist()) == [14, 15, 20, 21]
+    np.testing.assert_allclose(w, 0.25, atol=1e-12)
+    rng = np.random.default_rng(1)
+    S = rng.uniform(0, 10, (300, 2))
+    Q = rng.uniform(1, 9, (150, 2))
+    lin = lambda p: np.stack([1.5 + 0.3 * p[:, 0] - 0.7 * p[:, 1], -2 + 0.1 * p[:, 1]], axis=1)
+    np.testing.assert_allclose(natural_nearest(S, lin(S), Q), lin(Q), atol=1e-9)
+    # a query on a site takes its value; outside the hull the nearest site's
+    V = rng.normal(size=300)
+    np.testing.assert_array_equal(natural_nearest(S, V, S[:5]), V[:5])
+    far = np.array([[-5.0, -5.0], [20.0, 3.0]])
+    near = cKDTree(S).query(far)[1]
+    np.testing.assert_array_equal(natural_nearest(S, V, far), V[near])
+    # on the hull boundary: linear along the hull edge
+    sq = np.array([[0.0, 0.0], [4.0, 0.0], [4.0, 4.0], [0.0, 4.0], [2.0, 1.5]])
+    out = natural_nearest(sq, np.array([1.0, 3.0, 5.0, 7.0, 0.0]), np.array([[1.0, 0.0]]))
+    np.testing.assert_allclose(out, [1.5], atol=1e-12)
+
+
+def _mesh():
+    z = np.load(os.path.join(ROOT, "tests", "golden", "circular_hole_000_001.npz"))
+    return {"coordinatesFEM": z["coordinatesFEM"], "elementsFEM": z["elementsFEM"]}
+
+
+@pytest.mark.parametrize("order", [2, 3])
+def test_gauss_strain_linear_and_bilinear(order):
+    m = _mesh()
+    c = m["coordinatesFEM"]
+    E = m["elementsFEM"].shape[0]
+    A = np.array([[0.011, -0.004], [0.007, -0.013]])  # [[u_x, u_y], [v_x, v_y]]
+    U = np.empty(2 * c.shape[0])
+    U[0::2] = 0.5 + c @ A[0]
+    U[1::2] = -1.0 + c @ A[1]
+    sg, cg = gauss_point_strain(c, m["elementsFEM"], U, order)
+    assert sg.shape == (E * order * order, 4) and cg.shape == (E * order * order, 2)
+    np.testing.assert_allclose(sg, np.tile([A[0, 0], A[0, 1], A[1, 0], A[1, 1]], (sg.shape[0], 1)),
+                               atol=1e-12)
+    # u = c x y, v = 0: u_x = c y and u_y = c x at the Gauss points
+    U2 = np.zeros_like(U)
+    U2[0::2] = 1e-3 * c[:, 0] * c[:, 1]
+    sg2, cg2 = gauss_point_strain(c, m["elementsFEM"], U2, order)
+    np.testing.assert_allclose(sg2[:, 0], 1e-3 * cg2[:, 1], rtol=1e-10, atol=1e-10)
+    np.testing.assert_allclose(sg2[:, 1], 1e-3 * cg2[:, 0], rtol=1e-10, atol=1e-10)
+    np.testing.assert_allclose(sg2[:, 2:], 0, atol=1e-14)
+    # row o^2 (e-1) + o (i-1) + j: x point i, y point j of element e
+    from staq_dic_amd.global_step import gauss_points
+    pts, _ = gauss_points(order)
+    e = 17
+    n1, n3 = m["elementsFEM"][e, 0] - 1, m["elementsFEM"][e, 2] - 1
+    mid = 0.5 * (c[n1] + c[n3])
+    half = 0.5 * (c[n3] - c[n1])
+    blk = cg[e * order * order:(e + 1) * order * order]
+    for i in range(order):
+        for j in range(order):
+            np.testing.assert_allclose(blk[order * i + j], mid + half * [pts[i], pts[j]], atol=1e-12)
+
+
+def test_nodal_strain_on_reference_mesh():
+    m = _mesh()
+    c = m["coordinatesFEM"]
+    A = np.array([[0.011, -0.004], [0.007, -0.013]])
+    U = np.empty(2 * c.shape[0])
+    U[0::2] = 0.5 + c @ A[0]
+    U[1::2] = -1.0 + c @ A[1]
+    nodal, sg, cg = funGlobalNodalStrainQuadtree(m, U, 2, 0)
+    assert nodal.shape == (4 * c.shape[0],)
+    np.testing.assert_allclose(nodal.reshape(-1, 4), np.tile(A.ravel(), (c.shape[0], 1)), atol=1e-12)
+    # u = c x y: inside a uniform patch a node is the mean of its four
+    # nearest Gauss points, c y0 for u_x and c x0 for u_y
+    U2 = np.zeros_like(U)
+    U2[0::2] = 1e-3 * c[:, 0] * c[:, 1]
+    nodal2, sg2, cg2 = funGlobalNodalStrainQuadtree(m, U2, 2, 0)
+    d, _ = cKDTree(cg2).query(c, k=5)
+    sym = (np.abs(d[:, :4] - d[:, :1]) < 1e-9).all(axis=1) & (d[:, 4] > d[:, 0] * 1.5)
+    assert sym.sum() > 300
+    n2 = nodal2.reshape(-1, 4)
+    np.testing.assert_allclose(n2[sym, 0], 1e-3 * c[sym, 1], rtol=1e-9)
+    np.testing.assert_allclose(n2[sym, 1], 1e-3 * c[sym, 0], rtol=1e-9)
+    # border nodes lie outside the Gauss points' hull: a nearest Gauss value
+    border = (c[:, 0] == c[:, 0].min()) | (c[:, 1] == c[:, 1].min())
+    dn, near = cKDTree(cg2).query(c[border])
+    vals = sg2[:, 0]
+    for k, node in enumerate(np.flatnonzero(border)):
+        ties = np.flatnonzero(np.abs(np.hypot(*(cg2 - c[node]).T) - dn[k]) < 1e-9)
+        assert n2[node, 0] in vals[ties]

@@ -493,6 +493,43 @@ def main():
                                   "sample": f"{reps} x the full {N}-subset 6-DOF local step "
                                             f"(FP64 oracle, OpenMP dynamic), {t_cpu:.1f} s"}
         log(f"[bench] cpu {cconv / t_cpu:.0f} subsets/s on {cores} threads")
+        # one core on every 4th subset of the same step
+        sel = np.arange(0, N, 4)
+        t = time.perf_counter()
+        _, _, _, s1 = pyoracle.local_icgn(of, U0.reshape(-1, 2)[sel].reshape(-1), coords[sel],
+                                          win[sel], tol=1e-3, nthreads=1)
+        t1 = time.perf_counter() - t
+        c1 = int(((s1 >= 1) & (s1 <= 100)).sum())
+        # the integer search on the same pair (w 32, step 16, r 10), all cores
+        t = time.perf_counter()
+        xs, _, _, _, _ = pyoracle.integer_search(fr.f, fr.g, 10, [1, 1024], [1, 1024], 32, 16)
+        ts = time.perf_counter() - t
+        result["cpu_baseline"].update({
+            "single_core": {"value": c1 / t1, "unit": "subsets/s", "cores": 1,
+                            "sample": f"every 4th subset ({sel.size}) of the same step, {t1:.2f} s"},
+            "integer_search": {"value": xs.size / ts, "unit": "points/s", "cores": cores,
+                               "sample": f"the {xs.size}-point search of the same pair "
+                                         f"(w 32, step 16, r 10), {ts:.2f} s"}})
+        log(f"[bench] cpu one core {c1 / t1:.0f} subsets/s; search {xs.size / ts:.0f} points/s")
+
+        # the host stages around the path (this package's own host code, FP64):
+        # the global step and the FE nodal strain on the reference's 1434-node
+        # quadtree mesh with its stored answer
+        from staq_dic_amd.global_step import Subpb2Quadtree, funGlobalNodalStrainQuadtree
+        z = np.load(os.path.join(os.path.dirname(os.path.abspath(__file__)), "tests", "golden",
+                                 "circular_hole_000_001.npz"))
+        mesh = {"coordinatesFEM": z["coordinatesFEM"], "elementsFEM": z["elementsFEM"],
+                "dirichlet": z["dirichlet"]}
+        t = time.perf_counter()
+        Uh = Subpb2Quadtree(mesh, 2, 1e-3 * 16 ** 2, 1e-3, z["U1"], z["F1"], 0 * z["F1"],
+                            0 * z["U1"], 0.0, 16.0, 0)
+        t_s2 = time.perf_counter() - t
+        t = time.perf_counter()
+        funGlobalNodalStrainQuadtree(mesh, Uh, 2, 0)
+        t_ns = time.perf_counter() - t
+        result["host_stages"] = {"mesh": "reference circular-hole quadtree mesh, 1434 nodes",
+                                 "subpb2_s": t_s2, "nodal_strain_s": t_ns}
+        log(f"[bench] host Subpb2 {t_s2:.3f} s, nodal strain {t_ns:.3f} s")
 
     print(json.dumps(result))
 

@@ -14,9 +14,11 @@ Here that is computed without building either diagram whole:
    from the triangle that contains q);
 2. q's new cell is the intersection of the half-planes nearer to q than to
    each natural neighbour;
-3. the part of that cell that site i loses is the cell cut to the half-planes
-   nearer to i than to every other natural neighbour (inside q's cell, the
-   nearest old site is always a natural neighbour).
+3. the natural neighbours proper are the candidates whose bisector with q
+   carries an edge of that cell; the part of the cell that site i loses is
+   the cell cut to i's old Voronoi cell, the half-planes nearer to i than to
+   each of its Delaunay neighbours (co-circular ties only add half-planes
+   that cut nothing).
 
 The weights do not depend on how the triangulation breaks ties between
 co-circular sites (the Voronoi diagram is unique): such a tie only adds
@@ -34,31 +36,38 @@ from scipy.spatial import Delaunay, cKDTree
 __all__ = ["natural_nearest", "sibson_weights"]
 
 
-def _clip(poly, a, b):
-    """Convex polygon (k, 2) cut to the half-plane a . x <= b."""
-    s = poly @ a - b
-    inside = s <= 0.0
-    if inside.all():
-        return poly
-    if not inside.any():
-        return poly[:0]
+def _clip(poly, ax, ay, b):
+    """Convex polygon (list of (x, y)) cut to the half-plane ax x + ay y <= b.
+    Plain floats: the polygons here have a handful of vertices."""
     out = []
-    k = poly.shape[0]
-    for i in range(k):
-        j = i + 1 if i + 1 < k else 0
-        if inside[i]:
-            out.append(poly[i])
-        if inside[i] != inside[j]:
-            t = s[i] / (s[i] - s[j])
-            out.append(poly[i] + t * (poly[j] - poly[i]))
-    return np.array(out)
+    k = len(poly)
+    if k == 0:
+        return out
+    px, py = poly[-1]
+    sp = ax * px + ay * py - b
+    for cx, cy in poly:
+        sc = ax * cx + ay * cy - b
+        if sc <= 0.0:
+            if sp > 0.0:
+                t = sp / (sp - sc)
+                out.append((px + t * (cx - px), py + t * (cy - py)))
+            out.append((cx, cy))
+        elif sp <= 0.0:
+            t = sp / (sp - sc)
+            out.append((px + t * (cx - px), py + t * (cy - py)))
+        px, py, sp = cx, cy, sc
+    return out
 
 
 def _area(poly):
-    if poly.shape[0] < 3:
+    if len(poly) < 3:
         return 0.0
-    x, y = poly[:, 0], poly[:, 1]
-    return 0.5 * abs(np.dot(x, np.roll(y, -1)) - np.dot(y, np.roll(x, -1)))
+    s = 0.0
+    px, py = poly[-1]
+    for cx, cy in poly:
+        s += px * cy - py * cx
+        px, py = cx, cy
+    return 0.5 * abs(s)
 
 
 class _Triangulation:
@@ -79,6 +88,7 @@ class _Triangulation:
         span = np.ptp(self.p, axis=0).max()
         self.span = span if span > 0 else 1.0
         self.kd = cKDTree(self.p)
+        self.pl = [(float(x), float(y)) for x, y in self.p]
 
     def cavity(self, q, s0):
         """Vertices of the triangles whose circumcircle holds q, by a walk
@@ -121,28 +131,40 @@ class _Triangulation:
 
 def _sibson(T, q, s0):
     """Indices and Sibson weights of the natural neighbours of interior q."""
-    p = T.p
     cand = T.cavity(q, s0)
+    qx, qy = float(q[0]), float(q[1])
+    pts = [(float(T.p[j, 0]), float(T.p[j, 1])) for j in cand]
     big = 1e4 * T.span
-    cell = np.array([[q[0] - big, q[1] - big], [q[0] + big, q[1] - big],
-                     [q[0] + big, q[1] + big], [q[0] - big, q[1] + big]])
-    for j in cand:
-        a = p[j] - q
-        cell = _clip(cell, a, a @ (0.5 * (p[j] + q)))
-    areas = np.zeros(cand.size)
-    for k, i in enumerate(cand):
+    cell = [(qx - big, qy - big), (qx + big, qy - big), (qx + big, qy + big), (qx - big, qy + big)]
+    for (x, y) in pts:
+        ax, ay = x - qx, y - qy
+        cell = _clip(cell, ax, ay, ax * 0.5 * (x + qx) + ay * 0.5 * (y + qy))
+    # the natural neighbours proper: the candidates whose bisector with q
+    # carries an edge of the cell (two cell vertices on it); inside the cell
+    # the nearest old site is always one of them
+    cv = np.array(cell)
+    P = T.p[cand]
+    A = P - q
+    S = cv @ A.T - (0.5 * ((P + q) * A).sum(axis=1))[None, :]
+    tol = 1e-9 * T.span * np.sqrt((A * A).sum(axis=1))
+    nn = np.flatnonzero((np.abs(S) <= tol[None, :]).sum(axis=0) >= 2)
+    # site i's old cell is cut out by its Delaunay neighbours alone
+    ptr, nbr = T.tri.vertex_neighbor_vertices
+    areas = np.zeros(nn.size)
+    for k, i in enumerate(nn):
+        gi = cand[i]
+        xi, yi = pts[i]
         poly = cell
-        for j in cand:
-            if j == i:
-                continue
-            a = p[j] - p[i]
-            poly = _clip(poly, a, a @ (0.5 * (p[j] + p[i])))
-            if poly.shape[0] < 3:
+        for gj in nbr[ptr[gi]:ptr[gi + 1]]:
+            xj, yj = T.pl[gj]
+            ax, ay = xj - xi, yj - yi
+            poly = _clip(poly, ax, ay, ax * 0.5 * (xj + xi) + ay * 0.5 * (yj + yi))
+            if len(poly) < 3:
                 break
         areas[k] = _area(poly)
     tot = areas.sum()
     keep = areas > 0
-    return cand[keep], areas[keep] / tot
+    return cand[nn[keep]], areas[keep] / tot
 
 
 def sibson_weights(sites, q):

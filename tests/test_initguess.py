@@ -155,3 +155,24 @@ def test_front_half_on_reference_frames(ctx, ref_frames):
     bad, _ = classify_local(conv)
     d = np.hypot(Uf[0::2] - r["U1"][0::2], Uf[1::2] - r["U1"][1::2])
     assert np.median(d[~bad]) < 5e-3 and np.percentile(d[~bad], 90) < 0.06
+
+
+@pytest.mark.gpu
+def test_al_frame_on_reference_frames(ctx, ref_frames):
+    """Sections 4-6 of the incremental driver on the reference's own frames
+    and mesh (local 6-DOF step, thin-plate refit, beta tuning, global step
+    and FE nodal strain, two rounds of the 2-DOF step): the result stays
+    close to the reference's stored post-ADMM answer, which its v2 driver
+    reached with the RBF nodal strain instead of the FE one."""
+    from staq_dic_amd.aldic import ALDICQuadtreeFrame
+    r = ref_frames
+    print("AL frame on the reference frames", flush=True)
+    out = ALDICQuadtreeFrame(np.round(r["U1"]), r["mesh"], {"ImgRefMask": r["mask"]}, r["f"],
+                             r["g"], {"winsize": 32, "winstepsize": 16}, tol=1e-2, ctx=ctx)
+    U2 = out["USubpb2"]
+    assert np.isfinite(U2).all() and np.isfinite(out["FSubpb2"]).all()
+    assert out["ALSolveStep"] == 3
+    bl = np.array([1e-3, 1e-2, 1e-1]) * 256 * 1e-3
+    assert bl[0] <= out["beta"] <= bl[-1]
+    d = np.hypot(U2[0::2] - r["U1"][0::2], U2[1::2] - r["U1"][1::2])
+    assert np.median(d) < 0.05 and np.percentile(d, 90) < 0.15, np.percentile(d, [50, 90])

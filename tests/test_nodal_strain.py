@@ -134,3 +134,32 @@ def test_nodal_strain_on_reference_mesh():
     for k, node in enumerate(np.flatnonzero(border)):
         ties = np.flatnonzero(np.abs(np.hypot(*(cg2 - c[node]).T) - dn[k]) < 1e-9)
         assert n2[node, 0] in vals[ties]
+
+
+def test_al_driver_pieces():
+    """The AL driver's own arithmetic: the thin-plate refit reproduces the
+    finite nodes and fills the others exactly for a linear field; the beta
+    pick is the vertex of the parabola through the least sum and its two
+    neighbours, or the least sum itself at either end of the list."""
+    from staq_dic_amd.aldic import rbf_refit, tune_beta
+    m = _mesh()
+    c = m["coordinatesFEM"]
+    N = c.shape[0]
+    U = np.empty(2 * N)
+    U[0::2] = 0.3 + 0.01 * c[:, 0] - 0.02 * c[:, 1]
+    U[1::2] = -1.0 + 0.005 * c[:, 1]
+    F = np.tile([0.01, 0.0, -0.02, 0.005], N).astype(float)
+    Un = U.copy()
+    Un[2 * 100:2 * 110] = np.nan
+    Fn = F.copy()
+    Fn[4 * 50:4 * 60] = np.nan
+    Ur, Fr = rbf_refit(Un, Fn, c)
+    np.testing.assert_allclose(Ur, U, atol=1e-8)
+    np.testing.assert_allclose(Fr, F, atol=1e-10)
+    bl = np.array([1e-3, 1e-2, 1e-1]) * 256 * 1e-3
+    x = np.log10(bl)
+    xv = x[1] + 0.3
+    es = (x - xv) ** 2 - 1.0
+    assert abs(np.log10(tune_beta(es, bl)) - xv) < 1e-12
+    assert tune_beta(np.array([0.0, 1.0, 2.0]), bl) == bl[0]
+    assert tune_beta(np.array([2.0, 1.0, 0.0]), bl) == bl[2]

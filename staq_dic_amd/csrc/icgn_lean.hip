@@ -1,8 +1,8 @@
 // Lean wavefront-per-subset IC-GN kernel for subsets whose window holds no
 // masked pixel (gfx950).  It is the unmasked path of icgn_wave.hip and nothing
 // else, which keeps its register file small (several waves per SIMD instead of
-// one) and its iteration short (it is latency-bound: one subset's iterations
-// are a dependent chain):
+// one, so that the dependent chains of many subsets' iterations share each
+// SIMD) and its iteration short (fewer vector instructions per pixel):
 //
 //   * no bwselect regions, no window growth, no FP64 residual pass: a subset
 //     that meets any of them (a zero of f*mask in its window, a zero of the

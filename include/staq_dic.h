@@ -128,6 +128,19 @@ int staq_xcorr_full_peak(staq_ctx *ctx, int slot, const long box_f[4], const lon
 int staq_xcorr_boxes(staq_ctx *ctx, int slot, long nbox, const long *box_f, const long *box_g,
                      long pad, double *out, int *ok);
 
+/* staq_natural_interp (host only, no device work): Sibson natural-neighbour
+ * interpolation at interior query points, the 'natural' method of MATLAB's
+ * scatteredInterpolant as funGlobalNodalStrainQuadtree.m:285-292 and
+ * LocalICGNQuadtree.m:313-330 call it.  sites n_sites x 2 and values
+ * n_sites x m (row-major); tri / nbr: the Delaunay triangles (3 site indices
+ * each) and their neighbours (-1 at the hull); vptr / vadj: each site's
+ * Delaunay neighbours (CSR); qtri: the triangle holding each query (queries on
+ * a site, on the hull or outside it are the caller's).  out: n_q x m. */
+int staq_natural_interp(long n_sites, const double *sites, long n_tri, const int *tri,
+                        const int *nbr, const long *vptr, const int *vadj, long m,
+                        const double *values, long n_q, const double *queries, const int *qtri,
+                        double *out);
+
 /* Raw subset outputs of LocalICGNQuadtree (func_quadtree/LocalICGNQuadtree.m:63-147)
  * or, with params->plain, LocalICGN (func/LocalICGN.m:60-115), before the
  * bad-subset classification: U (2N), F (4N), Hpar (N x 21), status (N).

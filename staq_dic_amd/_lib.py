@@ -53,6 +53,8 @@ SIGNATURES = {
                                       C.c_long, C.c_long, _lp, _lp, _dp, _dp, _dp, _dp, _dp]),
     "staq_xcorr_full_peak": (C.c_int, [_vp, C.c_int, _lp, _lp, _dp]),
     "staq_xcorr_boxes": (C.c_int, [_vp, C.c_int, C.c_long, _lp, _lp, C.c_long, _dp, _ip]),
+    "staq_natural_interp": (C.c_int, [C.c_long, _dp, C.c_long, _ip, _ip, _lp, _ip, C.c_long, _dp,
+                                      C.c_long, _dp, _ip, _dp]),
     "staq_local_icgn": (C.c_int, [_vp, C.POINTER(StaqParams), C.c_long, _dp, _dp, _lp, _ip, _dp,
                                   _dp, _dp, _ip]),
     "staq_subpb1": (C.c_int, [_vp, C.POINTER(StaqParams), C.c_long, _dp, _dp, _dp, _dp, _dp, _lp,

@@ -35,6 +35,10 @@ from scipy.spatial import Delaunay, cKDTree
 
 __all__ = ["natural_nearest", "sibson_weights"]
 
+# ``_sibson`` below is the restatement in Python; the same steps run natively
+# (staq_natural_interp) for the interior queries of ``natural_nearest``, and
+# the tests hold the two to each other.
+
 
 def _clip(poly, ax, ay, b):
     """Convex polygon (list of (x, y)) cut to the half-plane ax x + ay y <= b.
@@ -177,9 +181,37 @@ def sibson_weights(sites, q):
     return _sibson(T, q, s0)
 
 
+def _native_interior(T, V, Q, simp):
+    """Sibson interpolation of the interior queries Q (triangles simp) by
+    staq_natural_interp in libstaqdic.so (the same steps as ``_sibson``)."""
+    import ctypes as C
+
+    from . import _lib
+    L = _lib.load()
+    tri = np.ascontiguousarray(T.tri.simplices, dtype=np.int32)
+    nbr = np.ascontiguousarray(T.tri.neighbors, dtype=np.int32)
+    ptr, adj = T.tri.vertex_neighbor_vertices
+    ptr = np.ascontiguousarray(ptr, dtype=np.int64)
+    adj = np.ascontiguousarray(adj, dtype=np.int32)
+    P = np.ascontiguousarray(T.p, dtype=np.float64)
+    Vc = np.ascontiguousarray(V, dtype=np.float64)
+    Qc = np.ascontiguousarray(Q, dtype=np.float64)
+    st = np.ascontiguousarray(simp, dtype=np.int32)
+    out = np.empty((Qc.shape[0], Vc.shape[1]))
+    dp = lambda a: a.ctypes.data_as(C.POINTER(C.c_double))
+    ip = lambda a: a.ctypes.data_as(C.POINTER(C.c_int))
+    rc = L.staq_natural_interp(P.shape[0], dp(P), tri.shape[0], ip(tri), ip(nbr),
+                               ptr.ctypes.data_as(C.POINTER(C.c_long)), ip(adj), Vc.shape[1],
+                               dp(Vc), Qc.shape[0], dp(Qc), ip(st), dp(out))
+    if rc != 0:
+        raise _lib.StaqError(f"staq_natural_interp failed ({rc})")
+    return out
+
+
 def natural_nearest(sites, values, queries):
     """scatteredInterpolant(sites, values, 'natural', 'nearest') evaluated at
-    queries: sites (n, 2), values (n,) or (n, m), queries (k, 2)."""
+    queries: sites (n, 2), values (n,) or (n, m), queries (k, 2).  Interior
+    queries go to the native routine in libstaqdic.so in one call."""
     sites = np.asarray(sites, dtype=np.float64)
     V = np.asarray(values, dtype=np.float64)
     squeeze = V.ndim == 1
@@ -192,6 +224,7 @@ def natural_nearest(sites, values, queries):
     T = _Triangulation(sites)
     simp = T.tri.find_simplex(Q)
     dist, near = T.kd.query(Q, k=1)
+    interior = []
     for k in range(Q.shape[0]):
         q = Q[k]
         if dist[k] <= 1e-12 * T.span:
@@ -207,7 +240,9 @@ def natural_nearest(sites, values, queries):
             i, j, t = edge
             out[k] = (1.0 - t) * V[i] + t * V[j]
             continue
-        idx, w = _sibson(T, q, int(simp[k]))
-        out[k] = w @ V[idx]
+        interior.append(k)
+    if interior:
+        ki = np.asarray(interior)
+        out[ki] = _native_interior(T, V, Q[ki], simp[ki])
     return out[:, 0] if squeeze else out
 

@@ -163,3 +163,30 @@ def test_al_driver_pieces():
     assert abs(np.log10(tune_beta(es, bl)) - xv) < 1e-12
     assert tune_beta(np.array([0.0, 1.0, 2.0]), bl) == bl[0]
     assert tune_beta(np.array([2.0, 1.0, 0.0]), bl) == bl[2]
+
+
+def test_native_sibson_matches_restatement():
+    """staq_natural_interp (libstaqdic.so, host code) against the Python
+    restatement of the same steps: equal up to the rounding of the clipped
+    cells (the order of the bisector cuts differs)."""
+    from staq_dic_amd import natural
+    rng = np.random.default_rng(5)
+    S = rng.uniform(0, 10, (500, 2))
+    V = rng.normal(size=(500, 3))
+    Q = rng.uniform(1, 9, (300, 2))
+    T = natural._Triangulation(S)
+    simp = T.tri.find_simplex(Q)
+    assert (simp >= 0).all()
+    nat = natural._native_interior(T, V, Q, simp)
+    for k in range(Q.shape[0]):
+        idx, w = natural._sibson(T, Q[k], int(simp[k]))
+        np.testing.assert_allclose(nat[k], w @ V[idx], rtol=0, atol=1e-8)
+    # a lattice, whose co-circular quadruples make the triangulation's choice
+    # arbitrary: the weights are not
+    g = np.arange(8.0)
+    X, Y = np.meshgrid(g, g, indexing="ij")
+    L = np.stack([X.ravel(), Y.ravel()], axis=1)
+    TL = natural._Triangulation(L)
+    q = np.array([[2.5, 3.5], [4.25, 1.75]])
+    out = natural._native_interior(TL, (L[:, :1] * 2 - L[:, 1:]), q, TL.tri.find_simplex(q))
+    np.testing.assert_allclose(out[:, 0], 2 * q[:, 0] - q[:, 1], atol=1e-9)

@@ -344,8 +344,8 @@ def main():
                          "three times, the known-size kernel that scales the HBM counters")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(json.dumps({"error": "this build runs on one GPU; multi-GPU sharding is described "
-                                   "in DESIGN.md but not launched", "n_gpus": args.gpus}))
+        print(json.dumps({"error": "this implementation runs on one GPU; sharding the subsets "
+                                   "across GPUs is designed but not launched", "n_gpus": args.gpus}))
         sys.exit(2)
 
     import torch

@@ -17,8 +17,11 @@ the integer search; the adjoint ZNSSD forward+backward on the 2048^2
 sinusoidal pair; the multi-instance batch (16 pairs of 512^2-2048^2 in one
 graph, per-pair latency percentiles); the large-deformation incremental pair
 (4096^2 16-bit, ~100 k subsets, integer search w 64 r 48); the accuracy
-against the analytic fields; the roofline of the step's dominant launch and
-the CPU baseline (the FP64 oracle on the host cores).
+against the analytic fields; one frame pair through the incremental driver's
+AL loop on the reference's own frames; the roofline of the step's dominant
+launch; the CPU baseline (the FP64 oracle on the host cores, also on one core,
+and its integer search) and the host times of the global step and the nodal
+strain.
 """
 from __future__ import annotations
 

@@ -754,3 +754,55 @@ def test_znssd_sinusoid_field(ctx, oracle):
         _, cm, _, _ = oracle.znssd(of, Pm, coords[sel], win[sel], want_grad=False)
         fd = (cp - cm) / (2 * h)
         np.testing.assert_allclose(g[sel, q], fd, rtol=0.02, atol=2e-3 * np.abs(fd).max())
+
+
+def test_two_contexts_from_two_threads():
+    """Distinct contexts used from distinct host threads at the same time (the
+    threading rule of include/staq_dic.h: a context is not re-entrant,
+    distinct contexts are independent): each thread's local steps equal the
+    same steps run alone, bit for bit."""
+    import threading
+
+    from staq_dic_amd import Context, synth
+    from staq_dic_amd.api import local_icgn_raw
+    cases = []
+    for seed in (1101, 1102):
+        fr, coords, win = synth.config_hole(seed=seed, H=384, W=384, n_speckles=1700, max_disp=3.0,
+                                            radius=60, margin=24, band=(96, 40))
+        U0 = np.round(synth.truth_at(fr.field, coords)[:, 4:6]).reshape(-1)
+        cases.append((fr, coords, win, U0))
+    _print("two contexts: each case alone")
+    alone = []
+    for fr, coords, win, U0 in cases:
+        c = Context(0)
+        try:
+            alone.append(local_icgn_raw(U0, coords, win, fr.f, fr.g, fr.mask, tol=1e-3, ctx=c))
+        finally:
+            c.close()
+    _print("two contexts: both at once from two threads, three rounds each")
+    results = [[], []]
+    errors = []
+
+    def work(i):
+        fr, coords, win, U0 = cases[i]
+        c = Context(0)
+        try:
+            for _ in range(3):
+                results[i].append(local_icgn_raw(U0, coords, win, fr.f, fr.g, fr.mask, tol=1e-3,
+                                                 ctx=c))
+        except Exception as e:  # reported below
+            errors.append(e)
+        finally:
+            c.close()
+
+    th = [threading.Thread(target=work, args=(i,)) for i in range(2)]
+    for t in th:
+        t.start()
+    for t in th:
+        t.join(timeout=120)
+    assert not errors, errors
+    for i in range(2):
+        assert len(results[i]) == 3
+        for out in results[i]:
+            for a, b in zip(out, alone[i]):
+                np.testing.assert_array_equal(a, b)

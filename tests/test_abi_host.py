@@ -123,3 +123,38 @@ def test_bench_gpus_flag_exits_cleanly():
     r = subprocess.run([sys.executable, os.path.join(ROOT, "bench.py"), "--gpus", "2"],
                        capture_output=True, text=True, timeout=300)
     assert r.returncode == 2 and "one GPU" in r.stdout
+
+
+def test_bad_arguments_without_a_device(lib):
+    """Calls with a null context or batch, or impossible sizes, return
+    STAQ_EINVAL (-1) before any device work; the host-only interpolation
+    entry point rejects too few sites and a query triangle out of range."""
+    import ctypes as C
+
+    import numpy as np
+    L = lib
+    null = C.c_void_p()
+    assert L.staq_local_icgn(null, None, 0, None, None, None, None, None, None, None, None) == -1
+    assert L.staq_batch_icgn6(null, None) == -1
+    assert L.staq_batch_icgn2(null, None) == -1
+    assert L.staq_batch_graph_launch(null) == -1
+    dp = lambda a: a.ctypes.data_as(C.POINTER(C.c_double))
+    ip = lambda a: a.ctypes.data_as(C.POINTER(C.c_int))
+    sites = np.array([[0.0, 0.0], [1.0, 0.0], [0.0, 1.0]])
+    tri = np.array([[0, 1, 2]], dtype=np.int32)
+    nbr = np.full((1, 3), -1, dtype=np.int32)
+    ptr = np.array([0, 2, 4, 6], dtype=np.int64)
+    adj = np.array([1, 2, 0, 2, 0, 1], dtype=np.int32)
+    vals = np.array([1.0, 2.0, 3.0])
+    q = np.array([[0.25, 0.25]])
+    out = np.zeros(1)
+    lp = ptr.ctypes.data_as(C.POINTER(C.c_long))
+    assert L.staq_natural_interp(2, dp(sites), 1, ip(tri), ip(nbr), lp, ip(adj), 1, dp(vals), 1,
+                                 dp(q), ip(np.array([0], dtype=np.int32)), dp(out)) == -1
+    assert L.staq_natural_interp(3, dp(sites), 1, ip(tri), ip(nbr), lp, ip(adj), 1, dp(vals), 1,
+                                 dp(q), ip(np.array([5], dtype=np.int32)), dp(out)) == -1
+    # a lone triangle: the call succeeds and reproduces the linear field
+    # through its corners (Sibson's coordinates have linear precision)
+    assert L.staq_natural_interp(3, dp(sites), 1, ip(tri), ip(nbr), lp, ip(adj), 1, dp(vals), 1,
+                                 dp(q), ip(np.array([0], dtype=np.int32)), dp(out)) == 0
+    assert abs(out[0] - (1.0 + 0.25 + 2 * 0.25)) < 1e-9

@@ -806,3 +806,31 @@ def test_two_contexts_from_two_threads():
         for out in results[i]:
             for a, b in zip(out, alone[i]):
                 np.testing.assert_array_equal(a, b)
+
+
+def test_bad_arguments_with_a_context(ctx):
+    """Argument errors through a live context raise with the library's
+    message and leave the context usable: no subsets, a window below 2 px,
+    an unknown kernel form, a graph launch before any capture."""
+    from staq_dic_amd import Batch, StaqError, params, synth
+    from staq_dic_amd.api import local_icgn_raw
+    fr, coords, win = synth.config_plain(H=256, W=256, n_speckles=800)
+    ctx.set_frame(fr.f, fr.g, fr.mask)
+    with pytest.raises(StaqError, match="bad batch arguments"):
+        Batch(ctx, coords[:0], win[:0])
+    with pytest.raises(StaqError):
+        Batch(ctx, coords[:4], np.ones(4, dtype=np.int64))
+    b = Batch(ctx, coords, win)
+    try:
+        with pytest.raises(StaqError, match="no captured graph"):
+            b.graph_launch()
+        with pytest.raises(StaqError):
+            b.set_forms([3] * len(b.bins()), 6)
+        with pytest.raises(StaqError, match="block_size"):
+            b.icgn6(params(tol=1e-3, block_size=3))
+    finally:
+        b.close()
+    # the context still runs a step
+    U0 = np.round(synth.truth_at(fr.field, coords)[:, 4:6]).reshape(-1)
+    U, F, H, st = local_icgn_raw(U0, coords, win, fr.f, fr.g, fr.mask, tol=1e-3, ctx=ctx)
+    assert ((st >= 1) & (st <= 100)).mean() > 0.8

@@ -834,3 +834,50 @@ def test_bad_arguments_with_a_context(ctx):
     U0 = np.round(synth.truth_at(fr.field, coords)[:, 4:6]).reshape(-1)
     U, F, H, st = local_icgn_raw(U0, coords, win, fr.f, fr.g, fr.mask, tol=1e-3, ctx=ctx)
     assert ((st >= 1) & (st <= 100)).mean() > 0.8
+
+
+def test_full_size_workloads_against_oracle(ctx, oracle):
+    """The measured workloads at their full sizes: the 1024^2 hole pair with
+    the reference's 32-px windows (6016 subsets) against the oracle subset by
+    subset, with the 0.01 px accuracy gate over the subsets the reference
+    keeps; the 4096^2 16-bit large-deformation pair (about 98 k subsets, one
+    batch) against the oracle on a random sample of 3000 subsets."""
+    from staq_dic_amd import Batch, params, synth
+    from staq_dic_amd.api import classify_local
+    from staq_dic_amd._lib import IN_U0
+    fr, coords, _ = synth.config_hole()
+    win = np.full(len(coords), 32, dtype=np.int64)
+    truth = synth.truth_at(fr.field, coords)
+    U0 = np.round(truth[:, 4:6]).reshape(-1)
+    ctx.set_frame(fr.f, fr.g, fr.mask, slot=0)
+    _print("hole pair 1024^2, reference windows, N =", len(coords))
+    b = Batch(ctx, coords, win)
+    b.set_input(IN_U0, U0)
+    b.icgn6(params(tol=1e-3))
+    Ug, Fg, _, sg = b.outputs()
+    b.close()
+    Uo, Fo, _, so = oracle.local_icgn(oracle.Frame(fr.f, fr.g, fr.mask), U0, coords, win, tol=1e-3)
+    assert _cmp_icgn(sg, so, Ug, Uo, Fg, Fo) > 0.95 * len(coords)
+    bad, _ = classify_local(sg)
+    kept = (sg >= 1) & (sg <= 100) & ~bad
+    err = np.hypot(Ug[0::2] - truth[:, 4], Ug[1::2] - truth[:, 5])[kept]
+    assert np.sqrt(np.mean(err ** 2)) <= 0.01
+
+    fr, coords, win = synth.config_large()
+    truth = synth.truth_at(fr.field, coords)
+    U0 = np.round(truth[:, 4:6]).reshape(-1)
+    ctx.set_frame(fr.f, fr.g, fr.mask, slot=0)
+    _print("large-deformation pair 4096^2, N =", len(coords))
+    assert len(coords) > 90000
+    b = Batch(ctx, coords, win)
+    b.set_input(IN_U0, U0)
+    b.icgn6(params(tol=1e-3))
+    Ug, Fg, _, sg = b.outputs()
+    b.close()
+    pick = np.sort(np.random.default_rng(7).choice(len(coords), 3000, replace=False))
+    Uo, Fo, _, so = oracle.local_icgn(oracle.Frame(fr.f, fr.g, fr.mask),
+                                      U0.reshape(-1, 2)[pick].reshape(-1), coords[pick], win[pick],
+                                      tol=1e-3)
+    Ugs = Ug.reshape(-1, 2)[pick].reshape(-1)
+    Fgs = Fg.reshape(-1, 4)[pick].reshape(-1)
+    assert _cmp_icgn(sg[pick], so, Ugs, Uo, Fgs, Fo) > 0.9 * len(pick)

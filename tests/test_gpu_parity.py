@@ -881,3 +881,63 @@ def test_full_size_workloads_against_oracle(ctx, oracle):
     Ugs = Ug.reshape(-1, 2)[pick].reshape(-1)
     Fgs = Fg.reshape(-1, 4)[pick].reshape(-1)
     assert _cmp_icgn(sg[pick], so, Ugs, Uo, Fgs, Fo) > 0.9 * len(pick)
+
+
+def test_full_size_multi_pair_and_adjoint(ctx, oracle):
+    """The multi-instance batch at full size (16 pairs of 512^2-2048^2 in
+    slots 0-15, 88 k subsets, one batch) against the oracle on 150 sampled
+    subsets of every pair; the adjoint pair at full size (2048^2, 15 625
+    subsets) against the FP64 oracle's cost and gradient on 2000 sampled
+    subsets."""
+    import torch
+
+    from staq_dic_amd import Batch, params, synth
+    from staq_dic_amd._lib import IN_U0
+    from staq_dic_amd.autograd import znssd
+    pairs = synth.config_multi()
+    for j, (fr, _, _) in enumerate(pairs):
+        ctx.set_frame(fr.f, fr.g, fr.mask, slot=j)
+    coords = np.concatenate([c for _, c, _ in pairs])
+    win = np.concatenate([w for _, _, w in pairs])
+    slot = np.concatenate([np.full(len(c), j, dtype=np.int32) for j, (_, c, _) in enumerate(pairs)])
+    U0 = np.concatenate([np.round(synth.truth_at(fr.field, c)[:, 4:6]) for fr, c, _ in pairs])
+    _print("multi-pair batch, N =", len(coords))
+    b = Batch(ctx, coords, win, slot=slot)
+    b.set_input(IN_U0, U0.reshape(-1))
+    b.icgn6(params(tol=1e-3))
+    Ug, Fg, _, sg = b.outputs()
+    b.close()
+    rng = np.random.default_rng(11)
+    n_ok = 0
+    for j, (fr, c, w) in enumerate(pairs):
+        idx = np.flatnonzero(slot == j)
+        pick = np.sort(rng.choice(idx, min(150, idx.size), replace=False))
+        Uo, Fo, _, so = oracle.local_icgn(oracle.Frame(fr.f, fr.g, fr.mask), U0[pick].reshape(-1),
+                                          coords[pick], win[pick], tol=1e-3)
+        n_ok += _cmp_icgn(sg[pick], so, Ug.reshape(-1, 2)[pick].reshape(-1), Uo,
+                          Fg.reshape(-1, 4)[pick].reshape(-1), Fo, max_status_mismatch=0.02)
+    assert n_ok > 0.9 * 150 * len(pairs)
+
+    fr, coords, win = synth.config_adjoint()
+    truth = synth.truth_at(fr.field, coords)
+    P0 = truth + np.random.default_rng(7).normal(0, [0.002, 0.002, 0.002, 0.002, 0.1, 0.1],
+                                                 truth.shape)
+    ctx.set_frame(fr.f, fr.g, fr.mask, slot=0)
+    _print("adjoint pair 2048^2, N =", len(coords))
+    dev = torch.device("cuda", 0)
+    P = torch.tensor(P0, dtype=torch.float64, device=dev, requires_grad=True)
+    cost, status = znssd(P, torch.tensor(coords, device=dev),
+                         torch.tensor(win, dtype=torch.int32, device=dev), ctx)
+    cost.sum().backward()
+    torch.cuda.synchronize()
+    pick = np.sort(np.random.default_rng(3).choice(len(coords), 2000, replace=False))
+    _, co, go, so = oracle.znssd(oracle.Frame(fr.f, fr.g, fr.mask), P0[pick], coords[pick],
+                                 win[pick])
+    st = status.cpu().numpy()[pick]
+    np.testing.assert_array_equal(st, so)
+    ok = so == 0
+    assert ok.mean() > 0.95
+    np.testing.assert_allclose(cost.detach().cpu().numpy()[pick][ok], co[ok], rtol=1e-4, atol=1e-6)
+    g = P.grad.cpu().numpy()[pick]
+    scale = np.abs(go[ok]).max(axis=0) + 1e-12
+    assert np.max(np.abs(g[ok] - go[ok]) / scale) < 2e-3

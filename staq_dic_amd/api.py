@@ -252,16 +252,19 @@ def _hole_nodes(coords, winsize, f, mask):
     """markCoordHoleStrain (LocalICGNQuadtree.m:71-82): nodes whose window holds
     a zero of f*mask, or cannot be read.  1-based indices."""
     H, W = f.shape
-    out = []
-    fm = f * mask
-    for j, (x, y) in enumerate(np.round(coords).astype(np.int64)):
-        h = int(winsize[j]) // 2
-        if x - h < 1 or y - h < 1 or x + h > H or y + h > W:
-            out.append(j + 1)
-            continue
-        if np.any(fm[x - h - 1:x + h, y - h - 1:y + h] == 0):
-            out.append(j + 1)
-    return np.asarray(out, dtype=np.int64)
+    # zeros of f*mask counted over each window from one summed-area table
+    z = np.zeros((H + 1, W + 1), dtype=np.int64)
+    z[1:, 1:] = np.cumsum(np.cumsum(np.asarray(f * mask) == 0, axis=0), axis=1)
+    xy = np.round(np.asarray(coords, dtype=np.float64)).astype(np.int64)
+    x, y = xy[:, 0], xy[:, 1]
+    h = np.asarray(winsize, dtype=np.int64) // 2
+    outside = (x - h < 1) | (y - h < 1) | (x + h > H) | (y + h > W)
+    a0 = np.clip(x - h - 1, 0, H)
+    a1 = np.clip(x + h, 0, H)
+    b0 = np.clip(y - h - 1, 0, W)
+    b1 = np.clip(y + h, 0, W)
+    zeros = z[a1, b1] - z[a0, b1] - z[a1, b0] + z[a0, b0]
+    return (np.flatnonzero(outside | (zeros > 0)) + 1).astype(np.int64)
 
 
 def LocalICGNQuadtree(U0, coordinatesFEM, Df, ImgRef, ImgDef, DICpara, ICGNmethod="GaussNewton",

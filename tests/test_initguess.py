@@ -176,3 +176,37 @@ def test_al_frame_on_reference_frames(ctx, ref_frames):
     assert bl[0] <= out["beta"] <= bl[-1]
     d = np.hypot(U2[0::2] - r["U1"][0::2], U2[1::2] - r["U1"][1::2])
     assert np.median(d) < 0.05 and np.percentile(d, 90) < 0.15, np.percentile(d, [50, 90])
+
+
+@pytest.mark.gpu
+def test_front_half_with_multigrid_search(ctx, ref_frames):
+    """The same chain with the multigrid search (InitFFTSearchMethod 0, the
+    incremental driver's choice): the uniform mesh and the hole NaNs as with
+    the whole-field search, and a guess within a tenth of a pixel (median) of
+    the stored answer off the hole, which the local step then takes to the
+    stored answer."""
+    from staq_dic_amd import LocalICGNQuadtree, classify_local
+    from staq_dic_amd.initguess import uniform_initial_guess
+    from staq_dic_amd.mesh import GenerateQuadtreeMesh, transfer_initial_guess
+    r = ref_frames
+    H, W = r["f"].shape
+    P = {"winsize": 32, "winstepsize": 16, "InitFFTSearchMethod": 0,
+         "gridxyROIRange": {"gridx": [1, H], "gridy": [1, W]}, "ImgRefMask": r["mask"]}
+    print("uniform initial guess with the multigrid search", flush=True)
+    P2, mesh0, U0, _, cc = uniform_initial_guess(r["f"], r["g"], P, r["mask"], r["mask"], ctx=ctx)
+    c0 = mesh0["coordinatesFEM"]
+    np.testing.assert_array_equal(c0, r["coords"][:529])
+    hole = r["mask"][c0[:, 0].astype(int) - 1, c0[:, 1].astype(int) - 1] == 0
+    assert np.isnan(U0[0::2][hole]).all() and np.isfinite(U0[0::2][~hole]).all()
+    U1 = r["U1"][:1058]
+    d0 = np.hypot(U0[0::2] - U1[0::2], U0[1::2] - U1[1::2])[~hole]
+    print("guess distance percentiles 50/90/100:", np.percentile(d0, [50, 90, 100]), flush=True)
+    assert np.median(d0) < 0.1 and np.percentile(d0, 90) < 0.25
+    c, E, irr, mark = GenerateQuadtreeMesh(c0, mesh0["elementsFEM"], r["mask"], 16, 32)
+    Uq = transfer_initial_guess(U0, c0, c, r["mask"])
+    DICpara = {"winsize": r["winsize"], "winsize_List": r["winsize_List"]}
+    Uf, F, Hp, t, conv, nbad, _ = LocalICGNQuadtree(Uq, c, {"ImgRefMask": r["mask"]}, r["f"],
+                                                    r["g"], DICpara, "GaussNewton", 1e-2, ctx=ctx)
+    bad, _ = classify_local(conv)
+    d = np.hypot(Uf[0::2] - r["U1"][0::2], Uf[1::2] - r["U1"][1::2])
+    assert np.median(d[~bad]) < 5e-3 and np.percentile(d[~bad], 90) < 0.06

@@ -30,7 +30,7 @@ from .mesh import MeshSetUp, uniform_nodes
 from .outliers import inpaint_nans
 from .refill import thinplate_eval, thinplate_fit
 
-__all__ = ["regularizeNd", "Init", "uniform_initial_guess"]
+__all__ = ["regularizeNd", "Init", "mask_nan_nodes", "uniform_initial_guess"]
 
 
 def _histc(x, grid):
@@ -214,6 +214,36 @@ def Init(u, v, Phi, x, y, index=0):
     return U0
 
 
+def mask_nan_nodes(X, Y, u, v, fMask=None, gMask=None):
+    """Nodes of main_ALDIC_Quadtree_v2.m:133-143 that get NaN:
+    ~fMask(x, y) || ~gMask(floor(x + u), floor(y + v)), evaluated inside a
+    try/catch, so a node whose index would raise (not an integer, outside the
+    image, NaN) is skipped; the g test runs only where f's passes.  All
+    arrays nx x ny (ndgrid layout); a mask of None is all ones."""
+    X = np.asarray(X, dtype=np.float64)
+    Y = np.asarray(Y, dtype=np.float64)
+
+    def lookup(mask, a, b):
+        ok = np.isfinite(a) & np.isfinite(b)
+        ok &= (a == np.floor(np.where(ok, a, 0))) & (b == np.floor(np.where(ok, b, 0)))
+        if mask is None:
+            return ok, np.ones(a.shape, dtype=bool)
+        m = np.asarray(mask)
+        ok &= (a >= 1) & (a <= m.shape[0]) & (b >= 1) & (b <= m.shape[1])
+        val = np.zeros(a.shape, dtype=bool)
+        ai = np.where(ok, a, 1).astype(np.int64) - 1
+        bi = np.where(ok, b, 1).astype(np.int64) - 1
+        val[ok] = m[ai[ok], bi[ok]] != 0
+        return ok, val
+
+    f_ok, f_val = lookup(fMask, X, Y)
+    if gMask is None:
+        return f_ok & ~f_val
+    with np.errstate(invalid="ignore"):
+        g_ok, g_val = lookup(gMask, np.floor(X + u), np.floor(Y + v))
+    return f_ok & (~f_val | (g_ok & ~g_val))
+
+
 def uniform_initial_guess(fNormalized, gNormalized, DICpara, fMask=None, gMask=None, ctx=None):
     """The initial guess of main_ALDIC_Quadtree_v2.m:100-141 on the uniform
     mesh.  Returns (DICpara, DICmesh, U0, (x0temp, y0temp, u, v), cc):
@@ -247,33 +277,8 @@ def uniform_initial_guess(fNormalized, gNormalized, DICpara, fMask=None, gMask=N
     U0 = Init(u, v, cc.get("max") if isinstance(cc, dict) else None, x0temp.T, y0temp.T, 0)
     fm = None if fMask is None else np.asarray(fMask)
     gm = None if gMask is None else np.asarray(gMask)
-    nx, ny = u.shape
-    def at(mask, a, b):
-        """mask(a, b) as MATLAB indexes it, None where that would raise."""
-        if not (np.isfinite(a) and np.isfinite(b) and a == int(a) and b == int(b)):
-            return None
-        if not (1 <= a <= mask.shape[0] and 1 <= b <= mask.shape[1]):
-            return None
-        return bool(mask[int(a) - 1, int(b) - 1])
-
-    for i in range(nx):
-        for j in range(ny):
-            xi, yj = x0temp[i, j], y0temp[i, j]
-            # ~fMask(x, y) || ~gMask(floor(x + u), floor(y + v)) inside a
-            # try/catch: an index that raises skips the node
-            hole = False
-            fv = True if fm is None else at(fm, xi, yj)
-            if fv is None:
-                continue
-            if not fv:
-                hole = True
-            elif gm is not None:
-                gv = at(gm, np.floor(xi + u[i, j]), np.floor(yj + v[i, j]))
-                if gv is None:
-                    continue
-                hole = not gv
-            if hole:
-                k = j + i * ny
-                U0[2 * k] = np.nan
-                U0[2 * k + 1] = np.nan
+    hole = mask_nan_nodes(x0temp, y0temp, u, v, fm, gm)
+    k = np.flatnonzero(hole.ravel(order="C"))  # node j + i ny of MeshSetUp's order
+    U0[2 * k] = np.nan
+    U0[2 * k + 1] = np.nan
     return P, DICmesh, U0, (x0temp, y0temp, u, v), cc

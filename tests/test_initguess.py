@@ -210,3 +210,26 @@ def test_front_half_with_multigrid_search(ctx, ref_frames):
     bad, _ = classify_local(conv)
     d = np.hypot(Uf[0::2] - r["U1"][0::2], Uf[1::2] - r["U1"][1::2])
     assert np.median(d[~bad]) < 5e-3 and np.percentile(d[~bad], 90) < 0.06
+
+
+def test_mask_nan_nodes_try_catch_rule():
+    """~fMask(x,y) || ~gMask(floor(x+u), floor(y+v)) inside a try/catch
+    (main_ALDIC_Quadtree_v2.m:133-143), node by node on a hand-made case."""
+    from staq_dic_amd.initguess import mask_nan_nodes
+    fm = np.ones((6, 6), dtype=bool)
+    gm = np.ones((6, 6), dtype=bool)
+    fm[1, 1] = False          # node (2, 2) sits on a hole of f
+    gm[3, 4] = False          # g's hole at (4, 5)
+    X = np.array([[2.0, 2.0, 2.0], [4.0, 4.0, 4.0], [2.5, 6.0, 7.0]])
+    Y = np.array([[2.0, 3.0, 4.0], [3.0, 4.0, 5.0], [2.0, 6.0, 2.0]])
+    u = np.array([[0.0, 0.0, 2.2], [0.9, 0.0, 0.3], [0.0, 1.0, 0.0]])
+    v = np.array([[0.0, 0.0, 1.1], [2.5, np.nan, 0.0], [0.0, 0.0, 0.0]])
+    hole = mask_nan_nodes(X, Y, u, v, fm, gm)
+    expect = np.array([
+        [True, False, True],     # f hole; clear; g(floor 4.2, floor 5.1) = (4, 5) hole
+        [True, False, True],     # g(4, 5) hole; NaN index skipped; g(4, 5) hole
+        [False, False, False],   # x not an integer: skipped; g(7, 6) outside: skipped; x outside f: skipped
+    ])
+    np.testing.assert_array_equal(hole, expect)
+    np.testing.assert_array_equal(mask_nan_nodes(X, Y, u, v, fm, None),
+                                  np.array([[True, False, False], [False] * 3, [False] * 3]))

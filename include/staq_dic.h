@@ -55,10 +55,18 @@ typedef struct staq_params {
                         windows without masked pixels with that many wavefronts (a subset
                         it cannot finish goes to the fallback launch)                    */
     int persistent;  /* 1: persistent grid pulling subsets from a device work queue      */
+    double guard;    /* guard band of the stopping tests: a subset whose |b|/|b_1| falls
+                        within a factor (1 -/+ guard) of tol (|b|: 4 guard), whose warped
+                        corner falls within 1e-3 px of the image test's bounds, or (6-DOF)
+                        whose Hessian is indefinite or has Cholesky pivots spread by more
+                        than 3e3, is solved again with FP64 arithmetic throughout (gather,
+                        statistics, residual), so that FP32 sums never decide a stopping
+                        or bounds test (funICGNQuadtree.m:154-157, 373-386;
+                        funICGN_Subpb1.m:403-417); 0 = off                               */
 } staq_params;
 
 /* Defaults of the reference's drivers: tol 1e-2 (main_ALDIC_Quadtree_v2.m:212),
- * mu 1e-3 (:256), max_iter 100. */
+ * mu 1e-3 (:256), max_iter 100; guard 5e-3. */
 void staq_params_default(staq_params *p);
 
 /* ---- context ---------------------------------------------------------- */
@@ -220,6 +228,10 @@ int staq_batch_autotune(staq_batch *b, const staq_params *p, int mode);
 /* Capture one local step (mode 6 or 2) into a hipGraph, then replay it. */
 int staq_batch_graph_capture(staq_batch *b, const staq_params *p, int mode);
 int staq_batch_graph_launch(staq_batch *b);
+/* Number of subsets the last local step solved again in FP64 because a
+ * stopping or bounds test fell inside the guard band (staq_params.guard).
+ * A measurement aid with no counterpart in the reference. */
+int staq_batch_exact_count(staq_batch *b, long *n);
 /* Device pointers of the batch arrays (for callers that wrap them). */
 typedef struct staq_batch_ptrs {
     double *coords, *U0, *UOld, *FOld, *vdual, *U, *F, *Hpar;

@@ -23,7 +23,8 @@ _vp = C.c_void_p
 
 class StaqParams(C.Structure):
     _fields_ = [("tol", C.c_double), ("mu", C.c_double), ("max_iter", C.c_int),
-                ("plain", C.c_int), ("block_size", C.c_int), ("persistent", C.c_int)]
+                ("plain", C.c_int), ("block_size", C.c_int), ("persistent", C.c_int),
+                ("guard", C.c_double)]
 
 
 class BatchPtrs(C.Structure):
@@ -76,6 +77,7 @@ SIGNATURES = {
     "staq_batch_autotune": (C.c_int, [_vp, C.POINTER(StaqParams), C.c_int]),
     "staq_batch_graph_capture": (C.c_int, [_vp, C.POINTER(StaqParams), C.c_int]),
     "staq_batch_graph_launch": (C.c_int, [_vp]),
+    "staq_batch_exact_count": (C.c_int, [_vp, _lp]),
     "staq_batch_get_ptrs": (C.c_int, [_vp, C.POINTER(BatchPtrs)]),
     "staq_batch_bins": (C.c_int, [_vp, _ip, _ip, _lp, _ip, _ip, _ip, C.c_int]),
     "staq_batch_bin_of": (C.c_int, [_vp, _ip]),
@@ -133,11 +135,16 @@ class StaqError(RuntimeError):
     pass
 
 
-def params(tol=1e-2, mu=1e-3, max_iter=100, plain=False, block_size=0, persistent=False):
+def params(tol=1e-2, mu=1e-3, max_iter=100, plain=False, block_size=0, persistent=False,
+           guard=None):
+    """staq_params with the library's defaults; guard None keeps the default
+    guard band (0 turns the FP64 re-solve off)."""
     p = StaqParams()
     load().staq_params_default(C.byref(p))
     p.tol, p.mu, p.max_iter = tol, mu, max_iter
     p.plain, p.block_size, p.persistent = int(plain), block_size, int(persistent)
+    if guard is not None:
+        p.guard = guard
     return p
 
 
@@ -282,6 +289,12 @@ class Batch:
         self.ctx.check(self.L.staq_batch_get_output(self.h, d(U), d(F), d(Hp), ip(st)),
                        "staq_batch_get_output")
         return U, F, Hp, st
+
+    def exact_count(self) -> int:
+        """Subsets the last step solved again in FP64 (guard band)."""
+        n = C.c_long(0)
+        self.ctx.check(self.L.staq_batch_exact_count(self.h, C.byref(n)), "staq_batch_exact_count")
+        return n.value
 
     def ptrs(self) -> BatchPtrs:
         p = BatchPtrs()

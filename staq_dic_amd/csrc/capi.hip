@@ -116,6 +116,7 @@ struct staq_batch {
     staq_ctx *ctx = nullptr;
     long N = 0;
     DevBuf coords, U0, UOld, FOld, vdual, U, F, H, status, wx, wy, slot, order, ints, handed;
+    DevBuf deflist;  // subsets left to the fallback launch in the current step
     std::vector<Bin> bins;
     // host copies behind the bins: window sizes, zeros of f*mask in each
     // window, the grown window's pixels and pixel slots, and whether the
@@ -172,6 +173,7 @@ void staq_params_default(staq_params *p)
     p->plain = 0;
     p->block_size = 0;
     p->persistent = 0;
+    p->guard = 5e-3;
 }
 
 int staq_ctx_create(staq_ctx **out, int device)
@@ -729,7 +731,7 @@ static int build_bins(staq_batch *b)
         HIPCHK(ctx, hipEventCreateWithFlags(&e, hipEventDisableTiming));
         b->ev.push_back(e);
     }
-    HIPCHK(ctx, b->ints.ensure((nb + 2) * 4));
+    HIPCHK(ctx, b->ints.ensure((nb + 5) * 4));
     HIPCHK(ctx, hipMemcpy(b->order.p, order.data(), N * 4, hipMemcpyHostToDevice));
     return STAQ_OK;
 }
@@ -830,6 +832,7 @@ int staq_batch_create(staq_ctx *ctx, staq_batch **out, long N, const double *coo
     b->general.assign(N, 0);
     for (long i = 0; i < N; ++i) b->general[i] = cnt[i] != 0;
     chk(b->handed.ensure(N * 4), "handed");
+    chk(b->deflist.ensure(N * 4), "defer list");
     if (rc == STAQ_OK) rc = build_bins(b);
     if (rc == STAQ_OK) {
         hipStream_t s = ctx->stream;
@@ -862,7 +865,8 @@ void staq_batch_destroy(staq_batch *b)
     for (hipEvent_t e : b->ev) hipEventDestroy(e);
     for (hipEvent_t e : b->tev) hipEventDestroy(e);
     DevBuf *all[] = {&b->coords, &b->U0, &b->UOld, &b->FOld, &b->vdual, &b->U, &b->F,
-                     &b->H, &b->status, &b->wx, &b->wy, &b->slot, &b->order, &b->ints};
+                     &b->H, &b->status, &b->wx, &b->wy, &b->slot, &b->order, &b->ints,
+                     &b->handed, &b->deflist};
     for (DevBuf *d : all) d->release();
     delete b;
 }
@@ -893,6 +897,19 @@ int staq_batch_get_output(staq_batch *b, double *U, double *F, double *Hpar, int
     if (Hpar) HIPCHK(ctx, hipMemcpyAsync(Hpar, b->H.p, N * 21 * 8, hipMemcpyDeviceToHost, s));
     if (status) HIPCHK(ctx, hipMemcpyAsync(status, b->status.p, N * 4, hipMemcpyDeviceToHost, s));
     HIPCHK(ctx, hipStreamSynchronize(s));
+    return STAQ_OK;
+}
+
+int staq_batch_exact_count(staq_batch *b, long *n)
+{
+    if (!b || !n) return STAQ_EINVAL;
+    staq_ctx *ctx = b->ctx;
+    hipSetDevice(ctx->device);
+    int v = 0;
+    const int nb = (int)b->bins.size();
+    HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+    HIPCHK(ctx, hipMemcpy(&v, b->ints.as<int>() + nb + 2, 4, hipMemcpyDeviceToHost));
+    *n = v;
     return STAQ_OK;
 }
 
@@ -983,11 +1000,19 @@ static IcgnArgs base_args(staq_batch *b, const staq_params *p)
     a.H21 = b->H.as<double>();
     a.status = b->status.as<int>();
     a.handed = b->handed.as<int>();
+    a.defer_list = b->deflist.as<int>();
     a.N = b->N;
     a.tol = p->tol;
     a.mu = p->mu;
     a.plain = p->plain;
     a.max_it = p->max_iter > 0 ? p->max_iter : 100;
+    // the band on |b| / tol, (1 - g, 1 + g), as a band on the squared norms
+    const double g = p->guard > 0 ? p->guard : 0.0;
+    a.guard2 = g > 0 ? g * (2.0 + g) : 0.0;
+    // |b| near tol is small in absolute terms, where the FP32 rounding is a
+    // larger fraction of it: a band four times as wide
+    const double ga = 4.0 * g;
+    a.guard2_abs = ga > 0 ? ga * (2.0 + ga) : 0.0;
     return a;
 }
 
@@ -1012,8 +1037,11 @@ static int enqueue_step(staq_batch *b, const staq_params *p, int mode, hipStream
 {
     staq_ctx *ctx = b->ctx;
     const int nb = (int)b->bins.size();
-    int *ints = b->ints.as<int>();  // [0, nb) bin queue heads, nb fallback head, nb+1 defer flag
-    hipLaunchKernelGGL(reset_ints_kernel, dim3(1), dim3(256), 0, s, ints, nb + 2);
+    // [0, nb) bin queue heads; nb, nb+1: queue head and list length of the small
+    // fallback launch; nb+2: count of subsets left to the FP64 re-solve; nb+3,
+    // nb+4: queue head and list length of the large fallback launch
+    int *ints = b->ints.as<int>();
+    hipLaunchKernelGGL(reset_ints_kernel, dim3(1), dim3(256), 0, s, ints, nb + 5);
     HIPCHK(ctx, hipGetLastError());
     const int mi = mode == 6 ? 0 : 1;
     const bool fork = bin_only < 0 && nb > 1 && (int)b->side.size() == nb - 1 &&
@@ -1034,6 +1062,8 @@ static int enqueue_step(staq_batch *b, const staq_params *p, int mode, hipStream
         a.tile_cap = bin.tile_cap;
         a.defer = 1;
         a.defer_out = ints + nb + 1;
+        a.defer_out2 = ints + nb + 4;
+        a.exact_count = ints + nb + 2;
         int bs = bs_override != kNoOverride ? bs_override
                                              : (p->block_size ? p->block_size : bin.bs[mi]);
         if (!valid_form(bs))
@@ -1052,7 +1082,9 @@ static int enqueue_step(staq_batch *b, const staq_params *p, int mode, hipStream
             a.tile_cap = side * side;
             if (p->persistent) {
                 a.work_counter = ints + k;
-                grid = occupancy_grid(64 * nw, sizeof(float) * a.tile_cap + 8192, bin.n);
+                // the g tile, f of every pixel slot, the exchange buffers
+                grid = occupancy_grid(64 * nw, sizeof(float) * (a.tile_cap + 64 * wppt * nw) + 8192,
+                                      bin.n);
             }
             if (lean)
                 HIPCHK(ctx, icgn_lean_launch(a, mode, wppt, nw, grid, sk));
@@ -1077,22 +1109,32 @@ static int enqueue_step(staq_batch *b, const staq_params *p, int mode, hipStream
         }
     }
     if (bin_only < 0) {
-        // fallback over the whole batch for windows that outgrew their bin;
-        // launched unconditionally (graph-capturable), its workgroups leave at
-        // once when the defer flag is clear
+        // fallback over the whole batch: the FP64 re-solve of the subsets
+        // whose stopping test fell inside the guard band and of the windows
+        // that outgrew their bin; launched unconditionally (graph-capturable),
+        // its workgroups leave at once when the defer flag is clear
+        // two launches: windows of up to 32 px (LDS for their grown windows
+        // only, so that five subsets run per CU), then the larger ones
         IcgnArgs a = base_args(b, p);
         a.item0 = 0;
         a.n_items = b->N;
-        a.npix_cap = b->fb_npix_cap;
-        a.tile_cap = b->fb_tile_cap;
         a.defer = 0;
+        a.guard2 = 0;
+        a.guard2_abs = 0;
         a.only_deferred = 1;
+        if (timed) HIPCHK(ctx, hipEventRecord(b->tev[2 * nb], s));
+        a.npix_cap = std::min(kSmallFallbackPix, b->fb_npix_cap);
+        a.tile_cap = tile_side(48) * tile_side(48);  // the warped support of a 48 x 48 window
         a.work_counter = ints + nb;
         a.defer_flag = ints + nb + 1;
-        const int bs = 256;
-        const int grid = (int)std::min<long>(b->N, 256);
-        if (timed) HIPCHK(ctx, hipEventRecord(b->tev[2 * nb], s));
-        HIPCHK(ctx, icgn_launch(a, mode, bs, grid, s));
+        a.list_back = 0;
+        HIPCHK(ctx, icgn_exact_launch(a, mode, 256, (int)std::min<long>(b->N, 1280), s));
+        a.npix_cap = b->fb_npix_cap;
+        a.tile_cap = 0;  // gathers from HBM
+        a.work_counter = ints + nb + 3;
+        a.defer_flag = ints + nb + 4;
+        a.list_back = 1;
+        HIPCHK(ctx, icgn_exact_launch(a, mode, 256, (int)std::min<long>(b->N, 256), s));
         if (timed) HIPCHK(ctx, hipEventRecord(b->tev[2 * nb + 1], s));
     }
     return STAQ_OK;
@@ -1155,6 +1197,7 @@ int staq_batch_icgn6_per_subset(staq_batch *b, const staq_params *p, double *ker
         a.n_items = 1;
         a.npix_cap = npix_cap;
         a.tile_cap = tile_cap;
+        a.guard2 = a.guard2_abs = 0;  // no fallback launch follows: every subset is finished here
         if (kernel_ms_sum) HIPCHK(ctx, hipEventRecord(e0, s));
         HIPCHK(ctx, icgn_launch(a, 6, bs, 1, s));
         if (kernel_ms_sum) {

@@ -22,8 +22,89 @@ struct FrameDev {
     int H, W;
 };
 
-// Status codes of the reference (funICGNQuadtree.m:422-442, LocalICGNQuadtree.m:92-97).
-enum : int { ST_DEFER = -2, ST_NOTCONV = 101, ST_MASKED = 102, ST_FEATURELESS = 103, ST_ERROR = -1 };
+// Status codes of the reference (funICGNQuadtree.m:422-442, LocalICGNQuadtree.m:92-97),
+// and two internal codes that never leave a local step: ST_DEFER (the window
+// or the warped support outgrew the kernel) and ST_EXACT (a stopping test
+// fell inside the guard band around tol; the subset is solved again with
+// FP64 arithmetic throughout, so that its iteration count is the one an FP64
+// evaluation gives).
+enum : int {
+    ST_EXACT = -3,
+    ST_DEFER = -2,
+    ST_NOTCONV = 101,
+    ST_MASKED = 102,
+    ST_FEATURELESS = 103,
+    ST_ERROR = -1
+};
+
+// Stopping test inside the guard band: x (a squared norm) within a relative
+// band of width g2 (> 0) around tol2 on either side.  FP32 sums place |b|
+// within a small relative error of its FP64 value, so only a test this close
+// to tol can come out differently.
+__device__ __forceinline__ bool near_tol(double x, double tol2, double g2)
+{
+    return g2 > 0 && x > tol2 * (1.0 - g2) && x < tol2 * (1.0 + g2);
+}
+
+// Spread max L_cc^2 / min L_cc^2 of the Cholesky pivots of the 6x6 Hessian,
+// a lower bound of its condition number.  Past kPivotSpread the FP32 sums of
+// H and b can move the solution of an ill-conditioned window (small windows
+// with weak texture along one direction) by more than 1e-3 px from an FP64
+// evaluation, so such subsets take the FP64 re-solve.
+constexpr double kPivotSpread = 3e3;
+__device__ __forceinline__ bool pivots_spread(const double d[6])
+{
+    double lo = d[0] * d[0], hi = lo;
+#pragma unroll
+    for (int q = 1; q < 6; ++q) {
+        lo = fmin(lo, d[q] * d[q]);
+        hi = fmax(hi, d[q] * d[q]);
+    }
+    return !(hi <= kPivotSpread * lo);
+}
+
+// A warped corner within 1e-3 px of either bound of the image test
+// (funICGNQuadtree.m:154-157), where the FP32 sums' effect on P could decide it.
+__device__ __forceinline__ bool near_edge(double u, double lo, double hi)
+{
+    return fabs(u - lo) < 1e-3 || fabs(u - hi) < 1e-3;
+}
+
+// Warp coordinates for the Keys gather, split so that the fraction that
+// decides the weights keeps FP32 precision.  Window pixel (x0 + dx, y0 + dy)
+// warps to u = (x0 + dx + floor(P5)) + s_u with
+//   s_u = P1 dx + P3 dy + (P5 - floor(P5)),
+// where x0 + dx is an integer (the window's corners are integers) and s_u is
+// a few pixels at most; s_u is formed in FP32, so its rounding error is that
+// of a number of magnitude |P1 dx + P3 dy| + 1 rather than of the tile-local
+// or absolute coordinate.  floor(u) = (x0 + dx + floor(P5)) + floor(s_u)
+// exactly, and the Keys fraction is s_u - floor(s_u), taken with the
+// hardware fract (v_fract_f32: the same value, clamped below 1).  v likewise.
+struct SplitWarp {
+    float p1, p2, p3, p4;  // P[0..3] in FP32
+    float fu, fv;          // fractional parts of P5, P6 (in [0, 1])
+    // cu = x0 - tu0 + floor(P5) - 1: the 0-based tile index of the first
+    // tap of window offset dx is dx + cu + floor(s_u) (cu is integer- or
+    // half-integer-valued like -dx, so the sum is exact in FP32); cv likewise
+    float cu, cv;
+};
+
+__device__ __forceinline__ SplitWarp split_warp(const double P[6], double x0, double y0, int tu0,
+                                                int tv0)
+{
+    SplitWarp s;
+    s.p1 = (float)P[0];
+    s.p2 = (float)P[1];
+    s.p3 = (float)P[2];
+    s.p4 = (float)P[3];
+    const double ip5 = floor(P[4]), ip6 = floor(P[5]);
+    s.fu = (float)(P[4] - ip5);
+    s.fv = (float)(P[5] - ip6);
+    s.cu = (float)(x0 - tu0 + ip5 - 1.0);
+    s.cv = (float)(y0 - tv0 + ip6 - 1.0);
+    return s;
+}
+
 
 constexpr int kWave = 64;
 
@@ -135,6 +216,28 @@ __device__ __forceinline__ void keys_dw(T t, T w[4])
     w[1] = T(0.5) * (T(-10) * t + T(9) * t2);
     w[2] = T(0.5) * (T(1) + T(8) * t - T(9) * t2);
     w[3] = T(0.5) * (T(-2) * t + T(3) * t2);
+}
+
+// Keys gather of window offset (dx, dy) from a tile of row pitch tnx (the
+// taps of the 4 x 4 support, weights from the split coordinates).
+__device__ __forceinline__ float split_gather(const SplitWarp &s, const float *tile, int tnx,
+                                              float dx, float dy)
+{
+    const float su = fmaf(s.p1, dx, fmaf(s.p3, dy, s.fu));
+    const float sv = fmaf(s.p2, dx, fmaf(s.p4, dy, s.fv));
+    const float fsu = floorf(su), fsv = floorf(sv);
+    float wu[4], wv[4];
+    keys_w(__builtin_amdgcn_fractf(su), wu);
+    keys_w(__builtin_amdgcn_fractf(sv), wv);
+    const float iuf = (dx + s.cu) + fsu, ivf = (dy + s.cv) + fsv;
+    const float *b = tile + (int)fmaf(ivf, (float)tnx, iuf);
+    float g = 0.f;
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        const float *c4 = b + q * tnx;
+        g = fmaf(wv[q], fmaf(wu[0], c4[0], fmaf(wu[1], c4[1], fmaf(wu[2], c4[2], wu[3] * c4[3]))), g);
+    }
+    return g;
 }
 
 template <typename T>
@@ -315,13 +418,39 @@ __device__ inline void solve_factored(const double *A, const int *piv, int kind,
     }
 }
 
+// Statistics of one iteration from the residual-form sums
+// acc = [sum r, sum r q, sum J_q r] over the window, r = g - kf f - ck and
+// q = g + kf f - c1k (ck, c1k: (1 -/+ kf) cf rounded to FP32), and the
+// set-up's sums about cf (F1c = sum (f - cf), F2c = sum (f - cf)^2,
+// FJc = sum J (f - cf)): dg = gbar - cf, var = sum (g - gbar)^2 and
+// T_q = sum J_q (g - cf), all in FP64.  With a = g - cf, e = f - cf:
+// r = a - kf e - corr, q = a + kf e - corr1, so sum a, sum a^2 and
+// sum J a follow exactly from the FP32 sums of the small r and r q.
+template <int NJ>
+__device__ __forceinline__ void residual_stats(double n, const float *acc, float kf, float ck,
+                                               float c1k, float cf, double F1c, double F2c,
+                                               const double *FJc, const double *SJ, double &dg,
+                                               double &var, double (&T)[NJ])
+{
+    const double k = kf;
+    const double corr = (double)ck - (1.0 - k) * (double)cf;
+    const double corr1 = (double)c1k - (1.0 + k) * (double)cf;
+    const double S1 = (double)acc[0] + k * F1c + n * corr;
+    const double S2 = (double)acc[1] + k * k * F2c + corr1 * (S1 - k * F1c) +
+                      corr * (S1 + k * F1c) - n * corr * corr1;
+    dg = S1 / n;
+    var = S2 - S1 * dg;
+#pragma unroll
+    for (int q = 0; q < NJ; ++q) T[q] = (double)acc[2 + q] + k * FJc[q] + corr * SJ[q];
+}
+
 // Inverse-compositional update W(P) <- W(P) o W(dP)^-1 with the closed-form
 // inverse of funICGNQuadtree.m:393-411.  Returns false if det(dP) == 0.
 __device__ inline bool ic_compose(double P[6], const double d[6])
 {
     const double det = (1 + d[0]) * (1 + d[3]) - d[1] * d[2];
     if (!(det != 0)) return false;
-    const double id = __builtin_amdgcn_rcp(det);
+    const double id = 1.0 / det;
     const double q1 = (-d[0] - d[0] * d[3] + d[1] * d[2]) * id;
     const double q2 = -d[1] * id;
     const double q3 = -d[2] * id;
@@ -353,7 +482,7 @@ __device__ __forceinline__ void block_sum(T (&v)[K], T *red, T (&out)[K])
         if constexpr (sizeof(T) == 4)
             s = wave_sum_dpp(v[k]);
         else
-            s = wave_sum(v[k]);
+            s = wave_sum_dpp_f64(v[k]);
         if (lane == 0) red[wid * K + k] = s;
     }
     __syncthreads();

@@ -25,6 +25,8 @@
 //   masked  - subsets that touch the mask (or whose g has exact zeros) take
 //             the exact path: bwselect flood fills in LDS and two-pass FP64
 //             statistics, following the reference line by line.
+#include <type_traits>
+
 #include "kernels.h"
 
 namespace staq {
@@ -44,7 +46,12 @@ struct Ctl {
     double tH[4];      // 2-DOF damped system
     double bvec[6];    // steepest-descent vector of the current iteration
     double gmean, bottomg, ng, nf;
-    float cf, a11, a12, a21, a22, cu, cv;
+    double FJc[6], F1c, F2c;  // sums about cf (residual form of the FP32 sums)
+    double kap;               // bottomg / bottomf of the last iteration
+    SplitWarp sw;             // FP32 warp coordinates of this iteration
+    float cf;
+    int rform;                // the window has no masked pixel: residual form of S1, S2
+    int exact;                // a stopping test fell inside the guard band
     int tile_u0, tile_v0, tnx, tny, use_tile;
     int xa, ya, wx, wy, nx, ny, npix, w0;
     int count, it, state, status, sum_bw2;
@@ -112,19 +119,26 @@ __device__ void flood_fill(const unsigned char *bw, unsigned char *out, int nr, 
     }
 }
 
-template <int BS, int MODE>
+// EXACT: the FP64 re-solve of subsets whose stopping test fell inside the
+// guard band (and of windows that outgrew their bin): the bicubic gather is
+// evaluated in FP64 (from the FP32 image, as every path reads it) and every
+// iteration takes the two-pass FP64 statistics and the per-pixel residual
+// of funICGNQuadtree.m:269-371 / funICGN_Subpb1.m:270-379, so the iteration
+// count is the one an FP64 evaluation gives.
+template <int BS, int MODE, bool EXACT>
 __global__ __launch_bounds__(BS) void icgn_kernel(IcgnArgs a)
 {
     constexpr int NW = BS / kWave;
     constexpr int NJ = MODE;  // steepest-descent components
+    using GT = typename std::conditional<EXACT, double, float>::type;
     extern __shared__ __align__(16) unsigned char smem[];
     __shared__ Ctl c;
     __shared__ double redd[NW * 21];
     __shared__ float redf[NW * 12];
     __shared__ long item_sh;
 
-    float *gbuf = reinterpret_cast<float *>(smem);
-    float *tile = gbuf + a.npix_cap;
+    GT *gbuf = reinterpret_cast<GT *>(smem);
+    float *tile = reinterpret_cast<float *>(gbuf + a.npix_cap);
     unsigned char *tm = reinterpret_cast<unsigned char *>(tile + a.tile_cap);
     unsigned char *sel0 = tm + a.npix_cap;
     unsigned char *sel = sel0 + a.npix_cap;
@@ -146,10 +160,14 @@ __global__ __launch_bounds__(BS) void icgn_kernel(IcgnArgs a)
         __syncthreads();
         const long item = item_sh;
         if (item >= a.item0 + a.n_items) break;
+        // the fallback launch takes the subsets the main launches listed
+        if (a.defer_list && a.only_deferred && item >= *(volatile const int *)a.defer_flag) break;
 
         // ------------------------------------------------------------ setup
         if (tid == 0) {
-            const long idx = a.order ? a.order[item] : item;
+            const long idx = (a.defer_list && a.only_deferred)
+                                 ? (long)a.defer_list[a.list_back ? a.N - 1 - item : item]
+                                 : (a.order ? a.order[item] : item);
             c.idx = idx;
             c.slot = a.slot ? a.slot[idx] : 0;
             double x0 = a.coords[idx], y0 = a.coords[idx + a.N];
@@ -182,8 +200,11 @@ __global__ __launch_bounds__(BS) void icgn_kernel(IcgnArgs a)
             c.sum_bw2 = 0;
             c.sel0_all = 1;
             c.fsel = 0;
+            c.kap = 1.0;
+            c.exact = 0;
+            c.rform = 0;
             const FrameDev fr = a.frames[c.slot];
-            if (a.only_deferred && a.status[idx] != ST_DEFER) {
+            if (a.only_deferred && a.status[idx] != ST_DEFER && a.status[idx] != ST_EXACT) {
                 c.state = 4;  // done by the main launch
             } else if (!win_ok(fr, x0, y0, wx, wy)) {
                 c.status = ST_ERROR;
@@ -341,10 +362,16 @@ __global__ __launch_bounds__(BS) void icgn_kernel(IcgnArgs a)
                 c.bottomf = bf;
                 c.sabs = o[3];
                 c.cf = isfinite(mf) ? (float)mf : 0.f;
+                // residual form: no masked pixel and every pixel in f's nonzero set
+                c.rform = c.sel0_all && c.count == 0 && n == (double)c.npix;
+                const double cfd = c.cf;
+                c.F1c = sf - n * cfd;
+                c.F2c = sf2 - 2.0 * cfd * sf + n * cfd * cfd;
                 if (MODE == 6) {
                     for (int k = 0; k < 6; ++k) {
                         c.SJ[k] = o[4 + k];
                         c.FJ[k] = o[10 + k] - mf * o[4 + k];
+                        c.FJc[k] = o[10 + k] - cfd * o[4 + k];
                     }
                 } else {
                     const double s2 = 2.0 / (bf * bf);
@@ -357,6 +384,8 @@ __global__ __launch_bounds__(BS) void icgn_kernel(IcgnArgs a)
                     c.SJ[1] = o[8];
                     c.FJ[0] = o[9] - mf * o[7];
                     c.FJ[1] = o[10] - mf * o[8];
+                    c.FJc[0] = o[9] - cfd * o[7];
+                    c.FJc[1] = o[10] - cfd * o[8];
                 }
             }
             __syncthreads();
@@ -393,6 +422,15 @@ __global__ __launch_bounds__(BS) void icgn_kernel(IcgnArgs a)
                         ++t;
                     }
                 factor_lds<6>(c.Hm, c.Lw, c.piv, &c.fkind);
+                if (a.guard2 > 0) {
+                    // an ill-conditioned or indefinite H: the FP64 re-solve
+                    double dg6[6];
+                    for (int q = 0; q < 6; ++q) dg6[q] = c.Hm[q + q * 6];
+                    if (c.fkind != 0 || pivots_spread(dg6)) {
+                        c.exact = 1;
+                        c.state = 3;
+                    }
+                }
             }
             __syncthreads();
         }
@@ -461,18 +499,23 @@ __global__ __launch_bounds__(BS) void icgn_kernel(IcgnArgs a)
                     }
                     if (c.state == 0) {
                         double umin = INFINITY, umax = -INFINITY, vmin = INFINITY, vmax = -INFINITY;
-                        bool out = false;
+                        bool out = false, edge = false;
                         for (int q = 0; q < 4; ++q) {
                             const double dx = (q & 1) ? hx : -hx, dy = (q & 2) ? hy : -hy;
                             const double u = (1 + P[0]) * dx + P[2] * dy + (c.x0 + P[4]);
                             const double w = P[1] * dx + (1 + P[3]) * dy + (c.y0 + P[5]);
                             if (u < 3 || u > fr.H - 2 || w < 3 || w > fr.W - 2) out = true;
+                            edge |= near_edge(u, 3.0, fr.H - 2.0) | near_edge(w, 3.0, fr.W - 2.0);
                             umin = fmin(umin, u);
                             umax = fmax(umax, u);
                             vmin = fmin(vmin, w);
                             vmax = fmax(vmax, w);
                         }
-                        if (out) {
+                        if (edge && a.guard2 > 0) {
+                            // the bounds test decided within FP32 reach: FP64 re-solve
+                            c.exact = 1;
+                            c.state = 3;
+                        } else if (out) {
                             c.normNew = 1e6;
                             c.state = 1;
                         } else {
@@ -482,12 +525,7 @@ __global__ __launch_bounds__(BS) void icgn_kernel(IcgnArgs a)
                             c.tnx = (int)floor(umax) - (int)floor(umin) + 6;
                             c.tny = (int)floor(vmax) - (int)floor(vmin) + 6;
                             c.use_tile = c.tnx * c.tny <= a.tile_cap;
-                            c.a11 = (float)(1 + P[0]);
-                            c.a12 = (float)P[2];
-                            c.a21 = (float)P[1];
-                            c.a22 = (float)(1 + P[3]);
-                            c.cu = (float)((c.x0 + P[4]) - tu0);
-                            c.cv = (float)((c.y0 + P[5]) - tv0);
+                            c.sw = split_warp(P, c.x0, c.y0, tu0, tv0);
                         }
                     }
                 }
@@ -523,10 +561,73 @@ __global__ __launch_bounds__(BS) void icgn_kernel(IcgnArgs a)
             __syncthreads();
 
             // gather + fused sums
-            {
-                const float a11 = c.a11, a12 = c.a12, a21 = c.a21, a22 = c.a22;
-                const float cu = c.cu, cv = c.cv, cf = c.cf;
+            if constexpr (EXACT) {
+                // FP64 gather of every window pixel (the exact path below
+                // takes the statistics); counts of zeros and NaNs
+                const double *P = c.P;
+                float acc[2] = {0.f, 0.f};
+                for (int p = tid; p < npix; p += BS) {
+                    int i, j;
+                    pix_ij(p, nx, rcp, i, j);
+                    const double dx = i - (double)hx, dy = j - (double)hy;
+                    const double u = (1 + P[0]) * dx + P[2] * dy + (c.x0 + P[4]);
+                    const double v = P[1] * dx + (1 + P[3]) * dy + (c.y0 + P[5]);
+                    const double fu = floor(u), fv = floor(v);
+                    double wu[4], wv[4];
+                    keys_w(u - fu, wu);
+                    keys_w(v - fv, wv);
+                    const int iu = (int)fu - 1, iv = (int)fv - 1;  // 1-based first taps
+                    double g = 0.0;
+                    if (use_tile) {
+                        // the same FP32 samples of g, staged in LDS (origin tu0, tv0)
+                        const float *b = tile + (iu - tu0) + (iv - tv0) * tnx;
+#pragma unroll
+                        for (int q = 0; q < 4; ++q) {
+                            const float *cq = b + q * tnx;
+                            double sq = 0.0;
+#pragma unroll
+                            for (int r = 0; r < 4; ++r) sq = fma(wu[r], (double)cq[r], sq);
+                            g = fma(wv[q], sq, g);
+                        }
+                    } else {
+#pragma unroll
+                        for (int q = 0; q < 4; ++q) {
+                            const int y = min(max(iv + q, 1), fr.W);
+                            const float *col = fr.g + (size_t)(y - 1) * fr.H;
+                            double sq = 0.0;
+#pragma unroll
+                            for (int r = 0; r < 4; ++r)
+                                sq = fma(wu[r], (double)gld(col, (size_t)(min(max(iu + r, 1), fr.H) - 1)), sq);
+                            g = fma(wv[q], sq, g);
+                        }
+                    }
+                    gbuf[p] = g;
+                    acc[0] += g == 0.0 ? 1.f : 0.f;
+                    acc[1] += isnan(g) ? 1.f : 0.f;
+                }
+                float o[2];
+                block_sum<BS, 2>(acc, redf, o);
+                if (tid == 0) {
+                    const bool quad = !(MODE == 6 && a.plain);
+                    if (quad && o[1] > 0) {
+                        c.status = ST_ERROR;  // logical(NaN) throws
+                        c.state = 2;
+                    } else {
+                        c.path = (quad && (c.count > 0 || o[0] > 0)) ? 1 : 2;
+                    }
+                }
+                __syncthreads();
+            } else {
+                const SplitWarp sw = c.sw;
+                const float cf = c.cf;
                 const bool all = c.sel0_all, wholestats = (MODE == 6 && a.plain);
+                // residual form of the sums (as icgn_lean.hip): S1, S2 without
+                // masked pixels (rf), T on the set-up's f side (rt)
+                const bool rf = c.rform && !(MODE == 2 && c.fsel);
+                const bool rt = !(MODE == 2 && c.fsel);
+                const float kf = (float)c.kap;
+                const float ck = (float)((1.0 - (double)kf) * (double)cf);
+                const float c1k = (float)((1.0 + (double)kf) * (double)cf);
                 float acc[6 + NJ];
 #pragma unroll
                 for (int k = 0; k < 6 + NJ; ++k) acc[k] = 0.f;
@@ -534,22 +635,17 @@ __global__ __launch_bounds__(BS) void icgn_kernel(IcgnArgs a)
                     int i, j;
                     pix_ij(p, nx, rcp, i, j);
                     const float dx = (float)i - hx, dy = (float)j - hy;
-                    const float lu = fmaf(a11, dx, fmaf(a12, dy, cu));
-                    const float lv = fmaf(a21, dx, fmaf(a22, dy, cv));
-                    const float fu = floorf(lu), fv = floorf(lv);
-                    float wu[4], wv[4];
-                    keys_w(lu - fu, wu);
-                    keys_w(lv - fv, wv);
-                    const int iu = (int)fu - 1, iv = (int)fv - 1;
                     float g = 0.f;
                     if (use_tile) {
-                        const float *b = tile + iu + iv * tnx;
-#pragma unroll
-                        for (int q = 0; q < 4; ++q) {
-                            const float *cq = b + q * tnx;
-                            g = fmaf(wv[q], fmaf(wu[0], cq[0], fmaf(wu[1], cq[1], fmaf(wu[2], cq[2], wu[3] * cq[3]))), g);
-                        }
+                        g = split_gather(sw, tile, tnx, dx, dy);
                     } else {
+                        const float su = fmaf(sw.p1, dx, fmaf(sw.p3, dy, sw.fu));
+                        const float sv = fmaf(sw.p2, dx, fmaf(sw.p4, dy, sw.fv));
+                        const float fsu = floorf(su), fsv = floorf(sv);
+                        float wu[4], wv[4];
+                        keys_w(__builtin_amdgcn_fractf(su), wu);
+                        keys_w(__builtin_amdgcn_fractf(sv), wv);
+                        const int iu = (int)((dx + sw.cu) + fsu), iv = (int)((dy + sw.cv) + fsv);
 #pragma unroll
                         for (int q = 0; q < 4; ++q) {
                             const int y = min(max(tv0 + iv + q, 1), fr.W);
@@ -561,17 +657,27 @@ __global__ __launch_bounds__(BS) void icgn_kernel(IcgnArgs a)
                         }
                     }
                     gbuf[p] = g;
+                    const bool s = all || sel0[p];
+                    const int x = xa + i, y = ya + j;
+                    // f on the set-up's f side, as the set-up summed it
+                    float fvv = 0.f;
+                    if (s) {
+                        fvv = fidx(fr.f, fr.H, x, y);
+                        if (!(MODE == 6 && a.plain)) fvv *= fidx(fr.mask, fr.H, x, y);
+                    }
                     const float d = g - cf;
+                    const float r = fmaf(-kf, fvv, g) - ck;
+                    const float qv = fmaf(kf, fvv, g) - c1k;
+                    const float s1 = rf ? r : d, s2 = rf ? qv : d, t = rt ? r : d;
                     const bool nz = wholestats || fabsf(g) > 1e-10f;
                     acc[0] += nz ? 1.f : 0.f;
-                    acc[1] += nz ? d : 0.f;
-                    acc[2] += nz ? d * d : 0.f;
+                    acc[1] += nz ? s1 : 0.f;
+                    acc[2] = nz ? fmaf(s1, s2, acc[2]) : acc[2];
                     acc[3] += g == 0.f ? 1.f : 0.f;
                     acc[4] += isnan(g) ? 1.f : 0.f;
-                    const bool s = all || sel0[p];
-                    const size_t gi = (size_t)((xa + i - 4) + (ya + j - 4) * Hd);
+                    const size_t gi = (size_t)((x - 4) + (y - 4) * Hd);
                     const float fxv = s ? gld(fr.dfdx, gi) : 0.f, fyv = s ? gld(fr.dfdy, gi) : 0.f;
-                    const float u = fxv * d, v = fyv * d;
+                    const float u = fxv * t, v = fyv * t;
                     if (MODE == 6) {
                         acc[6] = fmaf(dx, u, acc[6]);
                         acc[7] = fmaf(dx, v, acc[7]);
@@ -597,20 +703,38 @@ __global__ __launch_bounds__(BS) void icgn_kernel(IcgnArgs a)
                     } else if (MODE == 2 && c.fsel) {
                         c.path = 2;
                     } else {
-                        const double n = o[0], S1 = o[1], S2 = o[2];
-                        const double cfd = (double)cf;
-                        const double gbar = cfd + S1 / n;
-                        const double bg = n >= 2 ? sqrt(fmax(S2 - S1 * S1 / n, 0.0)) : (n == 1 ? 0.0 : NAN);
+                        // gbar - c = S1/n, bottomg^2 = S2 - S1^2/n; b = FJ - (bf/bg) (T - (gbar-c) SJ)
+                        const double n = o[0];
+                        double dg = 0, var = 0, T[NJ];
+                        bool form_ok = true;
+                        if (rf) {
+                            // the residual form takes the nonzero set of g as the whole window
+                            float sums[2 + NJ];
+                            sums[0] = o[1];
+                            sums[1] = o[2];
+                            for (int k = 0; k < NJ; ++k) sums[2 + k] = o[6 + k];
+                            if (n == (double)npix)
+                                residual_stats<NJ>(n, sums, kf, ck, c1k, cf, c.F1c, c.F2c, c.FJc, c.SJ,
+                                                   dg, var, T);
+                            else
+                                form_ok = false;
+                        } else {
+                            dg = o[1] / n;
+                            var = o[2] - o[1] * dg;
+                            const double k = kf;
+                            const double corr = (double)ck - (1.0 - k) * (double)cf;
+                            for (int q = 0; q < NJ; ++q)
+                                T[q] = rt ? (double)o[6 + q] + k * c.FJc[q] + corr * c.SJ[q] : (double)o[6 + q];
+                        }
                         const double bf = c.bottomf;
-                        if (!(bf > 0) || !(bg > 0) || !isfinite(bf) || !isfinite(bg) || !isfinite(gbar)) {
+                        if (!form_ok || !(n >= 2) || !(bf > 0) || !(var > 0) || !isfinite(bf) ||
+                            !isfinite(var) || !isfinite(dg)) {
                             c.path = 2;  // literal per-pixel residual for degenerate statistics
                         } else {
-                            double b[6];
-                            for (int k = 0; k < NJ; ++k) {
-                                const double G = (double)o[6 + k] - (gbar - cfd) * c.SJ[k];
-                                b[k] = c.FJ[k] - (bf / bg) * G;
-                            }
-                            for (int k = 0; k < NJ; ++k) c.bvec[k] = b[k];
+                            const double sv = sqrt(var);
+                            const double ratio = bf / sv;
+                            c.kap = sv / bf;
+                            for (int k = 0; k < NJ; ++k) c.bvec[k] = c.FJ[k] - ratio * (T[k] - dg * c.SJ[k]);
                         }
                     }
                 }
@@ -768,8 +892,11 @@ __global__ __launch_bounds__(BS) void icgn_kernel(IcgnArgs a)
                     }
                     double o[NJ];
                     block_sum<BS, NJ>(v, redd, o);
-                    if (tid == 0)
+                    if (tid == 0) {
                         for (int k = 0; k < NJ; ++k) c.bvec[k] = c.bottomf * o[k];
+                        const double kp = c.bottomg / c.bottomf;
+                        if (isfinite(kp) && kp > 0) c.kap = kp;
+                    }
                     __syncthreads();
                 }
             }
@@ -796,7 +923,12 @@ __global__ __launch_bounds__(BS) void icgn_kernel(IcgnArgs a)
                 if (c.it == 1) c.normInit = nn;
                 nn = c.normInit > tol ? nn / c.normInit : 0.0;
                 c.normNew = nn;
-                if (nn < tol || c.normAbs < tol) {
+                const double tol2 = tol * tol;
+                if (near_tol(nb2, tol2, a.guard2_abs) ||
+                    (c.normInit > tol && near_tol(nn * nn, tol2, a.guard2))) {
+                    c.exact = 1;  // the test is within FP32 reach of tol: FP64 re-solve
+                    c.state = 3;
+                } else if (nn < tol || c.normAbs < tol) {
                     c.state = 1;
                 } else {
                     if (MODE == 6) {
@@ -817,8 +949,9 @@ __global__ __launch_bounds__(BS) void icgn_kernel(IcgnArgs a)
 
         // ---------------------------------------------------------- outputs
         if (tid == 0 && c.state == 3) {
-            a.status[c.idx] = ST_DEFER;
-            if (a.defer_out) atomicOr(a.defer_out, 1);
+            a.status[c.idx] = c.exact ? ST_EXACT : ST_DEFER;
+            defer_append(a, c.idx, a.wx[c.idx], MODE == 2 ? a.wy[c.idx] : a.wx[c.idx]);
+            if (c.exact && a.exact_count) atomicAdd(a.exact_count, 1);
         }
         if (tid == 0 && c.state <= 2) {
             int st = c.status;
@@ -855,23 +988,25 @@ __global__ __launch_bounds__(BS) void icgn_kernel(IcgnArgs a)
 }
 
 // Host-side launch helpers (instantiated for the block sizes the autotuner may pick).
-size_t icgn_lds_bytes(int npix_cap, int tile_cap)
+size_t icgn_lds_bytes(int npix_cap, int tile_cap, bool exact)
 {
-    size_t b = sizeof(float) * (size_t)(npix_cap + tile_cap) + 4u * (size_t)npix_cap;
+    // g per pixel (FP64 in the re-solve), the g tile, four byte masks per pixel
+    size_t b = (exact ? sizeof(double) : sizeof(float)) * (size_t)npix_cap +
+               sizeof(float) * (size_t)tile_cap + 4u * (size_t)npix_cap;
     return (b + 15) & ~(size_t)15;
 }
 
-template <int BS, int MODE>
+template <int BS, int MODE, bool EXACT>
 static hipError_t launch_bs(const IcgnArgs &a, int grid, hipStream_t s)
 {
-    const size_t lds = icgn_lds_bytes(a.npix_cap, a.tile_cap);
+    const size_t lds = icgn_lds_bytes(a.npix_cap, a.tile_cap, EXACT);
     static bool attr_set = false;
     if (!attr_set) {
-        hipFuncSetAttribute((const void *)icgn_kernel<BS, MODE>,
+        hipFuncSetAttribute((const void *)icgn_kernel<BS, MODE, EXACT>,
                             hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024 - 8192);
         attr_set = true;
     }
-    hipLaunchKernelGGL((icgn_kernel<BS, MODE>), dim3(grid), dim3(BS), lds, s, a);
+    hipLaunchKernelGGL((icgn_kernel<BS, MODE, EXACT>), dim3(grid), dim3(BS), lds, s, a);
     return hipGetLastError();
 }
 
@@ -879,20 +1014,26 @@ hipError_t icgn_launch(const IcgnArgs &a, int mode, int bs, int grid, hipStream_
 {
     if (mode == 6) {
         switch (bs) {
-        case 64: return launch_bs<64, 6>(a, grid, s);
-        case 128: return launch_bs<128, 6>(a, grid, s);
-        case 256: return launch_bs<256, 6>(a, grid, s);
-        case 512: return launch_bs<512, 6>(a, grid, s);
+        case 64: return launch_bs<64, 6, false>(a, grid, s);
+        case 128: return launch_bs<128, 6, false>(a, grid, s);
+        case 256: return launch_bs<256, 6, false>(a, grid, s);
+        case 512: return launch_bs<512, 6, false>(a, grid, s);
         }
     } else {
         switch (bs) {
-        case 64: return launch_bs<64, 2>(a, grid, s);
-        case 128: return launch_bs<128, 2>(a, grid, s);
-        case 256: return launch_bs<256, 2>(a, grid, s);
-        case 512: return launch_bs<512, 2>(a, grid, s);
+        case 64: return launch_bs<64, 2, false>(a, grid, s);
+        case 128: return launch_bs<128, 2, false>(a, grid, s);
+        case 256: return launch_bs<256, 2, false>(a, grid, s);
+        case 512: return launch_bs<512, 2, false>(a, grid, s);
         }
     }
     return hipErrorInvalidValue;
+}
+
+hipError_t icgn_exact_launch(const IcgnArgs &a, int mode, int bs, int grid, hipStream_t s)
+{
+    if (bs == 64) return mode == 6 ? launch_bs<64, 6, true>(a, grid, s) : launch_bs<64, 2, true>(a, grid, s);
+    return mode == 6 ? launch_bs<256, 6, true>(a, grid, s) : launch_bs<256, 2, true>(a, grid, s);
 }
 
 }  // namespace staq

@@ -32,17 +32,25 @@ struct LeanLds {
     double Ld[6];    // 1 / diag(Lp)
     double H2i[4];   // 2-DOF: inverse of the damped 2x2 system
     double FJ[6], SJ[6];
+    // sums about cf = (float) meanf, for the residual form of the
+    // per-iteration sums: FJc = sum J (f - cf), F1c = sum (f - cf),
+    // F2c = sum (f - cf)^2
+    double FJc[6], F1c, F2c;
     // exchange between the wavefronts of one subset (double-buffered)
     float xf[2][NW][24];
     double xd[2][NW][16];
     int xi[2][NW][8];
     // eight or more wavefronts: the Gauss-Newton step of wavefront 0
-    double Pn[6], rel2, abs2, inv_init2;
+    double Pn[6], rel2, abs2, inv_init2, kap;
     int stop, init_ok;
 };
 
 template <int PPT, int MODE, int NW>
-__global__ __launch_bounds__(64 * NW) void icgn_lean_kernel(IcgnArgs a)
+// two wavefronts per subset: three waves per SIMD (the register allocator
+// otherwise settles on two, which measured slower on 33 x 33 windows); other
+// forms: the allocator's own choice (a three-wave cap measured slower there)
+__global__ __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu(NW == 2 ? 3 : 1)))
+void icgn_lean_kernel(IcgnArgs a)
 {
     constexpr int NJ = MODE;
     constexpr int CAP = 64 * PPT * NW;
@@ -50,6 +58,9 @@ __global__ __launch_bounds__(64 * NW) void icgn_lean_kernel(IcgnArgs a)
     __shared__ LeanLds<NW> L;
     __shared__ long item_sh;
     float *tile = reinterpret_cast<float *>(smem);
+    // f of every pixel slot (f * mask under the quadtree rules), read each
+    // iteration by the residual form of the sums; after the g tile
+    float *fwin = tile + a.tile_cap;
     const int tid = threadIdx.x;
     const int lane = tid & 63;
     const int wv = tid >> 6;
@@ -87,6 +98,7 @@ __global__ __launch_bounds__(64 * NW) void icgn_lean_kernel(IcgnArgs a)
             P[5] = gld(a.UOld, (size_t)(2 * idx + 1));
         }
         int state = 0, status = 0;  // 0 run, 1 loop done, 2 final status, 3 deferred
+        bool exact = false;  // deferred to the FP64 re-solve (guard band, ill-conditioned H)
         {
             const double xa = x0 - wx * 0.5, xb = x0 + wx * 0.5, ya = y0 - wy * 0.5,
                          yb = y0 + wy * 0.5;
@@ -174,6 +186,7 @@ __global__ __launch_bounds__(64 * NW) void icgn_lean_kernel(IcgnArgs a)
                     fyr[k] = gld(fr.dfdy, gi);
                     float f = fidx(fr.f, fr.H, x, y);
                     if (quad) f *= fidx(fr.mask, fr.H, x, y);
+                    fwin[p] = f;
                     const float fx = fxr[k], fy = fyr[k];
                     const bool nz = !quad || fabsf(f) > 1e-10f;
                     const float df = f - c0;
@@ -219,13 +232,21 @@ __global__ __launch_bounds__(64 * NW) void icgn_lean_kernel(IcgnArgs a)
             sabs = sd[3];
             meanf = n > 0 ? (double)c0 + S1 / n : NAN;
             bottomf = n >= 2 ? sqrt(fmax(S2 - S1 * S1 / n, 0.0)) : (n == 1 ? 0.0 : NAN);
+            // the residual form takes every window pixel as nonzero f: a
+            // window with |f| <= 1e-10 somewhere goes to the general kernel
+            if (n != (double)npix) state = 3;
             const double dm = (double)c0 - meanf;
+            // the same sums about cf = (float) meanf (exact shifts in FP64)
+            const double e0 = (double)c0 - (double)(isfinite(meanf) ? (float)meanf : 0.f);
             if (tid == 0) {
 #pragma unroll
                 for (int q = 0; q < NJ; ++q) {
                     L.SJ[q] = sd[4 + q];
                     L.FJ[q] = sd[4 + NJ + q] + dm * sd[4 + q];  // sum J (f - meanf)
+                    L.FJc[q] = sd[4 + NJ + q] + e0 * sd[4 + q];  // sum J (f - cf)
                 }
+                L.F1c = S1 + n * e0;
+                L.F2c = S2 + 2.0 * e0 * S1 + n * e0 * e0;
             }
             if (MODE == 6) {
                 double h[21];
@@ -238,8 +259,16 @@ __global__ __launch_bounds__(64 * NW) void icgn_lean_kernel(IcgnArgs a)
                         for (int q = 0; q < 21; ++q) L.Lp[q] = h[q];
                         for (int c = 0; c < 6; ++c) L.Ld[c] = 1.0 / h[c * 6 - (c * (c - 1)) / 2];
                     }
+                    double dg6[6];
+#pragma unroll
+                    for (int c = 0; c < 6; ++c) dg6[c] = h[c * 6 - (c * (c - 1)) / 2];
+                    if (a.guard2 > 0 && pivots_spread(dg6)) {
+                        state = 3;  // ill-conditioned H: the FP64 re-solve
+                        exact = true;
+                    }
                 } else {
-                    state = 3;  // LU of an indefinite H: the general kernel
+                    state = 3;  // LU of an indefinite H: the general kernel (or the FP64 re-solve)
+                    exact = a.guard2 > 0;
                 }
             } else {
                 set_h2(L, hs[0], hs[1], hs[2], bottomf, a.mu);
@@ -251,6 +280,7 @@ __global__ __launch_bounds__(64 * NW) void icgn_lean_kernel(IcgnArgs a)
         int it = 0;
         int tu0 = 0, tv0 = 0, tnx = 0, tny = 0;  // resident g tile (1-based origin, extents)
         bool have_tile = false;
+        double kap = 1.0;  // bottomg / bottomf of the last iteration
         const float cf = isfinite(meanf) ? (float)meanf : 0.f;
         double uo_u = 0.0, uo_v = 0.0, vd_u = 0.0, vd_v = 0.0;  // 2-DOF penalty targets
         if (MODE == 2) {
@@ -300,10 +330,18 @@ __global__ __launch_bounds__(64 * NW) void icgn_lean_kernel(IcgnArgs a)
                 vmin = fmin(fmin(v[0], v[1]), fmin(v[2], v[3]));
                 vmax = fmax(fmax(v[0], v[1]), fmax(v[2], v[3]));
                 // NaN corners compare false, as in the reference's test
-                bool o = false;
+                bool o = false, edge = false;
 #pragma unroll
-                for (int q = 0; q < 4; ++q) o |= (u[q] < 3) | (u[q] > fr.H - 2) | (v[q] < 3) | (v[q] > fr.W - 2);
+                for (int q = 0; q < 4; ++q) {
+                    o |= (u[q] < 3) | (u[q] > fr.H - 2) | (v[q] < 3) | (v[q] > fr.W - 2);
+                    edge |= near_edge(u[q], 3.0, fr.H - 2.0) | near_edge(v[q], 3.0, fr.W - 2.0);
+                }
                 out = o;
+                if (edge && a.guard2 > 0) {
+                    state = 3;  // the bounds test decided within FP32 reach: FP64 re-solve
+                    exact = true;
+                    break;
+                }
             }
             if (out) {
                 rel2 = 1e12;  // normOfWNew = 1e6 (:156)
@@ -332,9 +370,8 @@ __global__ __launch_bounds__(64 * NW) void icgn_lean_kernel(IcgnArgs a)
                 }
                 have_tile = true;
             }
-            const float a11 = (float)(1 + P[0]), a12 = (float)P[2], a21 = (float)P[1],
-                        a22 = (float)(1 + P[3]);
-            const float cu = (float)((x0 + P[4]) - tu0), cv = (float)((y0 + P[5]) - tv0);
+            // warp coordinates split into integer and FP32 fractional parts
+            const SplitWarp sw = split_warp(P, x0, y0, tu0, tv0);
             if (reload) {
                 __syncthreads();  // the last iteration's gathers are done with the old tile
                 const int total = tnx * tny;
@@ -359,27 +396,18 @@ __global__ __launch_bounds__(64 * NW) void icgn_lean_kernel(IcgnArgs a)
                 }
                 __syncthreads();
             }
-            // Keys bicubic of the window pixel at offset (dx, dy) from the resident tile
-            auto gather = [&](float dx, float dy) {
-                const float lu = fmaf(a11, dx, fmaf(a12, dy, cu));
-                const float lv = fmaf(a21, dx, fmaf(a22, dy, cv));
-                const float fu = floorf(lu), fvv = floorf(lv);
-                float wu[4], wv[4];
-                keys_w(lu - fu, wu);
-                keys_w(lv - fvv, wv);
-                const int iu = (int)fu - 1, iv = (int)fvv - 1;
-                float g = 0.f;
-                const float *b = tile + iu + iv * tnx;
-#pragma unroll
-                for (int q = 0; q < 4; ++q) {
-                    const float *c4 = b + q * tnx;
-                    g = fmaf(wv[q], fmaf(wu[0], c4[0], fmaf(wu[1], c4[1], fmaf(wu[2], c4[2], wu[3] * c4[3]))), g);
-                }
-                return g;
-            };
-            // S1 = sum(g - c), S2 = sum((g - c)^2) over the nonzero g,
-            // T_q = sum J_q (g - c); every lane runs every slot (no branch
-            // splits the unrolled slots, so their gathers overlap)
+            // Residual form of the sums, about kf * f: with r = g - kf f - ck
+            // and q = g + kf f - c1k (ck = (1 - kf) cf, c1k = (1 + kf) cf
+            // rounded to FP32), sum r, sum r q and sum J_q r are sums of small
+            // terms near convergence (kf = bottomg / bottomf of the last
+            // iteration), so their FP32 rounding stays far below the
+            // tolerance the stopping test resolves; the sums about cf follow
+            // exactly in FP64 from the set-up's sums of f (funICGNQuadtree.m:
+            // 269-270, 362-371).  Every lane runs every slot (no branch splits
+            // the unrolled slots, so their gathers overlap).
+            const float kf = (float)kap;
+            const float ck = (float)((1.0 - (double)kf) * (double)cf);
+            const float c1k = (float)((1.0 + (double)kf) * (double)cf);
             float acc[2 + NJ];
 #pragma unroll
             for (int q = 0; q < 2 + NJ; ++q) acc[q] = 0.f;
@@ -389,15 +417,17 @@ __global__ __launch_bounds__(64 * NW) void icgn_lean_kernel(IcgnArgs a)
 #pragma unroll
                 for (int k = 0; k < PPT; ++k) {
                     const bool valid = (vbits >> k) & 1u;
-                    const float g = gather(dxr[k], dyr[k]);
+                    const float g = split_gather(sw, tile, tnx, dxr[k], dyr[k]);
+                    const float fv = fwin[pixel(k)];
                     any_zero |= valid && g == 0.f;
                     any_nan |= valid && isnan(g);
                     const float ge = valid ? g : 0.f;
-                    const float d = ge - cf;
+                    const float r = fmaf(-kf, fv, ge) - ck;
+                    const float qv = fmaf(kf, fv, ge) - c1k;
                     const bool nz = valid && (!quad || fabsf(ge) > 1e-10f);
-                    acc[0] += nz ? d : 0.f;
-                    acc[1] += nz ? d * d : 0.f;
-                    const float u = valid ? fxr[k] * d : 0.f, v = valid ? fyr[k] * d : 0.f;
+                    acc[0] += nz ? r : 0.f;
+                    acc[1] = nz ? fmaf(r, qv, acc[1]) : acc[1];
+                    const float u = valid ? fxr[k] * r : 0.f, v = valid ? fyr[k] * r : 0.f;
                     if (MODE == 6) {
                         acc[2] = fmaf(dxr[k], u, acc[2]);
                         acc[3] = fmaf(dxr[k], v, acc[3]);
@@ -426,29 +456,33 @@ __global__ __launch_bounds__(64 * NW) void icgn_lean_kernel(IcgnArgs a)
                 state = 3;  // a zero of g: bwselect regions, the general kernel
                 break;
             }
-            // gbar - c = S1/n, bottomg^2 = S2 - S1^2/n; b = FJ - (bf/bg) (T - (gbar-c) SJ)
             if constexpr (NW == 1) group_reduce<NW, 2 + NJ, 0, 0>(L, par, acc, nullptr, nullptr);
             const double n = xv[0];
-            const double S1 = acc[0], S2 = acc[1];
-            const double inv_n = __builtin_amdgcn_rcp(n);
-            const double dg = S1 * inv_n;
-            const double var = S2 - S1 * dg;
             const double bf = bottomf;
+            // the residual form takes the nonzero set of g as the whole window
+            if (n != (double)npix) {
+                state = 3;
+                break;
+            }
+            double T[NJ], dg, var;
+            residual_stats<NJ>(n, acc, kf, ck, c1k, cf, L.F1c, L.F2c, L.FJc, L.SJ, dg, var, T);
             if (!(n >= 2) || !(bf > 0) || !(var > 0) || !isfinite(bf) || !isfinite(var) ||
                 !isfinite(dg)) {
                 state = 3;  // degenerate statistics: the reference's FP64 residual
                 break;
             }
-            // steepest-descent vector, norm test, Gauss-Newton step and
-            // inverse-compositional update (the same operations as below);
-            // false: the loop ends (state 1)
-            auto gn_step = [&]() -> bool {
+            const double sv = sqrt(var);
+            kap = sv / bf;
+            // steepest-descent vector b = FJ - (bottomf/bottomg) (T - (gbar-c) SJ),
+            // norm test (with its guard band), Gauss-Newton step and
+            // inverse-compositional update; returns 0 to go on, 1 when the
+            // loop ends, 2 when a test fell inside the guard band
+            auto gn_step = [&]() -> int {
                 double b[6] = {0, 0, 0, 0, 0, 0};
                 {
-                    const double ratio = bf * __builtin_amdgcn_rsq(var);
+                    const double ratio = bf / sv;
 #pragma unroll
-                    for (int q = 0; q < NJ; ++q)
-                        b[q] = L.FJ[q] - ratio * ((double)acc[2 + q] - dg * L.SJ[q]);
+                    for (int q = 0; q < NJ; ++q) b[q] = L.FJ[q] - ratio * (T[q] - dg * L.SJ[q]);
                 }
                 double nb2 = 0.0;
                 if (MODE == 6) {
@@ -456,7 +490,7 @@ __global__ __launch_bounds__(64 * NW) void icgn_lean_kernel(IcgnArgs a)
                     for (int q = 0; q < 6; ++q) nb2 += b[q] * b[q];
                 } else {
                     // :379 tempb = 2 b / bottomf^2 + mu (P - UOld - vdual)
-                    const double s2 = 2.0 * __builtin_amdgcn_rcp(bottomf * bottomf);
+                    const double s2 = 2.0 / (bottomf * bottomf);
                     b[0] = b[0] * s2 + a.mu * ((P[4] - uo_u) - vd_u);
                     b[1] = b[1] * s2 + a.mu * ((P[5] - uo_v) - vd_v);
                     nb2 = b[0] * b[0] + b[1] * b[1];
@@ -464,10 +498,12 @@ __global__ __launch_bounds__(64 * NW) void icgn_lean_kernel(IcgnArgs a)
                 abs2 = nb2;
                 if (it == 1) {
                     init_ok = nb2 > tol2;  // normOfWNewInit > tol
-                    inv_init2 = __builtin_amdgcn_rcp(nb2);
+                    inv_init2 = 1.0 / nb2;
                 }
                 rel2 = init_ok ? nb2 * inv_init2 : 0.0;
-                if (rel2 < tol2 || abs2 < tol2) return false;
+                if (near_tol(abs2, tol2, a.guard2_abs) || (init_ok && near_tol(rel2, tol2, a.guard2)))
+                    return 2;
+                if (rel2 < tol2 || abs2 < tol2) return 1;
                 double d[6] = {0, 0, 0, 0, 0, 0};
                 if (MODE == 6) {
                     solve_chol6(L.Lp, L.Ld, b, d);
@@ -475,13 +511,14 @@ __global__ __launch_bounds__(64 * NW) void icgn_lean_kernel(IcgnArgs a)
                     d[4] = -(L.H2i[0] * b[0] + L.H2i[2] * b[1]);
                     d[5] = -(L.H2i[1] * b[0] + L.H2i[3] * b[1]);
                 }
-                return ic_compose(P, d);
+                return ic_compose(P, d) ? 0 : 1;
             };
+            int go;
             if constexpr (NW >= 8) {
-                bool go = false;
                 // with eight or more wavefronts on one CU the redundant FP64
                 // step would queue behind itself on every SIMD: wavefront 0
                 // takes it and hands the new state to the others
+                go = 0;
                 if (wv == 0) {
                     go = gn_step();
                     if (lane == 0) {
@@ -491,7 +528,7 @@ __global__ __launch_bounds__(64 * NW) void icgn_lean_kernel(IcgnArgs a)
                         L.abs2 = abs2;
                         L.inv_init2 = inv_init2;
                         L.init_ok = init_ok;
-                        L.stop = !go;
+                        L.stop = go;
                     }
                 }
                 __syncthreads();
@@ -502,53 +539,20 @@ __global__ __launch_bounds__(64 * NW) void icgn_lean_kernel(IcgnArgs a)
                     abs2 = L.abs2;
                     inv_init2 = L.inv_init2;
                     init_ok = L.init_ok;
-                    go = !L.stop;
+                    go = L.stop;
                 }
-                go = __builtin_amdgcn_readfirstlane((int)go);
-                if (!go) {
-                    state = 1;
-                    break;
-                }
-                continue;
-            }
-            // up to four wavefronts (one per SIMD): every wavefront takes the
-            // same step, with no barrier on the iteration's critical path
-            double b[6] = {0, 0, 0, 0, 0, 0};
-            {
-                const double ratio = bf * __builtin_amdgcn_rsq(var);
-#pragma unroll
-                for (int q = 0; q < NJ; ++q) b[q] = L.FJ[q] - ratio * ((double)acc[2 + q] - dg * L.SJ[q]);
-            }
-            // norm test, Gauss-Newton step, inverse-compositional update
-            double nb2 = 0.0;
-            if (MODE == 6) {
-#pragma unroll
-                for (int q = 0; q < 6; ++q) nb2 += b[q] * b[q];
+                go = __builtin_amdgcn_readfirstlane(go);
             } else {
-                // :379 tempb = 2 b / bottomf^2 + mu (P - UOld - vdual)
-                const double s2 = 2.0 * __builtin_amdgcn_rcp(bottomf * bottomf);
-                b[0] = b[0] * s2 + a.mu * ((P[4] - uo_u) - vd_u);
-                b[1] = b[1] * s2 + a.mu * ((P[5] - uo_v) - vd_v);
-                nb2 = b[0] * b[0] + b[1] * b[1];
+                // up to four wavefronts (one per SIMD): every wavefront takes
+                // the same step, with no barrier on the iteration's critical path
+                go = gn_step();
             }
-            abs2 = nb2;
-            if (it == 1) {
-                init_ok = nb2 > tol2;  // normOfWNewInit > tol
-                inv_init2 = __builtin_amdgcn_rcp(nb2);
-            }
-            rel2 = init_ok ? nb2 * inv_init2 : 0.0;
-            if (rel2 < tol2 || abs2 < tol2) {
-                state = 1;
+            if (go == 2) {
+                state = 3;
+                exact = true;
                 break;
             }
-            double d[6] = {0, 0, 0, 0, 0, 0};
-            if (MODE == 6) {
-                solve_chol6(L.Lp, L.Ld, b, d);
-            } else {
-                d[4] = -(L.H2i[0] * b[0] + L.H2i[2] * b[1]);
-                d[5] = -(L.H2i[1] * b[0] + L.H2i[3] * b[1]);
-            }
-            if (!ic_compose(P, d)) {
+            if (go == 1) {
                 state = 1;
                 break;
             }
@@ -564,9 +568,14 @@ __global__ __launch_bounds__(64 * NW) void icgn_lean_kernel(IcgnArgs a)
         }
         if (tid == 0) {
             if (state == 3) {
-                a.status[idx] = ST_DEFER;
-                if (a.defer_out) atomicOr(a.defer_out, 1);
-                if (a.handed) a.handed[idx] = 1;  // the schedule moves it to the general kernel
+                a.status[idx] = exact ? ST_EXACT : ST_DEFER;
+                defer_append(a, idx, gld(a.wx, (size_t)idx), MODE == 2 ? gld(a.wy, (size_t)idx)
+                                                                     : gld(a.wx, (size_t)idx));
+                if (exact) {
+                    if (a.exact_count) atomicAdd(a.exact_count, 1);
+                } else if (a.handed) {
+                    a.handed[idx] = 1;  // the schedule moves it to the general kernel
+                }
             } else {
                 double Pout[6];
                 for (int k = 0; k < 6; ++k) Pout[k] = P[k];
@@ -595,7 +604,8 @@ __global__ __launch_bounds__(64 * NW) void icgn_lean_kernel(IcgnArgs a)
 template <int PPT, int MODE, int NW>
 hipError_t launch_lean(const IcgnArgs &a, int grid, hipStream_t s)
 {
-    const size_t lds = sizeof(float) * (size_t)a.tile_cap;
+    // the g tile, then f of every pixel slot
+    const size_t lds = sizeof(float) * ((size_t)a.tile_cap + 64 * PPT * NW);
     hipLaunchKernelGGL((icgn_lean_kernel<PPT, MODE, NW>), dim3(grid), dim3(64 * NW), lds, s, a);
     return hipGetLastError();
 }

@@ -81,6 +81,9 @@ struct WaveLds {
     int piv[6], kind;  // kind 0: Cholesky in Lp, 1: LU in A
     double H2i[4];     // 2-DOF: inverse of the damped 2x2 system
     double FJ[6], SJ[6];
+    // the same sums about cf = (float) meanf (FJc = sum J (f - cf),
+    // F1c = sum (f - cf), F2c = sum (f - cf)^2), for the residual form
+    double FJc[6], F1c, F2c;
     unsigned long long colw[64];  // scratch column words (atomic OR target)
     unsigned long long tmw[64];   // window-mask columns
     unsigned long long sel0w[64]; // setup region (4-connected mask region)
@@ -122,6 +125,9 @@ __global__ __launch_bounds__(64 * NW) void icgn_wave_kernel(IcgnArgs a)
     __shared__ WaveLds L;
     __shared__ long item_sh;
     float *tile = reinterpret_cast<float *>(smem);
+    // f of every pixel slot on the set-up's region (0 outside it), read each
+    // iteration by the residual form of the sums; after the g tile
+    float *fwin = tile + a.tile_cap;
     const int tid = threadIdx.x;
     const int lane = tid & 63;   // lane within the wavefront
     const int wv = tid >> 6;     // wavefront of the subset
@@ -161,6 +167,7 @@ __global__ __launch_bounds__(64 * NW) void icgn_wave_kernel(IcgnArgs a)
             P[5] = gld(a.UOld, (size_t)(2 * idx + 1));
         }
         int state = 0, status = 0;  // 0 run, 1 loop done, 2 final status, 3 deferred
+        bool exact = false;  // deferred to the FP64 re-solve (guard band, ill-conditioned H)
         auto win_fits = [&](int wxx, int wyy) {
             const double xa = x0 - wxx * 0.5, xb = x0 + wxx * 0.5, ya = y0 - wyy * 0.5,
                          yb = y0 + wyy * 0.5;
@@ -279,6 +286,7 @@ __global__ __launch_bounds__(64 * NW) void icgn_wave_kernel(IcgnArgs a)
         // nowhere on the iteration's critical path
         double meanf = 0, bottomf = 0, sabs = 0, rel2 = 1, abs2 = 1, inv_init2 = 0;
         bool init_ok = false;
+        bool rform_f = false;  // the set-up allows the residual form (no masked pixel)
         const double tol2 = a.tol * a.tol;
         if (state == 0) {
             const float c0 = fidx(fr.f, fr.H, xa + nx / 2, ya + ny / 2) *
@@ -309,6 +317,7 @@ __global__ __launch_bounds__(64 * NW) void icgn_wave_kernel(IcgnArgs a)
                     if (!s) s0bits &= ~(1u << k);
                     const float fx = s ? fxr[k] : 0.f, fy = s ? fyr[k] : 0.f;
                     if (!s) f = 0.f;
+                    fwin[p] = f;
                     const bool nz = !quad || fabsf(f) > 1e-10f;
                     const float df = f - c0;
                     s_n += nz ? 1.f : 0.f;
@@ -353,15 +362,23 @@ __global__ __launch_bounds__(64 * NW) void icgn_wave_kernel(IcgnArgs a)
             sabs = sd[3];
             meanf = n > 0 ? (double)c0 + S1 / n : NAN;
             bottomf = n >= 2 ? sqrt(fmax(S2 - S1 * S1 / n, 0.0)) : (n == 1 ? 0.0 : NAN);
+            // the residual form of the sums needs every pixel in f's nonzero set
+            rform_f = !masked_f && n == (double)npix;
             const double dm = (double)c0 - meanf;
+            const double e0 = (double)c0 - (double)(isfinite(meanf) ? (float)meanf : 0.f);
 #pragma unroll
             for (int q = 0; q < NJ; ++q) {
                 const double SJq = sd[4 + q];
                 const double SJFq = sd[4 + NJ + q];
                 if (tid == 0) {
                     L.SJ[q] = SJq;
-                    L.FJ[q] = SJFq + dm * SJq;  // sum J (f - meanf)
+                    L.FJ[q] = SJFq + dm * SJq;   // sum J (f - meanf)
+                    L.FJc[q] = SJFq + e0 * SJq;  // sum J (f - cf)
                 }
+            }
+            if (tid == 0) {
+                L.F1c = S1 + n * e0;
+                L.F2c = S2 + 2.0 * e0 * S1 + n * e0 * e0;
             }
             if (MODE == 6) {
                 double h[21];
@@ -370,6 +387,16 @@ __global__ __launch_bounds__(64 * NW) void icgn_wave_kernel(IcgnArgs a)
                 if (tid == 0)
                     for (int q = 0; q < 21; ++q) a.H21[idx + (long)q * a.N] = h[q];
                 const bool spd = chol6(h);
+                if (a.guard2 > 0) {
+                    // an ill-conditioned or indefinite H: the FP64 re-solve
+                    double dg6[6];
+#pragma unroll
+                    for (int c = 0; c < 6; ++c) dg6[c] = h[c * 6 - (c * (c - 1)) / 2];
+                    if (!spd || pivots_spread(dg6)) {
+                        state = 3;
+                        exact = true;
+                    }
+                }
                 if (tid == 0) {
                     if (spd) {
                         for (int q = 0; q < 21; ++q) L.Lp[q] = h[q];
@@ -414,6 +441,7 @@ __global__ __launch_bounds__(64 * NW) void icgn_wave_kernel(IcgnArgs a)
         // g tile resident in LDS: [tu0, tu0+tnx) x [tv0, tv0+tny) (1-based)
         int tu0 = 0, tv0 = 0, tnx = 0, tny = 0;
         bool have_tile = false;
+        double kap = 1.0;  // bottomg / bottomf of the last iteration
         const float cf = isfinite(meanf) ? (float)meanf : 0.f;
         double uo_u = 0.0, uo_v = 0.0, vd_u = 0.0, vd_v = 0.0;  // 2-DOF penalty targets
         if (MODE == 2) {
@@ -493,10 +521,18 @@ __global__ __launch_bounds__(64 * NW) void icgn_wave_kernel(IcgnArgs a)
                 umax = fmax(fmax(u[0], u[1]), fmax(u[2], u[3]));
                 vmin = fmin(fmin(v[0], v[1]), fmin(v[2], v[3]));
                 vmax = fmax(fmax(v[0], v[1]), fmax(v[2], v[3]));
-                bool o = false;
+                bool o = false, edge = false;
 #pragma unroll
-                for (int q = 0; q < 4; ++q) o |= (u[q] < 3) | (u[q] > fr.H - 2) | (v[q] < 3) | (v[q] > fr.W - 2);
+                for (int q = 0; q < 4; ++q) {
+                    o |= (u[q] < 3) | (u[q] > fr.H - 2) | (v[q] < 3) | (v[q] > fr.W - 2);
+                    edge |= near_edge(u[q], 3.0, fr.H - 2.0) | near_edge(v[q], 3.0, fr.W - 2.0);
+                }
                 out = o;
+                if (edge && a.guard2 > 0) {
+                    state = 3;  // the bounds test decided within FP32 reach: FP64 re-solve
+                    exact = true;
+                    break;
+                }
             }
             if (out) {
                 rel2 = 1e12;
@@ -528,9 +564,8 @@ __global__ __launch_bounds__(64 * NW) void icgn_wave_kernel(IcgnArgs a)
                     tny = nv1 - nv0 + 1;
                 }
             }
-            const float a11 = (float)(1 + P[0]), a12 = (float)P[2], a21 = (float)P[1],
-                        a22 = (float)(1 + P[3]);
-            const float cu = (float)((x0 + P[4]) - tu0), cv = (float)((y0 + P[5]) - tv0);
+            // warp coordinates split into integer and FP32 fractional parts
+            const SplitWarp sw = split_warp(P, x0, y0, tu0, tv0);
             if (use_tile && reload) {
                 __syncthreads();  // last iteration's gathers are done with the old tile
                 // flat over the tile, 8 independent loads in flight per lane
@@ -560,22 +595,18 @@ __global__ __launch_bounds__(64 * NW) void icgn_wave_kernel(IcgnArgs a)
             // resident tile or, if the warped support outgrew it, from HBM
             auto gather = [&](auto from_tile, int i, int j) {
                 const float dx = (float)i - hx, dy = (float)j - hy;
-                const float lu = fmaf(a11, dx, fmaf(a12, dy, cu));
-                const float lv = fmaf(a21, dx, fmaf(a22, dy, cv));
-                const float fu = floorf(lu), fvv = floorf(lv);
-                float wu[4], wv[4];
-                keys_w(lu - fu, wu);
-                keys_w(lv - fvv, wv);
-                const int iu = (int)fu - 1, iv = (int)fvv - 1;
-                float g = 0.f;
                 if constexpr (decltype(from_tile)::value) {
-                    const float *b = tile + iu + iv * tnx;
-#pragma unroll
-                    for (int q = 0; q < 4; ++q) {
-                        const float *c4 = b + q * tnx;
-                        g = fmaf(wv[q], fmaf(wu[0], c4[0], fmaf(wu[1], c4[1], fmaf(wu[2], c4[2], wu[3] * c4[3]))), g);
-                    }
+                    return split_gather(sw, tile, tnx, dx, dy);
                 } else {
+                    // the same taps and weights, read from HBM (clamped to the image)
+                    const float su = fmaf(sw.p1, dx, fmaf(sw.p3, dy, sw.fu));
+                    const float sv = fmaf(sw.p2, dx, fmaf(sw.p4, dy, sw.fv));
+                    const float fsu = floorf(su), fsv = floorf(sv);
+                    float wu[4], wv[4];
+                    keys_w(__builtin_amdgcn_fractf(su), wu);
+                    keys_w(__builtin_amdgcn_fractf(sv), wv);
+                    const int iu = (int)((dx + sw.cu) + fsu), iv = (int)((dy + sw.cv) + fsv);
+                    float g = 0.f;
 #pragma unroll
                     for (int q = 0; q < 4; ++q) {
                         const int y = min(max(tv0 + iv + q, 1), fr.W);
@@ -585,8 +616,8 @@ __global__ __launch_bounds__(64 * NW) void icgn_wave_kernel(IcgnArgs a)
                         for (int r = 0; r < 4; ++r) s = fmaf(wu[r], gld(col, (size_t)(min(max(tu0 + iu + r, 1), fr.H) - 1)), s);
                         g = fmaf(wv[q], s, g);
                     }
+                    return g;
                 }
-                return g;
             };
             auto gather_rt = [&](int i, int j) {
                 return use_tile ? gather(std::true_type{}, i, j) : gather(std::false_type{}, i, j);
@@ -599,6 +630,15 @@ __global__ __launch_bounds__(64 * NW) void icgn_wave_kernel(IcgnArgs a)
             // one iteration to the next, so this pass is usually the final one.
             const bool gspec = spec_next;
             const unsigned fbits = (MODE == 2 && fsel) ? selbits : s0bits;
+            // residual form of the sums (as icgn_lean.hip, same operations in
+            // the same order): S1, S2 need the whole window as the nonzero set
+            // (no mask region: rf), T the set-up's f side (rt; the 2-DOF
+            // solver's rebuilt f side takes the plain sums)
+            const bool rf = rform_f && !gspec && !(MODE == 2 && fsel);
+            const bool rt = !(MODE == 2 && fsel);
+            const float kf = (float)kap;
+            const float ck = (float)((1.0 - (double)kf) * (double)cf);
+            const float c1k = (float)((1.0 + (double)kf) * (double)cf);
             float gr[PPT];
             float acc[2 + NJ];
 #pragma unroll
@@ -627,12 +667,16 @@ __global__ __launch_bounds__(64 * NW) void icgn_wave_kernel(IcgnArgs a)
                     any_nan |= valid && isnan(g);
                     nzb |= (valid && g != 0.f) ? (1u << k) : 0u;
                     const float ge = (valid && (!gspec || ((selbits >> k) & 1u))) ? g : 0.f;
+                    const float fv = fwin[p];
                     const float d = ge - cf;
+                    const float r = fmaf(-kf, fv, ge) - ck;
+                    const float qv = fmaf(kf, fv, ge) - c1k;
                     nz = valid && (!quad || fabsf(ge) > 1e-10f);
-                    acc[0] += nz ? d : 0.f;
-                    acc[1] += nz ? d * d : 0.f;
+                    const float s1 = rf ? r : d, s2 = rf ? qv : d, t = rt ? r : d;
+                    acc[0] += nz ? s1 : 0.f;
+                    acc[1] = nz ? fmaf(s1, s2, acc[1]) : acc[1];
                     const bool s0 = valid && ((fbits >> k) & 1u);
-                    const float u = s0 ? fxr[k] * d : 0.f, v = s0 ? fyr[k] * d : 0.f;
+                    const float u = s0 ? fxr[k] * t : 0.f, v = s0 ? fyr[k] * t : 0.f;
                     if (MODE == 6) {
                         const float dx = (float)i - hx, dy = (float)j - hy;
                         acc[2] = fmaf(dx, u, acc[2]);
@@ -779,6 +823,8 @@ __global__ __launch_bounds__(64 * NW) void icgn_wave_kernel(IcgnArgs a)
             double b[6] = {0, 0, 0, 0, 0, 0};
             if (!literal) {
                 double n, S1, S2, T[6], SJv[6], FJv[6];
+                double dg = 0, var = 0;
+                bool have_stats = false;
 #pragma unroll
                 for (int q = 0; q < NJ; ++q) {
                     SJv[q] = L.SJ[q];
@@ -791,6 +837,24 @@ __global__ __launch_bounds__(64 * NW) void icgn_wave_kernel(IcgnArgs a)
                     S2 = acc[1];
 #pragma unroll
                     for (int q = 0; q < NJ; ++q) T[q] = acc[2 + q];
+                    if (rf) {
+                        // the residual form takes the nonzero set of g as the whole window
+                        if (n == (double)npix) {
+                            double Tr[NJ];
+                            residual_stats<NJ>(n, acc, kf, ck, c1k, cf, L.F1c, L.F2c, L.FJc, L.SJ,
+                                               dg, var, Tr);
+#pragma unroll
+                            for (int q = 0; q < NJ; ++q) T[q] = Tr[q];
+                            have_stats = true;
+                        } else {
+                            literal = true;
+                        }
+                    } else if (rt) {
+                        const double k = kf;
+                        const double corr = (double)ck - (1.0 - k) * (double)cf;
+#pragma unroll
+                        for (int q = 0; q < NJ; ++q) T[q] = (double)acc[2 + q] + k * L.FJc[q] + corr * L.SJ[q];
+                    }
                 } else {
                     // the same sums with g on the current region (or on no
                     // region); Subpb1 also rebuilds f, grad f, H and the f
@@ -888,15 +952,18 @@ __global__ __launch_bounds__(64 * NW) void icgn_wave_kernel(IcgnArgs a)
                     }
                 }
                 // gbar - c = S1/n, bottomg^2 = S2 - S1^2/n; b = FJ - (bf/bg) (T - (gbar-c) SJ)
-                const double inv_n = __builtin_amdgcn_rcp(n);
-                const double dg = S1 * inv_n;
-                const double var = S2 - S1 * dg;
+                if (!have_stats) {
+                    dg = S1 / n;
+                    var = S2 - S1 * dg;
+                }
                 const double bf = bottomf;
-                if (!(n >= 2) || !(bf > 0) || !(var > 0) || !isfinite(bf) || !isfinite(var) ||
-                    !isfinite(dg)) {
+                if (literal || !(n >= 2) || !(bf > 0) || !(var > 0) || !isfinite(bf) ||
+                    !isfinite(var) || !isfinite(dg)) {
                     literal = true;
                 } else {
-                    const double ratio = bf * __builtin_amdgcn_rsq(var);
+                    const double sv = sqrt(var);
+                    const double ratio = bf / sv;
+                    kap = sv / bf;
 #pragma unroll
                     for (int q = 0; q < NJ; ++q) b[q] = FJv[q] - ratio * (T[q] - dg * SJv[q]);
                 }
@@ -1025,6 +1092,7 @@ __global__ __launch_bounds__(64 * NW) void icgn_wave_kernel(IcgnArgs a)
                 group_reduce<NW, 0, NJ, 0>(L, par, nullptr, vb, nullptr);
 #pragma unroll
                 for (int q = 0; q < NJ; ++q) b[q] = bfx * vb[q];
+                if (bfx > 0 && bg > 0 && isfinite(bg / bfx)) kap = bg / bfx;
             }
             // norm test, Gauss-Newton step, inverse-compositional update
             double nb2 = 0.0;
@@ -1033,7 +1101,7 @@ __global__ __launch_bounds__(64 * NW) void icgn_wave_kernel(IcgnArgs a)
                 for (int q = 0; q < 6; ++q) nb2 += b[q] * b[q];
             } else {
                 // :379 tempb = 2 b / bottomf^2 + mu (P - UOld - vdual)
-                const double s2 = 2.0 * __builtin_amdgcn_rcp(bottomf * bottomf);
+                const double s2 = 2.0 / (bottomf * bottomf);
                 b[0] = b[0] * s2 + a.mu * ((P[4] - uo_u) - vd_u);
                 b[1] = b[1] * s2 + a.mu * ((P[5] - uo_v) - vd_v);
                 nb2 = b[0] * b[0] + b[1] * b[1];
@@ -1041,9 +1109,14 @@ __global__ __launch_bounds__(64 * NW) void icgn_wave_kernel(IcgnArgs a)
             abs2 = nb2;
             if (it == 1) {
                 init_ok = nb2 > tol2;  // normOfWNewInit > tol
-                inv_init2 = __builtin_amdgcn_rcp(nb2);
+                inv_init2 = 1.0 / nb2;
             }
             rel2 = init_ok ? nb2 * inv_init2 : 0.0;
+            if (near_tol(abs2, tol2, a.guard2_abs) || (init_ok && near_tol(rel2, tol2, a.guard2))) {
+                state = 3;  // the test is within FP32 reach of tol: FP64 re-solve
+                exact = true;
+                break;
+            }
             if (rel2 < tol2 || abs2 < tol2) {
                 state = 1;
                 break;
@@ -1071,8 +1144,10 @@ __global__ __launch_bounds__(64 * NW) void icgn_wave_kernel(IcgnArgs a)
         }
         if (tid == 0) {
             if (state == 3) {
-                a.status[idx] = ST_DEFER;
-                if (a.defer_out) atomicOr(a.defer_out, 1);
+                a.status[idx] = exact ? ST_EXACT : ST_DEFER;
+                defer_append(a, idx, gld(a.wx, (size_t)idx), MODE == 2 ? gld(a.wy, (size_t)idx)
+                                                                     : gld(a.wx, (size_t)idx));
+                if (exact && a.exact_count) atomicAdd(a.exact_count, 1);
             } else {
                 double Pout[6];
                 for (int k = 0; k < 6; ++k) Pout[k] = P[k];
@@ -1101,7 +1176,8 @@ __global__ __launch_bounds__(64 * NW) void icgn_wave_kernel(IcgnArgs a)
 template <int PPT, int MODE, int NW>
 hipError_t launch_wave(const IcgnArgs &a, int grid, hipStream_t s)
 {
-    const size_t lds = sizeof(float) * (size_t)a.tile_cap;
+    // the g tile, then f of every pixel slot
+    const size_t lds = sizeof(float) * ((size_t)a.tile_cap + 64 * PPT * NW);
     hipLaunchKernelGGL((icgn_wave_kernel<PPT, MODE, NW>), dim3(grid), dim3(64 * NW), lds, s, a);
     return hipGetLastError();
 }

@@ -18,8 +18,12 @@ struct IcgnArgs {
     double *U, *F, *H21;    // outputs (F, H21: 6-DOF)
     int *status;
     int *work_counter;      // persistent mode: device queue head (nullptr -> 1 subset/block)
-    const int *defer_flag;  // fallback launch: exit at once when no subset was deferred
-    int *defer_out;         // main launch: set to 1 when a subset is deferred
+    const int *defer_flag;  // fallback launch: number of subsets listed for it
+    int *defer_out;         // main launch: count of subsets listed for the small fallback launch
+    int *defer_out2;        // main launch: count of subsets listed for the large fallback launch
+    int *defer_list;        // N entries: subsets for the small fallback launch from the front
+                            // (windows of up to kSmallFallbackW px), the others from the back
+    int list_back;          // fallback launch: take the list from the back
     long N;                 // subset count (array extents)
     long n_items;           // items of this launch
     long item0;             // first item (one launch per subset uses this)
@@ -30,6 +34,9 @@ struct IcgnArgs {
     int defer;              // a window larger than npix_cap is left for a fallback launch
     int only_deferred;      // fallback launch: process only subsets left by the main launch
     int *handed;            // lean kernel: set to 1 for a subset it leaves to the fallback (may be null)
+    double guard2;          // guard band of the relative stopping test, on squared norms (0 = none)
+    double guard2_abs;      // guard band of the absolute stopping test |b| < tol (0 = none)
+    int *exact_count;       // incremented for each subset left to the FP64 re-solve (may be null)
 };
 
 struct ZnssdArgs {
@@ -57,6 +64,26 @@ struct SearchArgs {
     double *x0, *y0, *u, *v, *ccmax;  // ny x nx column-major outputs
 };
 
+// Windows of up to kSmallFallbackW px on each side grow to at most 47 x 47 px
+// (funICGNQuadtree.m:68-79, funICGN_Subpb1.m:78-79, funICGN.m:58), so the
+// fallback launch for them needs room for 48 x 48 pixels only.
+constexpr int kSmallFallbackW = 32;
+constexpr int kSmallFallbackPix = 48 * 48;
+
+// A main launch leaves subset idx (given window wx x wy) to a fallback launch.
+__device__ __forceinline__ void defer_append(const IcgnArgs &a, long idx, int wx, int wy)
+{
+    if (wx <= kSmallFallbackW && wy <= kSmallFallbackW) {
+        if (!a.defer_out) return;
+        const int pos = atomicAdd(a.defer_out, 1);
+        if (a.defer_list) a.defer_list[pos] = (int)idx;
+    } else {
+        if (!a.defer_out2) return;
+        const int pos = atomicAdd(a.defer_out2, 1);
+        if (a.defer_list) a.defer_list[a.N - 1 - pos] = (int)idx;
+    }
+}
+
 hipError_t icgn_launch(const IcgnArgs &a, int mode, int bs, int grid, hipStream_t s);
 // wave-per-subset kernel: pixels per lane for a window of npix_cap pixels (0 = too large)
 // pixels per lane for windows of npix_cap pixels spread over nw wavefronts
@@ -68,7 +95,10 @@ hipError_t icgn_wave_launch(const IcgnArgs &a, int mode, int ppt, int nw, int gr
 // icgn_wave_ppt up to four wavefronts (0: too large)
 int icgn_lean_ppt(int npix_cap, int nw);
 hipError_t icgn_lean_launch(const IcgnArgs &a, int mode, int ppt, int nw, int grid, hipStream_t s);
-size_t icgn_lds_bytes(int npix_cap, int tile_cap);
+size_t icgn_lds_bytes(int npix_cap, int tile_cap, bool exact = false);
+// FP64 re-solve (workgroup kernel, 256 threads) of the subsets a launch left
+// with status ST_EXACT or ST_DEFER (only_deferred set); no tile (tile_cap 0)
+hipError_t icgn_exact_launch(const IcgnArgs &a, int mode, int bs, int grid, hipStream_t s);
 hipError_t ba_interp2_launch(const void *F, long M, long N, long P, const void *X, const void *Y,
                              long nd, void *out, bool single, hipStream_t s);
 hipError_t img_gradient_launch(const float *f, const float *mask, int H, int W, float *dfdx,

@@ -1,11 +1,12 @@
 """GPU parity: libstaqdic.so against the FP64 oracle on the same inputs.
 
 Tolerances: integer-search offsets must be identical; bicubic FP64 gathers
-agree to 1e-12; FP32 paths to FP32 rounding; converged IC-GN displacements to
-1e-3 px and displacement gradients to 1e-4 (the FP32 residual sums and the FP32
-device gradient move the converged point by ~1e-5 px); statuses (iteration
-counts and the reference's codes) must agree for all but a handful of subsets
-whose stopping test sits within FP32 rounding of the tolerance.
+agree to 1e-12; FP32 paths to FP32 rounding; every IC-GN subset converged on
+both sides, whatever its iteration count, within 1e-3 px in displacement and
+1e-5 in displacement gradient. The stop test that sits within FP32 rounding
+of the tolerance is decided by an FP64 re-solve, so the reference's codes and
+the iteration counts agree except for the rare subset the FP64 re-solve and
+the oracle still split (at most 0.2 %).
 """
 import numpy as np
 import pytest
@@ -88,30 +89,32 @@ def test_integer_search_plain(ctx, oracle, plain_cfg):
     assert np.max(np.abs(u - tu)[inside]) < 0.6 and np.max(np.abs(v - tv)[inside]) < 0.6
 
 
-def _cmp_icgn(st_g, st_o, U_g, U_o, F_g=None, F_o=None, max_status_mismatch=0.01):
+def _cmp_icgn(st_g, st_o, U_g, U_o, F_g=None, F_o=None, max_status_mismatch=0.002, name=""):
+    """Every subset converged on both sides, whatever its iteration count:
+    |dU| < 1e-3 px and |dF| < 1e-5. The reference's codes (-1, 101, 102, the
+    converged range) must agree on all but `max_status_mismatch` of the
+    subsets; the count-mismatch fraction and the worst differences are
+    printed."""
     conv_g = (st_g >= 1) & (st_g <= 100)
     conv_o = (st_o >= 1) & (st_o <= 100)
     codes = lambda s: np.where((s >= 1) & (s <= 100), 1, s)
-    assert np.mean(codes(st_g) != codes(st_o)) <= max_status_mismatch, (
-        np.unique(st_g, return_counts=True), np.unique(st_o, return_counts=True))
-    assert np.mean(np.abs(st_g - st_o) > 1) <= max_status_mismatch
+    code_mis = np.mean(codes(st_g) != codes(st_o))
     both = conv_g & conv_o
     assert both.sum() > 0
-    # same iteration count: the FP32 sums move the converged point by ~1e-5 px;
-    # one iteration more or less (the stopping test within FP32 rounding of
-    # tol): the difference is one Gauss-Newton step at the tolerance
-    same = both & (st_g == st_o)
-    near = both & (st_g != st_o)
-    assert same.sum() >= 0.95 * both.sum()
-    du = np.abs(U_g.reshape(-1, 2) - U_o.reshape(-1, 2))
-    assert np.nanmax(du[same]) < 1e-3, np.nanmax(du[same])
-    if near.any():
-        assert np.nanmax(du[near]) < 1e-2, np.nanmax(du[near])
+    cnt_mis = np.sum(both & (st_g != st_o))
+    du = np.abs(U_g.reshape(-1, 2) - U_o.reshape(-1, 2)).max(axis=1)
+    line = (f"{name} N={len(st_g)} converged(both)={both.sum()} code mismatch={code_mis:.4%} "
+            f"count mismatch={cnt_mis} ({cnt_mis / both.sum():.4%}) max|dU|={np.max(du[both]):.3e}")
     if F_g is not None:
-        dF = np.abs(F_g.reshape(-1, 4) - F_o.reshape(-1, 4))
-        assert np.nanmax(dF[same]) < 1e-4, np.nanmax(dF[same])
-        if near.any():
-            assert np.nanmax(dF[near]) < 1e-3, np.nanmax(dF[near])
+        dF = np.abs(F_g.reshape(-1, 4) - F_o.reshape(-1, 4)).max(axis=1)
+        line += f" max|dF|={np.max(dF[both]):.3e}"
+    _print(line)
+    assert code_mis <= max_status_mismatch, (
+        np.unique(st_g, return_counts=True), np.unique(st_o, return_counts=True))
+    assert np.all(np.isfinite(du[both]))
+    assert np.max(du[both]) < 1e-3, np.max(du[both])
+    if F_g is not None:
+        assert np.max(dF[both]) < 1e-5, np.max(dF[both])
     err = (st_g == -1)
     assert np.all(np.isnan(U_g.reshape(-1, 2)[err]))
     return both.sum()
@@ -172,7 +175,7 @@ def test_hole_masked_path(ctx, oracle):
     of = oracle.Frame(fr.f, fr.g, fr.mask)
     Uo, Fo, Ho, so = oracle.local_icgn(of, U0, coords, win, tol=1e-3)
     assert (so == -1).sum() > 5 and (so == 102).sum() > 0
-    _cmp_icgn(sg, so, Ug, Uo, Fg, Fo, max_status_mismatch=0.02)
+    _cmp_icgn(sg, so, Ug, Uo, Fg, Fo)
     _print("hole 384^2: icgn2")
     FOld = np.where(np.isfinite(Fo), Fo, 0.0)
     UOld = np.where(np.isfinite(Uo), Uo, U0)
@@ -180,7 +183,7 @@ def test_hole_masked_path(ctx, oracle):
     Ug2, sg2 = subpb1_raw(UOld, FOld, vd, coords, win, win, 1e-3, fr.f, fr.g, fr.mask, tol=1e-3,
                           ctx=ctx)
     Uo2, so2 = oracle.subpb1(of, UOld, FOld, vd, coords, win, win, 1e-3, tol=1e-3)
-    _cmp_icgn(sg2, so2, Ug2, Uo2, max_status_mismatch=0.02)
+    _cmp_icgn(sg2, so2, Ug2, Uo2)
 
 
 def test_reference_frames(ctx, oracle, ref_frames):
@@ -215,7 +218,7 @@ def test_reference_frames(ctx, oracle, ref_frames):
     of = oracle.Frame(f, g, m)
     Uo, Fo, Ho, so = oracle.local_icgn(of, U0, r["coords"], win, tol=1e-2)
     assert (so == -1).sum() > 50 and (so == 102).sum() > 100
-    assert _cmp_icgn(sg, so, Ug, Uo, Fg, Fo, max_status_mismatch=0.01) > 650
+    assert _cmp_icgn(sg, so, Ug, Uo, Fg, Fo) > 650
 
     wl = r["winsize_List"]
     _print("reference frames: icgn2 on winsize_List")
@@ -223,7 +226,7 @@ def test_reference_frames(ctx, oracle, ref_frames):
     Ug2, sg2 = subpb1_raw(U1, r["F1"], vd, r["coords"], wl[:, 0], wl[:, 1], 1e-3, f, g, m,
                           tol=1e-2, ctx=ctx)
     Uo2, so2 = oracle.subpb1(of, U1, r["F1"], vd, r["coords"], wl[:, 0], wl[:, 1], 1e-3, tol=1e-2)
-    _cmp_icgn(sg2, so2, Ug2, Uo2, max_status_mismatch=0.01)
+    _cmp_icgn(sg2, so2, Ug2, Uo2)
 
 
 def test_forced_status_codes(ctx, oracle):
@@ -362,7 +365,7 @@ def test_multi_wave_forms_on_masked_subsets(ctx, oracle, form):
     Ug, Fg, Hg, sg = b.outputs()
     of = oracle.Frame(fr.f, fr.g, fr.mask)
     Uo, Fo, Ho, so = oracle.local_icgn(of, U0, coords, win, tol=1e-3)
-    _cmp_icgn(sg, so, Ug, Uo, Fg, Fo, max_status_mismatch=0.02)
+    _cmp_icgn(sg, so, Ug, Uo, Fg, Fo)
     FOld = np.where(np.isfinite(Fo), Fo, 0.0)
     UOld = np.where(np.isfinite(Uo), Uo, U0)
     b.set_input(IN_UOLD, UOld)
@@ -370,7 +373,7 @@ def test_multi_wave_forms_on_masked_subsets(ctx, oracle, form):
     b.icgn2(params(tol=1e-3, mu=1e-3, block_size=form))
     Ug2, _, _, sg2 = b.outputs()
     Uo2, so2 = oracle.subpb1(of, UOld, FOld, np.zeros_like(UOld), coords, win, win, 1e-3, tol=1e-3)
-    _cmp_icgn(sg2, so2, Ug2, Uo2, max_status_mismatch=0.02)
+    _cmp_icgn(sg2, so2, Ug2, Uo2)
     b.close()
 
 
@@ -574,8 +577,7 @@ def test_lean_kernel_matches_general(ctx, oracle):
             same &= np.all(Fl.reshape(-1, 4) == Fg.reshape(-1, 4), axis=1) | \
                 np.isnan(Fl.reshape(-1, 4)).all(axis=1)
         assert np.mean(same) > 0.97, np.mean(same)
-        _cmp_icgn(sl, sg, Ul, Ug, Fl if with_f else None, Fg if with_f else None,
-                  max_status_mismatch=0.01)
+        _cmp_icgn(sl, sg, Ul, Ug, Fl if with_f else None, Fg if with_f else None)
 
     for nw in (1, 2, 4):
         general = 0 if nw == 1 else -nw
@@ -627,8 +629,7 @@ def test_handed_back_subsets_move_to_the_general_kernel(ctx, plain_cfg):
     assert n1 > n0
     b.icgn6(p)
     second = b.outputs()
-    _cmp_icgn(first[3], second[3], first[0], second[0], first[1], second[1],
-              max_status_mismatch=0.01)
+    _cmp_icgn(first[3], second[3], first[0], second[0], first[1], second[1])
     assert np.mean(first[3] == second[3]) > 0.99
     b.reorder()
     assert general_count() == n1
@@ -837,11 +838,11 @@ def test_bad_arguments_with_a_context(ctx):
 
 
 def test_full_size_workloads_against_oracle(ctx, oracle):
-    """The measured workloads at their full sizes: the 1024^2 hole pair with
-    the reference's 32-px windows (6016 subsets) against the oracle subset by
-    subset, with the 0.01 px accuracy gate over the subsets the reference
+    """The measured workloads at their full sizes, every subset against the
+    oracle: the 1024^2 hole pair with the reference's 32-px windows (6016
+    subsets), with the 0.01 px accuracy gate over the subsets the reference
     keeps; the 4096^2 16-bit large-deformation pair (about 98 k subsets, one
-    batch) against the oracle on a random sample of 3000 subsets."""
+    batch) with the reference's 32-px windows and with the leaf-size rule."""
     from staq_dic_amd import Batch, params, synth
     from staq_dic_amd.api import classify_local
     from staq_dic_amd._lib import IN_U0
@@ -857,38 +858,39 @@ def test_full_size_workloads_against_oracle(ctx, oracle):
     Ug, Fg, _, sg = b.outputs()
     b.close()
     Uo, Fo, _, so = oracle.local_icgn(oracle.Frame(fr.f, fr.g, fr.mask), U0, coords, win, tol=1e-3)
-    assert _cmp_icgn(sg, so, Ug, Uo, Fg, Fo) > 0.95 * len(coords)
+    assert _cmp_icgn(sg, so, Ug, Uo, Fg, Fo, name="hole 1024^2 w32") > 0.95 * len(coords)
     bad, _ = classify_local(sg)
     kept = (sg >= 1) & (sg <= 100) & ~bad
     err = np.hypot(Ug[0::2] - truth[:, 4], Ug[1::2] - truth[:, 5])[kept]
     assert np.sqrt(np.mean(err ** 2)) <= 0.01
 
-    fr, coords, win = synth.config_large()
+    fr, coords, leaf = synth.config_large()
     truth = synth.truth_at(fr.field, coords)
     U0 = np.round(truth[:, 4:6]).reshape(-1)
     ctx.set_frame(fr.f, fr.g, fr.mask, slot=0)
-    _print("large-deformation pair 4096^2, N =", len(coords))
     assert len(coords) > 90000
-    b = Batch(ctx, coords, win)
-    b.set_input(IN_U0, U0)
-    b.icgn6(params(tol=1e-3))
-    Ug, Fg, _, sg = b.outputs()
-    b.close()
-    pick = np.sort(np.random.default_rng(7).choice(len(coords), 3000, replace=False))
-    Uo, Fo, _, so = oracle.local_icgn(oracle.Frame(fr.f, fr.g, fr.mask),
-                                      U0.reshape(-1, 2)[pick].reshape(-1), coords[pick], win[pick],
-                                      tol=1e-3)
-    Ugs = Ug.reshape(-1, 2)[pick].reshape(-1)
-    Fgs = Fg.reshape(-1, 4)[pick].reshape(-1)
-    assert _cmp_icgn(sg[pick], so, Ugs, Uo, Fgs, Fo) > 0.9 * len(pick)
+    of = oracle.Frame(fr.f, fr.g, fr.mask)
+    for rule, win in (("w32", np.full(len(coords), 32, dtype=np.int64)), ("leaf", leaf)):
+        _print("large-deformation pair 4096^2,", rule, "windows, N =", len(coords))
+        b = Batch(ctx, coords, win)
+        b.set_input(IN_U0, U0)
+        b.icgn6(params(tol=1e-3))
+        Ug, Fg, _, sg = b.outputs()
+        b.close()
+        Uo, Fo, _, so = oracle.local_icgn(of, U0, coords, win, tol=1e-3)
+        assert _cmp_icgn(sg, so, Ug, Uo, Fg, Fo, name="4096^2 " + rule) > 0.9 * len(coords)
+        ok = (sg >= 1) & (sg <= 100) & (win >= 32)
+        err = np.hypot(Ug[0::2] - truth[:, 4], Ug[1::2] - truth[:, 5])[ok]
+        _print("   RMS error against the analytic field, windows >= 32 px:",
+               np.sqrt(np.mean(err ** 2)))
+        assert np.sqrt(np.mean(err ** 2)) <= 0.01
 
 
 def test_full_size_multi_pair_and_adjoint(ctx, oracle):
     """The multi-instance batch at full size (16 pairs of 512^2-2048^2 in
-    slots 0-15, 88 k subsets, one batch) against the oracle on 150 sampled
-    subsets of every pair; the adjoint pair at full size (2048^2, 15 625
-    subsets) against the FP64 oracle's cost and gradient on 2000 sampled
-    subsets."""
+    slots 0-15, 88 k subsets, one batch) against the oracle on every subset,
+    pair by pair; the adjoint pair at full size (2048^2, 15 625 subsets)
+    against the FP64 oracle's cost and gradient on every subset."""
     import torch
 
     from staq_dic_amd import Batch, params, synth
@@ -907,16 +909,17 @@ def test_full_size_multi_pair_and_adjoint(ctx, oracle):
     b.icgn6(params(tol=1e-3))
     Ug, Fg, _, sg = b.outputs()
     b.close()
-    rng = np.random.default_rng(11)
-    n_ok = 0
+    Uo = np.empty_like(Ug)
+    Fo = np.empty_like(Fg)
+    so = np.empty_like(sg)
     for j, (fr, c, w) in enumerate(pairs):
         idx = np.flatnonzero(slot == j)
-        pick = np.sort(rng.choice(idx, min(150, idx.size), replace=False))
-        Uo, Fo, _, so = oracle.local_icgn(oracle.Frame(fr.f, fr.g, fr.mask), U0[pick].reshape(-1),
-                                          coords[pick], win[pick], tol=1e-3)
-        n_ok += _cmp_icgn(sg[pick], so, Ug.reshape(-1, 2)[pick].reshape(-1), Uo,
-                          Fg.reshape(-1, 4)[pick].reshape(-1), Fo, max_status_mismatch=0.02)
-    assert n_ok > 0.9 * 150 * len(pairs)
+        u, f_, _, s = oracle.local_icgn(oracle.Frame(fr.f, fr.g, fr.mask), U0[idx].reshape(-1),
+                                        coords[idx], win[idx], tol=1e-3)
+        Uo.reshape(-1, 2)[idx] = u.reshape(-1, 2)
+        Fo.reshape(-1, 4)[idx] = f_.reshape(-1, 4)
+        so[idx] = s
+    assert _cmp_icgn(sg, so, Ug, Uo, Fg, Fo, name="16 pairs") > 0.9 * len(coords)
 
     fr, coords, win = synth.config_adjoint()
     truth = synth.truth_at(fr.field, coords)
@@ -930,14 +933,14 @@ def test_full_size_multi_pair_and_adjoint(ctx, oracle):
                          torch.tensor(win, dtype=torch.int32, device=dev), ctx)
     cost.sum().backward()
     torch.cuda.synchronize()
-    pick = np.sort(np.random.default_rng(3).choice(len(coords), 2000, replace=False))
-    _, co, go, so = oracle.znssd(oracle.Frame(fr.f, fr.g, fr.mask), P0[pick], coords[pick],
-                                 win[pick])
-    st = status.cpu().numpy()[pick]
+    _, co, go, so = oracle.znssd(oracle.Frame(fr.f, fr.g, fr.mask), P0, coords, win)
+    st = status.cpu().numpy()
     np.testing.assert_array_equal(st, so)
     ok = so == 0
     assert ok.mean() > 0.95
-    np.testing.assert_allclose(cost.detach().cpu().numpy()[pick][ok], co[ok], rtol=1e-4, atol=1e-6)
-    g = P.grad.cpu().numpy()[pick]
+    np.testing.assert_allclose(cost.detach().cpu().numpy()[ok], co[ok], rtol=1e-4, atol=1e-6)
+    g = P.grad.cpu().numpy()
     scale = np.abs(go[ok]).max(axis=0) + 1e-12
-    assert np.max(np.abs(g[ok] - go[ok]) / scale) < 2e-3
+    rel = np.max(np.abs(g[ok] - go[ok]) / scale)
+    _print("   adjoint: largest gradient error relative to each parameter's scale", rel)
+    assert rel < 2e-3

@@ -275,11 +275,10 @@ __global__ __launch_bounds__(64 * NW) void icgn_wave_kernel(IcgnArgs a)
             __syncthreads();
         }
 
-        // per-pixel df/dx, df/dy (and f*mask for the 2-DOF region rebuild) in
-        // registers, s0bits = pixels inside the setup region; f statistics and
-        // steepest-descent sums in one pass about the shift c0 = f at the centre
-        constexpr bool FM = MODE == 2 && PPT <= 9;
-        float fxr[PPT], fyr[PPT], fmr[FM ? PPT : 1];
+        // per-pixel df/dx, df/dy in registers, s0bits = pixels inside the
+        // setup region; f statistics and steepest-descent sums in one pass
+        // about the shift c0 = f at the centre
+        float fxr[PPT], fyr[PPT];
         unsigned s0bits = ~0u;
         // convergence bookkeeping on squared norms (funICGNQuadtree.m:373-386):
         // rel2 = (|b| / |b_1|)^2 or 0, abs2 = |b|^2; a square root is taken
@@ -303,7 +302,6 @@ __global__ __launch_bounds__(64 * NW) void icgn_wave_kernel(IcgnArgs a)
             for (int k = 0; k < PPT; ++k) {
                 const int p = pixel(k);
                 fxr[k] = fyr[k] = 0.f;
-                if (FM) fmr[FM ? k : 0] = 0.f;
                 if (p < npix) {
                     const int j = (int)(((float)p + 0.5f) * rnx), i = p - j * nx;
                     const int x = xa + i, y = ya + j;
@@ -312,7 +310,6 @@ __global__ __launch_bounds__(64 * NW) void icgn_wave_kernel(IcgnArgs a)
                     fyr[k] = gld(fr.dfdy, gi);
                     float f = fidx(fr.f, fr.H, x, y);
                     if (quad) f *= fidx(fr.mask, fr.H, x, y);
-                    if (FM) fmr[FM ? k : 0] = f;
                     const bool s = !masked_f || ((L.sel0w[j] >> i) & 1ull);
                     if (!s) s0bits &= ~(1u << k);
                     const float fx = s ? fxr[k] : 0.f, fy = s ? fyr[k] : 0.f;
@@ -816,10 +813,7 @@ __global__ __launch_bounds__(64 * NW) void icgn_wave_kernel(IcgnArgs a)
                 stands = !gspec;
             const bool resum = !stands;
             spec_next = masked_it && (MODE == 2 || !region_is_nz);
-            const bool fs = MODE == 2 && fsel;
-            // the 2-DOF region rebuild needs f*mask per pixel; windows too large
-            // to keep it in registers take the literal path
-            bool literal = fs && !FM;
+            bool literal = false;
             double b[6] = {0, 0, 0, 0, 0, 0};
             if (!literal) {
                 double n, S1, S2, T[6], SJv[6], FJv[6];
@@ -896,7 +890,13 @@ __global__ __launch_bounds__(64 * NW) void icgn_wave_kernel(IcgnArgs a)
                                 r[2] += fx * d;
                                 r[3] += fy * d;
                                 if (fre) {
-                                    const float tf = sf ? fmr[FM ? k : 0] : 0.f;
+                                    // f*mask on the new region, read again (the
+                                    // region changes in few iterations)
+                                    float tf = 0.f;
+                                    if (sf) {
+                                        const int j = (int)(((float)p + 0.5f) * rnx), i = p - j * nx;
+                                        tf = fidx(fr.f, fr.H, xa + i, ya + j) * fidx(fr.mask, fr.H, xa + i, ya + j);
+                                    }
                                     const float df = tf - cf;
                                     const bool fnz = fabsf(tf) > 1e-10f;
                                     q6[0] += fnz ? 1.f : 0.f;

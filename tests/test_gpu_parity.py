@@ -345,12 +345,14 @@ def test_batch_forms_agree(ctx, plain_cfg):
     b.close()
 
 
-@pytest.mark.parametrize("form", [-2, -4, 1, 4])
+@pytest.mark.parametrize("form", [0, -1, -2, -4, 1, 4, 64, 256])
 def test_multi_wave_forms_on_masked_subsets(ctx, oracle, form):
-    """Two / four wavefronts per subset through the masked path (region
-    floods, window growth, -1 and 102) and the 2-DOF step, against the oracle;
-    forms 1 and 4 force the lean kernel on every bin, so every subset that
-    touches the mask goes through its hand-over to the fallback launch."""
+    """Every kernel form forced on every bin through the masked path (region
+    floods, window growth, -1 and 102) and the 2-DOF step, against the oracle:
+    one, two and four wavefronts per subset (0 / -1, -2, -4), the workgroup
+    kernel (64, 256); forms 1 and 4 force the lean kernel on every bin, so
+    every subset that touches the mask goes through its hand-over to the
+    fallback launch."""
     from staq_dic_amd import Batch, params, synth
     from staq_dic_amd._lib import IN_FOLD, IN_U0, IN_UOLD
     fr, coords, win = synth.config_hole(H=384, W=384, n_speckles=1700, max_disp=3.0, radius=60,

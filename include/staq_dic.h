@@ -166,6 +166,21 @@ int staq_subpb1(staq_ctx *ctx, const staq_params *p, long N, const double *UOld,
                 const double *coords, const long *wx, const long *wy, const int *slot,
                 double *U, int *status);
 
+/* No counterpart in the reference: staq_local_icgn for callers that cannot
+ * allocate device memory (code under HIP graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc). Every pointer is a device pointer:
+ * U0 (2N), coords (N x 2, column-major), winsize (N, int32, at most wmax),
+ * outputs U (2N), F (4N), Hpar (N x 21), status (N), as staq_local_icgn
+ * returns them; every subset on frame `slot`. The step's counters and list
+ * of deferred subsets live in a fixed device array of the library (one per
+ * GPU, about 4 MiB, kept until the process ends), so N is at most 1048576
+ * and wmax at most 64 (STAQ_EINVAL beyond). Calls that overlap on one GPU
+ * share that array and race: serialize them (one stream, or events).
+ * Asynchronous on `stream` (NULL = the context stream). */
+int staq_local_icgn_noalloc(staq_ctx *ctx, const staq_params *p, int slot, long N, int wmax,
+                            const double *U0, const double *coords, const int *winsize,
+                            double *U, double *F, double *Hpar, int *status, void *stream);
+
 /* ---- ZNSSD mismatch and its adjoint (no reference counterpart) ---------- */
 /* cost_i = sum_p (fhat - ghat(W(P_i)))^2 over the (w+1)^2 window of node i
  * (whole-window statistics); status 1 = window or warp outside the image.

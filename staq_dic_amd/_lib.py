@@ -60,6 +60,8 @@ SIGNATURES = {
                                   _dp, _dp, _ip]),
     "staq_subpb1": (C.c_int, [_vp, C.POINTER(StaqParams), C.c_long, _dp, _dp, _dp, _dp, _dp, _lp,
                               _lp, _ip, _dp, _ip]),
+    "staq_local_icgn_noalloc": (C.c_int, [_vp, C.POINTER(StaqParams), C.c_int, C.c_long, C.c_int,
+                                          _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp]),
     "staq_znssd_fwd": (C.c_int, [_vp, C.c_long, _dp, _dp, _lp, _ip, _dp, _ip]),
     "staq_znssd_bwd": (C.c_int, [_vp, C.c_long, _dp, _dp, _lp, _ip, _dp, _dp, _dp]),
     "staq_znssd_fwd_dev": (C.c_int, [_vp, C.c_long, C.c_int, _vp, _vp, _vp, _vp, _vp, _vp, _vp]),

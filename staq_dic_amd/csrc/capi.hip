@@ -32,6 +32,16 @@ __global__ void reset_ints_kernel(int *p, int n)
     if (i < n) p[i] = 0;
 }
 
+// Device memory of staq_local_icgn_noalloc, one copy per GPU for the life of
+// the process: the step's counters (queue heads and list lengths of the
+// fallback launches, the count of FP64 re-solves) in the first 64 ints, then
+// the defer list, one int per subset. The name is a host-side stand-in: host
+// code uses only the address hipGetSymbolAddress returns for it on each
+// device (staq_ctx::noalloc).
+constexpr long kNoAllocMaxN = 1L << 20;  // subsets per call
+constexpr int kNoAllocHead = 64;         // ints before the defer list
+static __device__ __align__(256) int scratch[kNoAllocHead + kNoAllocMaxN];
+
 __global__ void chain_kernel(const double *U, const double *F, double *UOld, double *FOld,
                              double *vdual, long N)
 {
@@ -98,6 +108,7 @@ struct staq_ctx {
     std::vector<Frame> frames;
     DevBuf d_frames;
     DevBuf s[12];  // scratch for host-pointer calls
+    int *noalloc = nullptr;  // this device's copy of `scratch` (staq_local_icgn_noalloc)
 };
 
 struct Bin {
@@ -187,6 +198,13 @@ int staq_ctx_create(staq_ctx **out, int device)
     staq_ctx *c = new staq_ctx();
     c->device = device;
     if (hipStreamCreate(&c->stream) != hipSuccess) {
+        delete c;
+        return STAQ_EHIP;
+    }
+    // this device's copy of the no-alloc entry's memory (device current)
+    if (hipGetSymbolAddress((void **)&c->noalloc, HIP_SYMBOL(scratch)) != hipSuccess ||
+        !c->noalloc) {
+        hipStreamDestroy(c->stream);
         delete c;
         return STAQ_EHIP;
     }
@@ -981,6 +999,22 @@ extern "C" int staq_batch_bin_of(staq_batch *b, int *bin)
     return STAQ_OK;
 }
 
+// Stopping rule, damping, iteration cap and guard band of a local step.
+static void set_step_params(IcgnArgs &a, const staq_params *p)
+{
+    a.tol = p->tol;
+    a.mu = p->mu;
+    a.plain = p->plain;
+    a.max_it = p->max_iter > 0 ? p->max_iter : 100;
+    // the band on |b| / tol, (1 - g, 1 + g), as a band on the squared norms
+    const double g = p->guard > 0 ? p->guard : 0.0;
+    a.guard2 = g > 0 ? g * (2.0 + g) : 0.0;
+    // |b| near tol is small in absolute terms, where the FP32 rounding is a
+    // larger fraction of it: a band four times as wide
+    const double ga = 4.0 * g;
+    a.guard2_abs = ga > 0 ? ga * (2.0 + ga) : 0.0;
+}
+
 static IcgnArgs base_args(staq_batch *b, const staq_params *p)
 {
     IcgnArgs a;
@@ -1002,18 +1036,39 @@ static IcgnArgs base_args(staq_batch *b, const staq_params *p)
     a.handed = b->handed.as<int>();
     a.defer_list = b->deflist.as<int>();
     a.N = b->N;
-    a.tol = p->tol;
-    a.mu = p->mu;
-    a.plain = p->plain;
-    a.max_it = p->max_iter > 0 ? p->max_iter : 100;
-    // the band on |b| / tol, (1 - g, 1 + g), as a band on the squared norms
-    const double g = p->guard > 0 ? p->guard : 0.0;
-    a.guard2 = g > 0 ? g * (2.0 + g) : 0.0;
-    // |b| near tol is small in absolute terms, where the FP32 rounding is a
-    // larger fraction of it: a band four times as wide
-    const double ga = 4.0 * g;
-    a.guard2_abs = ga > 0 ? ga * (2.0 + ga) : 0.0;
+    set_step_params(a, p);
     return a;
+}
+
+// The fallback launches over the defer list of a step: the FP64 re-solve of
+// the subsets whose stopping test fell inside the guard band and of the
+// windows that outgrew their kernel. Launched unconditionally (capturable in
+// a graph); their workgroups leave at once when the list is empty. Two
+// launches: windows of up to 32 px (LDS for their grown windows only, so
+// that five subsets run per CU), then the larger ones, gathering from HBM.
+// q[0], q[1]: queue head and list length of the first; q[3], q[4]: of the
+// second (q[2] counts the FP64 re-solves).
+static hipError_t fallback_launches(IcgnArgs a, int mode, int *q, int fb_npix_cap, hipStream_t s)
+{
+    a.item0 = 0;
+    a.n_items = a.N;
+    a.defer = 0;
+    a.guard2 = 0;
+    a.guard2_abs = 0;
+    a.only_deferred = 1;
+    a.npix_cap = std::min(kSmallFallbackPix, fb_npix_cap);
+    a.tile_cap = tile_side(48) * tile_side(48);  // the warped support of a 48 x 48 window
+    a.work_counter = q;
+    a.defer_flag = q + 1;
+    a.list_back = 0;
+    hipError_t e = icgn_exact_launch(a, mode, 256, (int)std::min<long>(a.N, 1280), s);
+    if (e != hipSuccess) return e;
+    a.npix_cap = fb_npix_cap;
+    a.tile_cap = 0;  // gathers from HBM
+    a.work_counter = q + 3;
+    a.defer_flag = q + 4;
+    a.list_back = 1;
+    return icgn_exact_launch(a, mode, 256, (int)std::min<long>(a.N, 256), s);
 }
 
 static int occupancy_grid(int bs, size_t lds, long n)
@@ -1109,32 +1164,9 @@ static int enqueue_step(staq_batch *b, const staq_params *p, int mode, hipStream
         }
     }
     if (bin_only < 0) {
-        // fallback over the whole batch: the FP64 re-solve of the subsets
-        // whose stopping test fell inside the guard band and of the windows
-        // that outgrew their bin; launched unconditionally (graph-capturable),
-        // its workgroups leave at once when the defer flag is clear
-        // two launches: windows of up to 32 px (LDS for their grown windows
-        // only, so that five subsets run per CU), then the larger ones
-        IcgnArgs a = base_args(b, p);
-        a.item0 = 0;
-        a.n_items = b->N;
-        a.defer = 0;
-        a.guard2 = 0;
-        a.guard2_abs = 0;
-        a.only_deferred = 1;
+        // fallback over the whole batch
         if (timed) HIPCHK(ctx, hipEventRecord(b->tev[2 * nb], s));
-        a.npix_cap = std::min(kSmallFallbackPix, b->fb_npix_cap);
-        a.tile_cap = tile_side(48) * tile_side(48);  // the warped support of a 48 x 48 window
-        a.work_counter = ints + nb;
-        a.defer_flag = ints + nb + 1;
-        a.list_back = 0;
-        HIPCHK(ctx, icgn_exact_launch(a, mode, 256, (int)std::min<long>(b->N, 1280), s));
-        a.npix_cap = b->fb_npix_cap;
-        a.tile_cap = 0;  // gathers from HBM
-        a.work_counter = ints + nb + 3;
-        a.defer_flag = ints + nb + 4;
-        a.list_back = 1;
-        HIPCHK(ctx, icgn_exact_launch(a, mode, 256, (int)std::min<long>(b->N, 256), s));
+        HIPCHK(ctx, fallback_launches(base_args(b, p), mode, ints + nb, b->fb_npix_cap, s));
         if (timed) HIPCHK(ctx, hipEventRecord(b->tev[2 * nb + 1], s));
     }
     return STAQ_OK;
@@ -1423,6 +1455,65 @@ int staq_subpb1(staq_ctx *ctx, const staq_params *p, long N, const double *UOld,
     if (!rc) rc = staq_batch_get_output(b, U, nullptr, nullptr, status);
     staq_batch_destroy(b);
     return rc;
+}
+
+// ---------------------------------------------------------------------------
+// the 6-DOF local step without device allocation
+// ---------------------------------------------------------------------------
+int staq_local_icgn_noalloc(staq_ctx *ctx, const staq_params *p, int slot, long N, int wmax,
+                            const double *U0, const double *coords, const int *winsize,
+                            double *U, double *F, double *Hpar, int *status, void *stream)
+{
+    if (!ctx || !p || !U0 || !coords || !winsize || !U || !F || !Hpar || !status)
+        return fail(ctx, STAQ_EINVAL, "null argument");
+    if (N <= 0 || N > kNoAllocMaxN)
+        return fail(ctx, STAQ_EINVAL, "staq_local_icgn_noalloc takes 1 to 1048576 subsets per call");
+    if (wmax < 2 || wmax > 64)
+        return fail(ctx, STAQ_EINVAL, "staq_local_icgn_noalloc takes windows of 2 to 64 px");
+    if (slot < 0 || (size_t)slot >= ctx->frames.size() || !ctx->frames[slot].set)
+        return fail(ctx, STAQ_ENOFRAME, "frame slot not set");
+    if (!ctx->noalloc) return fail(ctx, STAQ_EHIP, "no device memory for the no-alloc entry");
+    hipSetDevice(ctx->device);
+    hipStream_t s = pick_stream(ctx, stream);
+    int *q = ctx->noalloc;  // counters, then the defer list
+    IcgnArgs a;
+    std::memset(&a, 0, sizeof(a));
+    a.frames = ctx->d_frames.as<FrameDev>() + slot;  // every subset on this slot
+    a.coords = coords;
+    a.wx = a.wy = winsize;
+    a.U0 = U0;
+    a.U = U;
+    a.F = F;
+    a.H21 = Hpar;
+    a.status = status;
+    a.defer_list = q + kNoAllocHead;
+    a.N = N;
+    a.item0 = 0;
+    a.n_items = N;
+    a.defer = 1;
+    a.defer_out = q + 1;
+    a.defer_out2 = q + 4;
+    a.exact_count = q + 2;
+    set_step_params(a, p);
+    // the largest window after the mask-driven growth
+    const int gw = grown_window(wmax);
+    const int gnpix = (gw + 1) * (gw + 1);
+    hipLaunchKernelGGL(reset_ints_kernel, dim3(1), dim3(64), 0, s, q, 5);
+    HIPCHK(ctx, hipGetLastError());
+    const int ppt = icgn_wave_ppt(gnpix, 4);
+    if (ppt) {
+        // four wavefronts per subset; windows that outgrow them go to the fallback
+        const int side = tile_side((int)std::sqrt(64.0 * ppt * 4));
+        a.tile_cap = side * side;
+        HIPCHK(ctx, icgn_wave_launch(a, 6, ppt, 4, (int)N, s));
+    } else {
+        // the workgroup kernel holds the grown windows
+        a.npix_cap = gnpix;
+        a.tile_cap = tile_side(gw + 1) * tile_side(gw + 1);
+        HIPCHK(ctx, icgn_launch(a, 6, 256, (int)N, s));
+    }
+    HIPCHK(ctx, fallback_launches(a, 6, q, gnpix, s));
+    return STAQ_OK;
 }
 
 // ---------------------------------------------------------------------------

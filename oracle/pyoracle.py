@@ -144,12 +144,15 @@ def normxcorr2_full(Cm, D):
     return out
 
 
-def integer_search_mg(f, g, gridx, gridy, winsize, winstepsize):
+def integer_search_mg(f, g, gridx, gridy, winsize, winstepsize, trace=None):
     """funIntegerSearchMg (func/funIntegerSearchMg.m:3-285) with direct FP64
     normxcorr2 maps: returns the scattered box centres and shifts before the
     gridfit (tempx, tempy, tempu, tempv, tempPhi) and the final grid limits
     (gridx, gridy after :256-261).  Small images only (the level-1 map is
-    computed by direct sums)."""
+    computed by direct sums).  ``trace``: a list that receives one dict per
+    level (level 1: the shift of the whole region; levels >= 2: the f and g
+    boxes searched and the shifts found, before the border and even-width
+    fixes of :221-254)."""
     f = np.asarray(f, dtype=np.float64)
     g = np.asarray(g, dtype=np.float64)
     H, W = f.shape
@@ -172,6 +175,9 @@ def integer_search_mg(f, g, gridx, gridy, winsize, winstepsize):
     v1, u1, _ = fpk(X)
     u = u1 - np.ceil((X.shape[0] + 1) / 2.0)
     v = v1 - np.ceil((X.shape[1] + 1) / 2.0)
+    if trace is not None:
+        trace.append({"xf": [list(map(float, gx))], "yf": [list(map(float, gy))],
+                      "u": [u], "v": [v]})
 
     def clamp_boxes(xf, yf, xg, yg):
         # :30-37 / :221-245: move g boxes inside the border gap, f boxes alike
@@ -262,6 +268,10 @@ def integer_search_mg(f, g, gridx, gridy, winsize, winstepsize):
                 phi[b] = mf
             except ValueError:
                 un[b] = np.nan; vn[b] = np.nan; phi[b] = 0.0
+        if trace is not None:
+            trace.append({"xf": [list(r) for r in nxf], "yf": [list(r) for r in nyf],
+                          "xg": [list(r) for r in nxg], "yg": [list(r) for r in nyg],
+                          "u": list(un), "v": list(vn)})
         if nwx < winsize and nwy < winsize:
             break
         xf, yf = nxf, nyf

@@ -59,6 +59,94 @@ def test_oracle_levels_recover_a_translation(oracle):
     assert np.median(phi[ok]) > 0.99
 
 
+def _piecewise_pair():
+    """256 x 256 pair sampled from one analytic speckle pattern (rounded to
+    FP32, as frames are held): moved by s1 in the quadrant x <= 128, y >= 126
+    of g and by s0 elsewhere."""
+    H = W = 256
+    rng = np.random.default_rng(17)
+    n = 800
+    cx, cy = rng.uniform(-10, H + 10, n), rng.uniform(-10, W + 10, n)
+    s = rng.uniform(1.8, 3.0, n)
+
+    def pat(X, Y):
+        out = np.zeros_like(X)
+        for k in range(n):
+            out += np.exp(-((X - cx[k]) ** 2 + (Y - cy[k]) ** 2) / (2 * s[k] ** 2))
+        return out
+
+    X, Y = np.meshgrid(np.arange(1, H + 1, dtype=float), np.arange(1, W + 1, dtype=float),
+                       indexing="ij")
+    s0 = np.array([-0.35, -0.4])
+    s1 = np.array([2.4, -3.4])
+    f = pat(X, Y)
+    g = np.where((X <= 128) & (Y >= 126), pat(X - s1[0], Y - s1[1]), pat(X - s0[0], Y - s0[1]))
+    r32 = lambda a: a.astype(np.float32).astype(np.float64)
+    return r32(f), r32(g), s0, s1
+
+
+def test_oracle_levels_on_a_piecewise_field(oracle):
+    """The level bookkeeping of funIntegerSearchMg.m:17-254 on a field with
+    two motions: g is sampled exactly from one analytic speckle pattern moved
+    by s1 = (2.4, -3.4) in the quadrant x <= 128, y >= 126 of g and by
+    s0 = (-0.35, -0.4) elsewhere (256 x 256, winsize 32).  The boxes follow
+    from the rules by hand: border gap 1 + round(24) = 25; level 1 finds
+    u in (0, 1), v in (-1, 0), so its g box moves by ceil = (1, 0) and is
+    clamped at 256 - 25, which shrinks the f box to x in [25, 230], trimmed to
+    the even width [25, 229] (:30-41); levels 2-4 split every box in four
+    (ratio 1), keep the halves' shared midpoints, and trim odd widths after
+    levels 2 and 3 (:221-254); the s1 quadrant's g boxes move by
+    ceil(u) = 3, ceil(v) = -3 from level 2 on.  The last level holds
+    4^3 = 64 boxes whose centres are the products of the eight values below,
+    in the (lo x, lo y), (hi x, lo y), (lo x, hi y), (hi x, hi y) order of
+    :131-145 at every level, and each box away from the quadrant boundary
+    carries its own quadrant's shift (ceil'd shifts accumulated level by
+    level, :164-168, :240).  The grid limits of :256-261 take the shifts of
+    the first box at the smallest and largest centres."""
+    f, g, s0, s1 = _piecewise_pair()
+    H, W = f.shape
+    tr = []
+    tx, ty, tu, tv, phi, gx, gy = oracle.integer_search_mg(f, g, [1, H], [1, W], 32, 16, trace=tr)
+    assert len(tr) == 4 and [len(t["xf"]) for t in tr] == [1, 4, 16, 64]
+    # level 1: the whole region [25, 231]^2
+    assert tr[0]["xf"] == [[25.0, 231.0]] and tr[0]["yf"] == [[25.0, 231.0]]
+    assert 0 < tr[0]["u"][0] < 1 and -1 < tr[0]["v"][0] < 0
+    # level 2: the four quadrants of the clamped, trimmed region
+    l2 = tr[1]
+    assert [list(map(float, b)) for b in l2["xf"]] == [[25, 127], [127, 229], [25, 127], [127, 229]]
+    assert [list(map(float, b)) for b in l2["yf"]] == [[25, 128], [25, 128], [128, 231], [128, 231]]
+    assert [list(map(float, b)) for b in l2["xg"]] == [[26, 128], [128, 230], [26, 128], [128, 230]]
+    assert [list(map(float, b)) for b in l2["yg"]] == [[25, 128], [25, 128], [128, 231], [128, 231]]
+    want2 = np.array([s0, s0, s1, s0])
+    got2 = np.stack([l2["u"], l2["v"]], axis=1)
+    assert np.abs(got2 - want2).max() < 0.2
+    # last level: centres and order
+    xs = [37.5, 62.5, 88.5, 113.5, 139.5, 164.5, 190.5, 215.5]
+    ys = [37.5, 62.5, 88.5, 113.5, 140.5, 165.5, 191.5, 216.5]
+    exp_x, exp_y = [], []
+    for b2 in range(4):  # level-2 box: (lo x, lo y), (hi x, lo y), (lo x, hi y), (hi x, hi y)
+        for b3 in range(4):
+            for b4 in range(4):
+                ix = 4 * (b2 % 2) + 2 * (b3 % 2) + (b4 % 2)
+                iy = 4 * (b2 // 2) + 2 * (b3 // 2) + (b4 // 2)
+                exp_x.append(xs[ix])
+                exp_y.append(ys[iy])
+    np.testing.assert_array_equal(tx, exp_x)
+    np.testing.assert_array_equal(ty, exp_y)
+    # shifts of the boxes clear of the quadrant boundary
+    in1 = (tx < 113) & (ty > 141)
+    clear = ~np.isin(tx, [113.5, 139.5]) & ~np.isin(ty, [113.5, 140.5])
+    want = np.where(in1[:, None], s1, s0)
+    err = np.abs(np.stack([tu, tv], 1) - want)[clear]
+    assert clear.sum() == 36 and err.max() < 0.2
+    for sel, sh in ((in1 & clear, s1), (~in1 & clear, s0)):
+        assert np.abs(np.median(tu[sel]) - sh[0]) < 0.05 and np.abs(np.median(tv[sel]) - sh[1]) < 0.05
+    assert np.all(phi[clear] > 0.98)
+    # grid limits: ceil(1 + 1.25 * 32 + shift) of the first box at each extreme
+    # centre; the first box at the largest y centre lies in the s1 quadrant
+    assert gx == [41.0, 216.0] and gy == [41.0, 219.0]
+
+
 def test_gridfit_constant_and_plane():
     from staq_dic_amd import gridfit
     rng = np.random.default_rng(5)
@@ -170,3 +258,21 @@ def test_mg_search_against_oracle(ctx, oracle):
     tu_, tv_ = field(x0, y0)
     assert np.mean(np.abs(u - tu_) <= 1.0) > 0.9 and np.mean(np.abs(v - tv_) <= 1.0) > 0.9
     assert P["InitFFTSearchMethod"] == 0
+
+
+@pytest.mark.gpu
+def test_mg_scatter_on_a_piecewise_field(ctx, oracle):
+    """The device multigrid levels against the oracle's on the two-motion
+    field whose boxes and shifts the CPU test above fixes by hand."""
+    from staq_dic_amd import mgsearch
+    f, g, _, _ = _piecewise_pair()
+    H, W = f.shape
+    ctx.set_frame(f, g, None, slot=0)
+    tx, ty, tu, tv, phi, gx, gy = mgsearch.mg_scatter(ctx, 0, H, W, [1, H], [1, W], 32)
+    ox, oy, ou, ov, ophi, ogx, ogy = oracle.integer_search_mg(f, g, [1, H], [1, W], 32, 16)
+    np.testing.assert_array_equal(tx, ox)
+    np.testing.assert_array_equal(ty, oy)
+    np.testing.assert_array_equal(tu, ou)
+    np.testing.assert_array_equal(tv, ov)
+    np.testing.assert_allclose(phi, ophi, rtol=0, atol=1e-9)
+    assert list(gx) == list(ogx) and list(gy) == list(ogy)

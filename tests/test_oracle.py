@@ -232,6 +232,123 @@ def test_icgn2_limits(oracle):
     assert np.all(np.sign(U - UOld) == -np.sign(vdual))
 
 
+@pytest.mark.parametrize("mu", [1e-3, 1e-2, 3e-2])
+def test_icgn2_penalty_weight_at_the_fixed_point(oracle, mu):
+    """funICGN_Subpb1.m:111,379 stops where tempb = 2 b / bottomf^2 +
+    mu (P - UOld - vdual) vanishes.  b is taken here from the ZNSSD cost alone,
+    by central differences of orc_znssd (itself pinned by finite differences
+    above) through the first-order identity b = -(bottomf bottomg / 2) dC/dP
+    (the steepest-descent vector of funICGNQuadtree.m:362-371 against the
+    derivative of C = sum (fhat - ghat)^2), with bottomf and bottomg from
+    the window's pixels.  Dropping the 2/bottomf^2 weight (from H2 and tempb
+    alike) or the factor 2 moves the fixed point by far more than the 10 %
+    allowed; a penalty of the other sign fails the direction test."""
+    f, g, _ = _smooth_pair(P=(0.0, 0.0, 0.0, 0.0, 1.4, -0.6))
+    fr = oracle.Frame(f, g)
+    x0, y0, w = 80.0, 75.0, 40
+    coords = np.array([[x0, y0]])
+    UOld, FOld, vdual = np.array([1.0, -1.0]), np.zeros(4), np.array([0.25, 0.1])
+    target = UOld + vdual
+    U0, st0 = oracle.subpb1(fr, UOld, FOld, vdual, coords, w, w, 0.0, tol=1e-12)
+    U, st = oracle.subpb1(fr, UOld, FOld, vdual, coords, w, w, mu, tol=1e-12)
+    assert 1 <= st0[0] <= 100 and 1 <= st[0] <= 100
+    # the window's pixels: f on the 41 x 41 window, g bicubic at the warped
+    # positions (the oracle's ba_interp2 is pinned by the Keys kernel above)
+    XX, YY = np.meshgrid(np.arange(x0 - 20, x0 + 21), np.arange(y0 - 20, y0 + 21), indexing="ij")
+    fw = f[int(x0) - 21:int(x0) + 20, int(y0) - 21:int(y0) + 20]
+    gw = oracle.ba_interp2(g, (YY + U[1]).ravel(), (XX + U[0]).ravel())
+    bf = np.sqrt(np.sum((fw - fw.mean()) ** 2))
+    bg = np.sqrt(np.sum((gw - gw.mean()) ** 2))
+    h = 1e-4
+    b = np.zeros(2)
+    for k in range(2):
+        Pp = np.zeros(6)
+        Pp[4:6] = U
+        Pm = Pp.copy()
+        Pp[4 + k] += h
+        Pm[4 + k] -= h
+        cp = oracle.znssd(fr, Pp[None], coords, w, want_grad=False)[1][0]
+        cm = oracle.znssd(fr, Pm[None], coords, w, want_grad=False)[1][0]
+        b[k] = -(bf * bg / 2.0) * (cp - cm) / (2 * h)
+    pen = mu * (U - target)
+    assert np.all(np.abs(2 * b / bf ** 2 + pen) <= 0.1 * np.abs(pen))
+    # the penalty moves the solution away from UOld + vdual (the sign of :379
+    # under the inverse-compositional update)
+    assert np.all(np.sign(U - U0) == np.sign(U0 - target))
+
+
+def _two_motion_pair(transposed):
+    """A window split in two by a masked band, each side with its own integer
+    shift (g sampled exactly from the same analytic speckle pattern, so
+    bicubic samples at integer positions reproduce it bit for bit).  Window
+    (wx, wy) = (40, 20) at (80, 80), or (20, 40) with the band across y."""
+    H = W = 160
+    x0 = y0 = 80
+    rng = np.random.default_rng(7)
+    cx, cy = rng.uniform(-10, H + 10, 1000), rng.uniform(-10, W + 10, 1000)
+    s = rng.uniform(1.8, 3.0, 1000)
+
+    def pat(X, Y):
+        out = np.zeros_like(X)
+        for k in range(1000):
+            out += np.exp(-((X - cx[k]) ** 2 + (Y - cy[k]) ** 2) / (2 * s[k] ** 2))
+        return out
+
+    X, Y = np.meshgrid(np.arange(1, H + 1, dtype=float), np.arange(1, W + 1, dtype=float),
+                       indexing="ij")
+    if not transposed:
+        sA, sB, coord, lo, hi, k, wx, wy = [2.0, -1.0], [-1.0, 1.0], X, x0 - 10, x0 - 3, 0, 40, 20
+    else:
+        sA, sB, coord, lo, hi, k, wx, wy = [-1.0, 2.0], [1.0, -1.0], Y, y0 - 9, y0 - 2, 1, 20, 40
+    sA, sB = np.array(sA), np.array(sB)
+    mask = np.where((coord >= lo) & (coord <= hi), 0.0, 1.0)
+    f = pat(X, Y) * mask
+    g = np.where(coord < lo + sA[k], pat(X - sA[0], Y - sA[1]),
+                 np.where(coord > hi + sB[k], pat(X - sB[0], Y - sB[1]), 0.0))
+    return f, g, mask, sA, sB, (x0, y0, wx, wy)
+
+
+@pytest.mark.parametrize("transposed", [False, True])
+def test_icgn2_masked_region_seed(oracle, transposed):
+    """The 2-DOF masked path (funICGN_Subpb1.m:58-93,150-190): the window is
+    grown for its masked pixels (:74-88) and every iteration keeps the
+    4-connected component of nonzero g through bwselect(BW, c, r) with
+    c = floor((wx+1)/2) a COLUMN (dim-2) index and r = floor((wy+1)/2) a ROW
+    index; f, its gradient, H and the f statistics are rebuilt on that region.
+    The band splits the window into side A, holding the seed, and side B,
+    holding the centre and the swapped seed (row c, column r); the sides move
+    by sA and sB, 3.6 px apart.  Started exactly at sA, the residual on A is
+    zero, so the first norm test stops with U = sA (a region taken from B, or
+    f statistics taken over the whole window, leave a nonzero b); started
+    0.3 px away it converges next to sA (the rows next to the band hold
+    bicubic samples that straddle it, which moves the fixed point by a few
+    hundredths of a pixel), far from sB."""
+    f, g, mask, sA, sB, (x0, y0, wx, wy) = _two_motion_pair(transposed)
+    fr = oracle.Frame(f, g, mask)
+    coords = np.array([[x0, y0]], dtype=float)
+    # construction check: with the window grown as :74-79 grows it (47 x 23,
+    # or 25 x 45; 168 masked pixels), the component of nonzero g at sA through
+    # the reference's seed is side A only
+    gwx, gwy = (46, 22) if not transposed else (24, 44)
+    xa, ya = x0 - gwx // 2, y0 - gwy // 2
+    XX, YY = np.meshgrid(np.arange(xa, xa + gwx + 1, dtype=float),
+                         np.arange(ya, ya + gwy + 1, dtype=float), indexing="ij")
+    gs = oracle.ba_interp2(g, (YY + sA[1]).ravel(), (XX + sA[0]).ravel()).reshape(XX.shape)
+    lab, _ = ndimage.label(gs != 0)
+    r, c = (gwy + 1) // 2, (gwx + 1) // 2  # bwselect(BW, c, r): row r, column c (1-based)
+    region = lab == lab[r - 1, c - 1]
+    assert lab[c - 1, r - 1] != lab[r - 1, c - 1]  # the swapped seed lies on side B
+    fw = f[xa - 1:xa + gwx, ya - 1:ya + gwy]
+    assert np.array_equal(fw[region], gs[region]) and region.sum() > 250
+    # the exact start stops at the first test with U = sA
+    U, st = oracle.subpb1(fr, sA, np.zeros(4), np.zeros(2), coords, wx, wy, 0.0, tol=1e-6)
+    assert st[0] == 1 and np.array_equal(U, sA)
+    for start in (sA + [0.3, -0.2], sA + [-0.3, 0.25]):
+        U, st = oracle.subpb1(fr, start, np.zeros(4), np.zeros(2), coords, wx, wy, 0.0, tol=1e-6)
+        assert 1 <= st[0] <= 100
+        assert np.abs(U - sA).max() < 0.2 and np.hypot(*(U - sB)) > 3.0
+
+
 def test_icgn_status_codes(oracle):
     rng = np.random.default_rng(11)
     H = W = 96

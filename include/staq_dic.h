@@ -66,7 +66,7 @@ typedef struct staq_params {
 } staq_params;
 
 /* Defaults of the reference's drivers: tol 1e-2 (main_ALDIC_Quadtree_v2.m:212),
- * mu 1e-3 (:256), max_iter 100; guard 5e-3. */
+ * mu 1e-3 (:256), max_iter 100; guard 2e-3. */
 void staq_params_default(staq_params *p);
 
 /* ---- context ---------------------------------------------------------- */

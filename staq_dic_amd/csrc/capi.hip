@@ -184,7 +184,7 @@ void staq_params_default(staq_params *p)
     p->plain = 0;
     p->block_size = 0;
     p->persistent = 0;
-    p->guard = 5e-3;
+    p->guard = 2e-3;
 }
 
 int staq_ctx_create(staq_ctx **out, int device)
@@ -703,15 +703,18 @@ static int build_bins(staq_batch *b)
         // (or grows although classified clear) defers to the fallback launch
         bin.wave_npix = std::max(bin.wave_npix, bin.npix_cap);
         bin.ppt = icgn_wave_ppt(bin.wave_npix, 1);
-        // default form (measured on the hole workload): bins without masked
-        // pixels take the lean kernel, one wavefront per subset while five
-        // pixel slots hold the window, else four (two) wavefronts; bins for
-        // the general kernel take four (two, one) wavefronts; windows beyond
-        // that take the workgroup kernel
+        // default form (measured on the hole and large-deformation
+        // workloads): bins without masked pixels take the lean kernel, one
+        // wavefront per subset while five pixel slots hold the window, else
+        // two (up to 1152 pixels: the 33 x 33 windows), four or eight; bins
+        // for the general kernel take four (two, one) wavefronts; windows
+        // beyond that take the workgroup kernel
         int form = default_bs(bin.wave_npix);
         const int g = bin.masked ? -1 : 1;
         if (!bin.masked && bin.ppt && bin.ppt <= 5)
             form = 1;
+        else if (!bin.masked && icgn_lean_ppt(bin.wave_npix, 2))
+            form = 2;
         else if (icgn_wave_ppt(bin.wave_npix, 4))
             form = 4 * g;
         else if (icgn_wave_ppt(bin.wave_npix, 2))

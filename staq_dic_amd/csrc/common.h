@@ -218,8 +218,22 @@ __device__ __forceinline__ void keys_dw(T t, T w[4])
     w[3] = T(0.5) * (T(-2) * t + T(3) * t2);
 }
 
+// Twice the Keys weights, 2 w(t), as explicit FMAs (the contraction is
+// pinned, so every kernel rounds them alike).  Scaling by 2 is exact, so a
+// gather with these weights is exactly 4 times the gather with w(t).
+__device__ __forceinline__ void keys_w2(float t, float w[4])
+{
+    const float t2 = t * t, t3 = t2 * t;
+    w[0] = fmaf(2.f, t2, -t) - t3;           // -t + 2 t^2 - t^3
+    w[1] = fmaf(3.f, t3, fmaf(-5.f, t2, 2.f));  // 2 - 5 t^2 + 3 t^3
+    w[2] = fmaf(-3.f, t3, fmaf(4.f, t2, t));    // t + 4 t^2 - 3 t^3
+    w[3] = t3 - t2;                          // t^3 - t^2
+}
+
 // Keys gather of window offset (dx, dy) from a tile of row pitch tnx (the
-// taps of the 4 x 4 support, weights from the split coordinates).
+// taps of the 4 x 4 support, weights from the split coordinates).  The taps
+// are combined with 2 w in both directions and the sum scaled by 1/4, which
+// is exact: the result is the gather with the weights w themselves.
 __device__ __forceinline__ float split_gather(const SplitWarp &s, const float *tile, int tnx,
                                               float dx, float dy)
 {
@@ -227,8 +241,8 @@ __device__ __forceinline__ float split_gather(const SplitWarp &s, const float *t
     const float sv = fmaf(s.p2, dx, fmaf(s.p4, dy, s.fv));
     const float fsu = floorf(su), fsv = floorf(sv);
     float wu[4], wv[4];
-    keys_w(__builtin_amdgcn_fractf(su), wu);
-    keys_w(__builtin_amdgcn_fractf(sv), wv);
+    keys_w2(__builtin_amdgcn_fractf(su), wu);
+    keys_w2(__builtin_amdgcn_fractf(sv), wv);
     const float iuf = (dx + s.cu) + fsu, ivf = (dy + s.cv) + fsv;
     const float *b = tile + (int)fmaf(ivf, (float)tnx, iuf);
     float g = 0.f;
@@ -237,7 +251,7 @@ __device__ __forceinline__ float split_gather(const SplitWarp &s, const float *t
         const float *c4 = b + q * tnx;
         g = fmaf(wv[q], fmaf(wu[0], c4[0], fmaf(wu[1], c4[1], fmaf(wu[2], c4[2], wu[3] * c4[3]))), g);
     }
-    return g;
+    return 0.25f * g;
 }
 
 template <typename T>

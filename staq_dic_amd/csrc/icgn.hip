@@ -562,20 +562,32 @@ __global__ __launch_bounds__(BS) void icgn_kernel(IcgnArgs a)
 
             // gather + fused sums
             if constexpr (EXACT) {
-                // FP64 gather of every window pixel (the exact path below
-                // takes the statistics); counts of zeros and NaNs
+                // FP64 gather of every window pixel, fused with the sums of
+                // this iteration about c = cf: n, sum (g - c), sum (g - c)^2
+                // over the nonzero g, T_q = sum J_q (g - c) over the f side,
+                // and the counts of zeros and NaNs.  A window without a zero
+                // of g (and a 2-DOF f side that is not rebuilt) takes b from
+                // these sums in one pass; the others take the passes of the
+                // exact path below
                 const double *P = c.P;
+                const double cd = c.cf;
+                const bool whole = MODE == 6 && a.plain;
+                const bool all = c.sel0_all;
+                constexpr int K = 3 + NJ;
+                double v[K];
+#pragma unroll
+                for (int k = 0; k < K; ++k) v[k] = 0.0;
                 float acc[2] = {0.f, 0.f};
                 for (int p = tid; p < npix; p += BS) {
                     int i, j;
                     pix_ij(p, nx, rcp, i, j);
                     const double dx = i - (double)hx, dy = j - (double)hy;
                     const double u = (1 + P[0]) * dx + P[2] * dy + (c.x0 + P[4]);
-                    const double v = P[1] * dx + (1 + P[3]) * dy + (c.y0 + P[5]);
-                    const double fu = floor(u), fv = floor(v);
+                    const double vv = P[1] * dx + (1 + P[3]) * dy + (c.y0 + P[5]);
+                    const double fu = floor(u), fv = floor(vv);
                     double wu[4], wv[4];
                     keys_w(u - fu, wu);
-                    keys_w(v - fv, wv);
+                    keys_w(vv - fv, wv);
                     const int iu = (int)fu - 1, iv = (int)fv - 1;  // 1-based first taps
                     double g = 0.0;
                     if (use_tile) {
@@ -604,16 +616,57 @@ __global__ __launch_bounds__(BS) void icgn_kernel(IcgnArgs a)
                     gbuf[p] = g;
                     acc[0] += g == 0.0 ? 1.f : 0.f;
                     acc[1] += isnan(g) ? 1.f : 0.f;
+                    const bool nz = whole || fabs(g) > 1e-10;
+                    const double e = g - cd;
+                    v[0] += nz ? 1.0 : 0.0;
+                    v[1] += nz ? e : 0.0;
+                    v[2] += nz ? e * e : 0.0;
+                    const bool sf = all || sel0[p];
+                    if (sf) {
+                        const size_t gi = (size_t)((xa + i - 4) + (ya + j - 4) * Hd);
+                        const double fx = gld(fr.dfdx, gi), fy = gld(fr.dfdy, gi);
+                        if (MODE == 6) {
+                            v[3] += fx * dx * e;
+                            v[4] += fy * dx * e;
+                            v[5] += fx * dy * e;
+                            v[6] += fy * dy * e;
+                            v[7] += fx * e;
+                            v[8] += fy * e;
+                        } else {
+                            v[3] += fx * e;
+                            v[4] += fy * e;
+                        }
+                    }
                 }
                 float o[2];
+                double od[K];
                 block_sum<BS, 2>(acc, redf, o);
+                block_sum<BS, K>(v, redd, od);
                 if (tid == 0) {
                     const bool quad = !(MODE == 6 && a.plain);
                     if (quad && o[1] > 0) {
                         c.status = ST_ERROR;  // logical(NaN) throws
                         c.state = 2;
+                    } else if (quad && (c.count > 0 || o[0] > 0)) {
+                        c.path = 1;
+                    } else if (MODE == 2 && c.fsel) {
+                        c.path = 2;
                     } else {
-                        c.path = (quad && (c.count > 0 || o[0] > 0)) ? 1 : 2;
+                        // b = bottomf sum J (fhat - ghat) = FJ - (bf/bg) (T - (gbar - c) SJ)
+                        // (funICGNQuadtree.m:269-270, 362-371) in FP64
+                        const double n = od[0], S1 = od[1], S2 = od[2];
+                        const double dg = S1 / n, var = S2 - S1 * dg, bf = c.bottomf;
+                        if (!(n >= 2) || !(bf > 0) || !(var > 0) || !isfinite(bf) ||
+                            !isfinite(var) || !isfinite(dg)) {
+                            c.path = 2;  // literal per-pixel residual for degenerate statistics
+                        } else {
+                            const double bg = sqrt(var);
+                            const double ratio = bf / bg;
+                            c.kap = bg / bf;
+                            for (int k = 0; k < NJ; ++k)
+                                c.bvec[k] = c.FJ[k] - ratio * (od[3 + k] - dg * c.SJ[k]);
+                            c.path = 0;
+                        }
                     }
                 }
                 __syncthreads();
@@ -643,8 +696,8 @@ __global__ __launch_bounds__(BS) void icgn_kernel(IcgnArgs a)
                         const float sv = fmaf(sw.p2, dx, fmaf(sw.p4, dy, sw.fv));
                         const float fsu = floorf(su), fsv = floorf(sv);
                         float wu[4], wv[4];
-                        keys_w(__builtin_amdgcn_fractf(su), wu);
-                        keys_w(__builtin_amdgcn_fractf(sv), wv);
+                        keys_w2(__builtin_amdgcn_fractf(su), wu);
+                        keys_w2(__builtin_amdgcn_fractf(sv), wv);
                         const int iu = (int)((dx + sw.cu) + fsu), iv = (int)((dy + sw.cv) + fsv);
 #pragma unroll
                         for (int q = 0; q < 4; ++q) {
@@ -655,6 +708,7 @@ __global__ __launch_bounds__(BS) void icgn_kernel(IcgnArgs a)
                             for (int r = 0; r < 4; ++r) s = fmaf(wu[r], gld(col, (size_t)(min(max(tu0 + iu + r, 1), fr.H) - 1)), s);
                             g = fmaf(wv[q], s, g);
                         }
+                        g *= 0.25f;  // the weights are 2 w
                     }
                     gbuf[p] = g;
                     const bool s = all || sel0[p];

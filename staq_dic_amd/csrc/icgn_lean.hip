@@ -26,18 +26,34 @@ namespace staq {
 
 namespace {
 
+// Window offsets (dx, dy) of one pixel slot in one register: two FP16 values,
+// exact for the integers and half-integers of windows up to 2^10 pixels wide.
+__device__ __forceinline__ unsigned pack_dd(float dx, float dy)
+{
+    const unsigned lo = __builtin_bit_cast(unsigned short, (_Float16)dx);
+    const unsigned hi = __builtin_bit_cast(unsigned short, (_Float16)dy);
+    return lo | (hi << 16);
+}
+
+__device__ __forceinline__ void unpack_dd(unsigned v, float &dx, float &dy)
+{
+    dx = (float)__builtin_bit_cast(_Float16, (unsigned short)(v & 0xffffu));
+    dy = (float)__builtin_bit_cast(_Float16, (unsigned short)(v >> 16));
+}
+
 template <int NW>
 struct LeanLds {
     double Lp[21];   // packed Cholesky factor of H (column-major lower triangle)
     double Ld[6];    // 1 / diag(Lp)
     double H2i[4];   // 2-DOF: inverse of the damped 2x2 system
+    float Hs[21];    // H as summed (FP32), for its lane-parallel store
     double FJ[6], SJ[6];
     // sums about cf = (float) meanf, for the residual form of the
     // per-iteration sums: FJc = sum J (f - cf), F1c = sum (f - cf),
     // F2c = sum (f - cf)^2
     double FJc[6], F1c, F2c;
     // exchange between the wavefronts of one subset (double-buffered)
-    float xf[2][NW][24];
+    float xf[2][NW][40];
     double xd[2][NW][16];
     int xi[2][NW][8];
     // eight or more wavefronts: the Gauss-Newton step of wavefront 0
@@ -118,37 +134,12 @@ void icgn_lean_kernel(IcgnArgs a)
         const int xa = (int)x0 - wx / 2, ya = (int)y0 - wy / 2;
         const float rnx = 1.0f / (float)nx;
         const float hx = wx * 0.5f, hy = wy * 0.5f;
-        if (state == 0) {
-            // masked-pixel count (funICGNQuadtree.m:61 / funICGN.m:54)
-            int cnt = 0;
-#pragma unroll
-            for (int k = 0; k < PPT; ++k) {
-                const int p = pixel(k);
-                if (p < npix) {
-                    const int j = (int)(((float)p + 0.5f) * rnx), i = p - j * nx;
-                    const int x = xa + i, y = ya + j;
-                    if (MODE == 6 && a.plain)
-                        cnt += gld(fr.dfdx, (size_t)((x - 4) + (y - 4) * Hd)) == 0.f;
-                    else
-                        cnt += fidx(fr.f, fr.H, x, y) * fidx(fr.mask, fr.H, x, y) == 0.f;
-                }
-            }
-            int ci[1] = {(int)wave_sum_dpp((float)cnt)};
-            group_reduce<NW, 0, 0, 1>(L, par, nullptr, nullptr, ci);
-            const int count = ci[0];
-            const double n0 = (double)nx * ny;
-            const double cap = (MODE == 6 && a.plain) ? 0.5 * n0 : 0.4 * n0;
-            if (!(count < cap)) {
-                state = 2;
-                status = ST_MASKED;
-            } else if ((MODE == 6 && a.plain) ? count > 0.1 * n0 : count > 0) {
-                state = 3;  // window growth and mask regions: the general kernel
-            }
-        }
-
-        // per-slot df/dx, df/dy and window offsets; f statistics and
-        // steepest-descent sums in one pass about c0 = f at the centre
-        float fxr[PPT], fyr[PPT], dxr[PPT], dyr[PPT];
+        // per-slot df/dx, df/dy and the window offsets (dx, dy) as a pair of
+        // FP16 values (exact: integers or half-integers below 2^10); f of
+        // every slot sits in LDS.  f statistics and steepest-descent sums in
+        // one pass about c0 = f at the centre
+        float fxr[PPT], fyr[PPT];
+        unsigned ddr[PPT];
         unsigned vbits = 0u;  // bit k: slot k holds a window pixel
         double meanf = 0, bottomf = 0, sabs = 0, rel2 = 1, abs2 = 1, inv_init2 = 0;
         bool init_ok = false;
@@ -159,14 +150,17 @@ void icgn_lean_kernel(IcgnArgs a)
             const int p = pixel(k);
             const int pc = p < npix ? p : npix - 1;
             const int j = (int)(((float)pc + 0.5f) * rnx), i = pc - j * nx;
-            dxr[k] = (float)i - hx;
-            dyr[k] = (float)j - hy;
+            ddr[k] = pack_dd((float)i - hx, (float)j - hy);
             fxr[k] = fyr[k] = 0.f;
             vbits |= p < npix ? (1u << k) : 0u;
+            if (p >= npix) fwin[p] = 0.f;
         }
         if (state == 0) {
             const float c0 = fidx(fr.f, fr.H, xa + nx / 2, ya + ny / 2) *
                              (quad ? fidx(fr.mask, fr.H, xa + nx / 2, ya + ny / 2) : 1.f);
+            // the masked-pixel count (funICGNQuadtree.m:61 / funICGN.m:54) in
+            // the same pass; a window with masked pixels leaves the sums unused
+            int cnt = 0;
             float s_n = 0.f, s_1 = 0.f, s_2 = 0.f, s_abs = 0.f;
             float sj[NJ], sjf[NJ];
             constexpr int NH = MODE == 6 ? 21 : 3;
@@ -188,6 +182,7 @@ void icgn_lean_kernel(IcgnArgs a)
                     if (quad) f *= fidx(fr.mask, fr.H, x, y);
                     fwin[p] = f;
                     const float fx = fxr[k], fy = fyr[k];
+                    cnt += (MODE == 6 && a.plain) ? fx == 0.f : f == 0.f;
                     const bool nz = !quad || fabsf(f) > 1e-10f;
                     const float df = f - c0;
                     s_n += nz ? 1.f : 0.f;
@@ -216,17 +211,38 @@ void icgn_lean_kernel(IcgnArgs a)
                     }
                 }
             }
-            double sd[4 + 2 * NJ];
-            sd[0] = s_n;
-            sd[1] = s_1;
-            sd[2] = s_2;
-            sd[3] = s_abs;
+            // every sum over the window in one exchange (FP32 totals)
+            constexpr int NS = NH + 4 + 2 * NJ;
+            float sf[NS];
+#pragma unroll
+            for (int q = 0; q < NH; ++q) sf[q] = hs[q];
+            sf[NH] = s_n;
+            sf[NH + 1] = s_1;
+            sf[NH + 2] = s_2;
+            sf[NH + 3] = s_abs;
 #pragma unroll
             for (int q = 0; q < NJ; ++q) {
-                sd[4 + q] = sj[q];
-                sd[4 + NJ + q] = sjf[q];
+                sf[NH + 4 + q] = sj[q];
+                sf[NH + 4 + NJ + q] = sjf[q];
             }
-            group_reduce<NW, NH, 4 + 2 * NJ, 0>(L, par, hs, sd, nullptr);
+            int ci[1] = {(int)wave_sum_dpp((float)cnt)};
+            group_reduce<NW, NS, 0, 1>(L, par, sf, nullptr, ci);
+#pragma unroll
+            for (int q = 0; q < NH; ++q) hs[q] = sf[q];
+            double sd[4 + 2 * NJ];
+#pragma unroll
+            for (int q = 0; q < 4 + 2 * NJ; ++q) sd[q] = sf[NH + q];
+            {
+                const int count = ci[0];
+                const double n0 = (double)nx * ny;
+                const double cap = (MODE == 6 && a.plain) ? 0.5 * n0 : 0.4 * n0;
+                if (!(count < cap)) {
+                    state = 2;
+                    status = ST_MASKED;
+                } else if ((MODE == 6 && a.plain) ? count > 0.1 * n0 : count > 0) {
+                    state = 3;  // window growth and mask regions: the general kernel
+                }
+            }
             const double n = sd[0];
             const double S1 = sd[1], S2 = sd[2];
             sabs = sd[3];
@@ -253,7 +269,7 @@ void icgn_lean_kernel(IcgnArgs a)
 #pragma unroll
                 for (int q = 0; q < 21; ++q) h[q] = hs[q];
                 if (tid == 0)
-                    for (int q = 0; q < 21; ++q) a.H21[idx + (long)q * a.N] = h[q];
+                    for (int q = 0; q < 21; ++q) L.Hs[q] = hs[q];
                 if (chol6(h)) {
                     if (tid == 0) {
                         for (int q = 0; q < 21; ++q) L.Lp[q] = h[q];
@@ -274,6 +290,8 @@ void icgn_lean_kernel(IcgnArgs a)
                 set_h2(L, hs[0], hs[1], hs[2], bottomf, a.mu);
             }
             __syncthreads();
+            // H to HBM, one entry per lane (funICGNQuadtree.m:448)
+            if (MODE == 6 && tid < 21) a.H21[idx + (long)tid * a.N] = (double)L.Hs[tid];
         }
 
         // ---------------------------------------------------------- iterate
@@ -411,61 +429,60 @@ void icgn_lean_kernel(IcgnArgs a)
             float acc[2 + NJ];
 #pragma unroll
             for (int q = 0; q < 2 + NJ; ++q) acc[q] = 0.f;
-            int n_nz = 0;
-            bool any_zero = false, any_nan = false;
-            {
+            // a warped g of zero (the quadtree rules' mask regions), below
+            // 1e-10 in magnitude (outside the nonzero set the statistics
+            // take) or NaN (logical(NaN) throws): the general kernel decides
+            bool small = false;
 #pragma unroll
-                for (int k = 0; k < PPT; ++k) {
-                    const bool valid = (vbits >> k) & 1u;
-                    const float g = split_gather(sw, tile, tnx, dxr[k], dyr[k]);
-                    const float fv = fwin[pixel(k)];
-                    any_zero |= valid && g == 0.f;
-                    any_nan |= valid && isnan(g);
-                    const float ge = valid ? g : 0.f;
-                    const float r = fmaf(-kf, fv, ge) - ck;
-                    const float qv = fmaf(kf, fv, ge) - c1k;
-                    const bool nz = valid && (!quad || fabsf(ge) > 1e-10f);
-                    acc[0] += nz ? r : 0.f;
-                    acc[1] = nz ? fmaf(r, qv, acc[1]) : acc[1];
-                    const float u = valid ? fxr[k] * r : 0.f, v = valid ? fyr[k] * r : 0.f;
-                    if (MODE == 6) {
-                        acc[2] = fmaf(dxr[k], u, acc[2]);
-                        acc[3] = fmaf(dxr[k], v, acc[3]);
-                        acc[4] = fmaf(dyr[k], u, acc[4]);
-                        acc[5] = fmaf(dyr[k], v, acc[5]);
-                        acc[6] += u;
-                        acc[7] += v;
-                    } else {
-                        acc[2] += u;
-                        acc[3] += v;
-                    }
-                    n_nz += (int)__popcll(__ballot(nz));
+            for (int k = 0; k < PPT; ++k) {
+                // no lane of this wavefront holds a pixel of slot k
+                if (64 * (k * NW + wv) >= npix) continue;
+                const bool valid = (vbits >> k) & 1u;
+                float dx, dy;
+                unpack_dd(ddr[k], dx, dy);
+                const float g = split_gather(sw, tile, tnx, dx, dy);
+                const float fv = fwin[pixel(k)];
+                small |= valid && !(fabsf(g) > 1e-10f);
+                const float r = valid ? fmaf(-kf, fv, g) - ck : 0.f;
+                const float qv = fmaf(kf, fv, g) - c1k;
+                acc[0] += r;
+                acc[1] = fmaf(r, qv, acc[1]);
+                const float u = fxr[k] * r, v = fyr[k] * r;
+                if (MODE == 6) {
+                    acc[2] = fmaf(dx, u, acc[2]);
+                    acc[3] = fmaf(dx, v, acc[3]);
+                    acc[4] = fmaf(dy, u, acc[4]);
+                    acc[5] = fmaf(dy, v, acc[5]);
+                    acc[6] += u;
+                    acc[7] += v;
+                } else {
+                    acc[2] += u;
+                    acc[3] += v;
                 }
             }
-            int xv[4] = {n_nz, wave_any(any_zero) ? 1 : 0, wave_any(any_nan) ? 1 : 0, 0};
-            if constexpr (NW == 1)
-                group_reduce<NW, 0, 0, 4>(L, par, nullptr, nullptr, xv);
-            else
-                group_reduce<NW, 2 + NJ, 0, 4>(L, par, acc, nullptr, xv);
-            if (quad && xv[2] > 0) {
-                state = 2;
-                status = ST_ERROR;  // logical(NaN) throws
+            int xv[1] = {quad && wave_any(small) ? 1 : 0};
+            group_reduce<NW, 2 + NJ, 0, 1>(L, par, acc, nullptr, xv);
+            if (xv[0] > 0) {
+                state = 3;  // the general kernel (mask regions, statistics, NaN status)
                 break;
             }
-            if (quad && xv[1] > 0) {
-                state = 3;  // a zero of g: bwselect regions, the general kernel
-                break;
-            }
-            if constexpr (NW == 1) group_reduce<NW, 2 + NJ, 0, 0>(L, par, acc, nullptr, nullptr);
-            const double n = xv[0];
+            // every window pixel is in the nonzero set of g
+            const double n = npix;
             const double bf = bottomf;
-            // the residual form takes the nonzero set of g as the whole window
-            if (n != (double)npix) {
-                state = 3;
-                break;
+            // statistics of g from the residual-form sums (residual_stats in
+            // wave_common.h; T follows in the Gauss-Newton step)
+            double dg, var;
+            {
+                const double k = kf;
+                const double corr = (double)ck - (1.0 - k) * (double)cf;
+                const double corr1 = (double)c1k - (1.0 + k) * (double)cf;
+                const double F1c = L.F1c, F2c = L.F2c;
+                const double S1 = (double)acc[0] + k * F1c + n * corr;
+                const double S2 = (double)acc[1] + k * k * F2c + corr1 * (S1 - k * F1c) +
+                                  corr * (S1 + k * F1c) - n * corr * corr1;
+                dg = S1 / n;
+                var = S2 - S1 * dg;
             }
-            double T[NJ], dg, var;
-            residual_stats<NJ>(n, acc, kf, ck, c1k, cf, L.F1c, L.F2c, L.FJc, L.SJ, dg, var, T);
             if (!(n >= 2) || !(bf > 0) || !(var > 0) || !isfinite(bf) || !isfinite(var) ||
                 !isfinite(dg)) {
                 state = 3;  // degenerate statistics: the reference's FP64 residual
@@ -480,9 +497,15 @@ void icgn_lean_kernel(IcgnArgs a)
             auto gn_step = [&]() -> int {
                 double b[6] = {0, 0, 0, 0, 0, 0};
                 {
+                    // T_q = sum J_q r + kf FJc_q + corr SJ_q (residual_stats)
+                    const double k = kf;
+                    const double corr = (double)ck - (1.0 - k) * (double)cf;
                     const double ratio = bf / sv;
 #pragma unroll
-                    for (int q = 0; q < NJ; ++q) b[q] = L.FJ[q] - ratio * (T[q] - dg * L.SJ[q]);
+                    for (int q = 0; q < NJ; ++q) {
+                        const double Tq = (double)acc[2 + q] + k * L.FJc[q] + corr * L.SJ[q];
+                        b[q] = L.FJ[q] - ratio * (Tq - dg * L.SJ[q]);
+                    }
                 }
                 double nb2 = 0.0;
                 if (MODE == 6) {
@@ -588,14 +611,14 @@ void icgn_lean_kernel(IcgnArgs a)
                 }
                 a.U[2 * idx] = Pout[4];
                 a.U[2 * idx + 1] = Pout[5];
-                if (MODE == 6) {
+                if (MODE == 6)
                     for (int k = 0; k < 4; ++k) a.F[4 * idx + k] = Pout[k];
-                    if (status == ST_ERROR || status == ST_MASKED)
-                        for (int k = 0; k < 21; ++k) a.H21[idx + (long)k * a.N] = 0.0;
-                }
                 a.status[idx] = status;
             }
         }
+        // the reference's H of zeros for these codes, one entry per lane
+        if (MODE == 6 && state == 2 && (status == ST_ERROR || status == ST_MASKED) && tid < 21)
+            a.H21[idx + (long)tid * a.N] = 0.0;
         __syncthreads();
         if (!a.work_counter) break;
     }

@@ -600,8 +600,8 @@ __global__ __launch_bounds__(64 * NW) void icgn_wave_kernel(IcgnArgs a)
                     const float sv = fmaf(sw.p2, dx, fmaf(sw.p4, dy, sw.fv));
                     const float fsu = floorf(su), fsv = floorf(sv);
                     float wu[4], wv[4];
-                    keys_w(__builtin_amdgcn_fractf(su), wu);
-                    keys_w(__builtin_amdgcn_fractf(sv), wv);
+                    keys_w2(__builtin_amdgcn_fractf(su), wu);
+                    keys_w2(__builtin_amdgcn_fractf(sv), wv);
                     const int iu = (int)((dx + sw.cu) + fsu), iv = (int)((dy + sw.cv) + fsv);
                     float g = 0.f;
 #pragma unroll
@@ -613,7 +613,7 @@ __global__ __launch_bounds__(64 * NW) void icgn_wave_kernel(IcgnArgs a)
                         for (int r = 0; r < 4; ++r) s = fmaf(wu[r], gld(col, (size_t)(min(max(tu0 + iu + r, 1), fr.H) - 1)), s);
                         g = fmaf(wv[q], s, g);
                     }
-                    return g;
+                    return 0.25f * g;  // the weights are 2 w
                 }
             };
             auto gather_rt = [&](int i, int j) {

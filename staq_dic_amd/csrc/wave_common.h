@@ -95,7 +95,32 @@ __device__ __forceinline__ void group_reduce(Lds &L, int &par, float *f, double 
             for (int q = 0; q < NI; ++q) L.xi[par][w][q] = iv[q];
         }
         __syncthreads();
-        if constexpr (ND > 0 || NW > 4) {
+        if constexpr (ND == 0 && NF > 24) {
+            // many floats (the lean kernel's set-up sums): lane q < NF adds
+            // the wavefronts' partials of value q, ints in lanes 56..
+            constexpr int XW = sizeof(L.xf[0][0]) / sizeof(float);
+            static_assert(NF <= XW && NF <= 56 && NI <= 8, "exchange layout");
+            const int ln = threadIdx.x & 63;
+            float vf = 0.f;
+            int vi = 0;
+            if (ln < NF) {
+                float t = L.xf[par][0][ln];
+#pragma unroll
+                for (int ww = 1; ww < NW; ++ww) t += L.xf[par][ww][ln];
+                vf = t;
+            } else if (ln >= 56 && ln < 56 + NI) {
+                int t = L.xi[par][0][ln - 56];
+#pragma unroll
+                for (int ww = 1; ww < NW; ++ww) t += L.xi[par][ww][ln - 56];
+                vi = t;
+            }
+#pragma unroll
+            for (int q = 0; q < NF; ++q) f[q] = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(vf), q));
+#pragma unroll
+            for (int q = 0; q < NI; ++q) iv[q] = __builtin_amdgcn_readlane(vi, 56 + q);
+            par ^= 1;
+            return;
+        } else if constexpr (ND > 0 || NW > 4) {
             // many values (the set-up sums) or many wavefronts: lane q adds
             // the wavefronts' partials of value q (floats in lanes 0..,
             // doubles in lanes 32.., ints in lanes 56..), then each total is

@@ -547,10 +547,10 @@ def test_reference_level_integer_search(ctx, oracle, ref_frames):
 def test_lean_kernel_matches_general(ctx, oracle):
     """The lean kernel (windows without masked pixels, the default form of
     such bins) against the general wavefront kernel on the same bins with the
-    same number of wavefronts: the same operations in the same order, so
-    bit-identical results wherever the lean kernel kept the subset; subsets it
-    handed to the fallback launch (a zero of g under the quadtree rules)
-    differ at most by the fallback's summation order.  6-DOF and 2-DOF."""
+    same number of wavefronts, and both against the oracle: the same
+    arithmetic up to the FP32 summation order of the per-iteration sums, so
+    both hold the oracle's bar on every subset converged on both sides.
+    6-DOF and 2-DOF."""
     from staq_dic_amd import Batch, params, synth
     from staq_dic_amd._lib import IN_FOLD, IN_U0, IN_UOLD
     fr, coords, win = synth.config_hole(H=384, W=384, n_speckles=1700, max_disp=3.0, radius=60,
@@ -570,33 +570,37 @@ def test_lean_kernel_matches_general(ctx, oracle):
         (b.icgn6 if mode == 6 else b.icgn2)(p6 if mode == 6 else p2)
         return b.outputs()
 
-    def compare(out_l, out_g, with_f):
+    of = oracle.Frame(fr.f, fr.g, fr.mask)
+    Uo6, Fo6, _, so6 = oracle.local_icgn(of, U0, coords, win, tol=1e-3)
+
+    def compare(out_l, out_g, oracle_out, with_f):
         Ul, Fl, _, sl = out_l
         Ug, Fg, _, sg = out_g
-        same = (sl == sg) & (np.all(Ul.reshape(-1, 2) == Ug.reshape(-1, 2), axis=1) |
-                             np.isnan(Ul.reshape(-1, 2)).all(axis=1))
-        if with_f:
-            same &= np.all(Fl.reshape(-1, 4) == Fg.reshape(-1, 4), axis=1) | \
-                np.isnan(Fl.reshape(-1, 4)).all(axis=1)
-        assert np.mean(same) > 0.97, np.mean(same)
-        _cmp_icgn(sl, sg, Ul, Ug, Fl if with_f else None, Fg if with_f else None)
+        Uo, Fo, so = oracle_out
+        _cmp_icgn(sl, sg, Ul, Ug, Fl if with_f else None, Fg if with_f else None,
+                  name="lean vs general")
+        _cmp_icgn(sl, so, Ul, Uo, Fl if with_f else None, Fo if with_f else None,
+                  name="lean vs oracle")
+        _cmp_icgn(sg, so, Ug, Uo, Fg if with_f else None, Fo if with_f else None,
+                  name="general vs oracle")
 
     for nw in (1, 2, 4):
         general = 0 if nw == 1 else -nw
         _print("6-DOF: lean kernel with", nw, "wavefronts against the general kernel")
         out_l = run(6, [nw if c else general for c in clear])
         out_g = run(6, [general] * len(clear))
-        compare(out_l, out_g, True)
+        compare(out_l, out_g, (Uo6, Fo6, so6), True)
     UOld = np.where(np.isfinite(out_g[0]), out_g[0], U0)
     FOld = np.where(np.isfinite(out_g[1]), out_g[1], 0.0)
     b.set_input(IN_UOLD, UOld)
     b.set_input(IN_FOLD, FOld)
+    Uo2, so2 = oracle.subpb1(of, UOld, FOld, np.zeros_like(UOld), coords, win, win, 1e-3, tol=1e-3)
     for nw in (1, 2):
         general = 0 if nw == 1 else -nw
         _print("2-DOF: lean kernel with", nw, "wavefronts against the general kernel")
         out_l = run(2, [nw if c else general for c in clear])
         out_g = run(2, [general] * len(clear))
-        compare(out_l, out_g, False)
+        compare(out_l, out_g, (Uo2, None, so2), False)
     b.close()
 
 

@@ -40,17 +40,18 @@ __all__ = ["rbf_refit", "tune_beta", "ALDICQuadtreeFrame"]
 
 
 def rbf_refit(U, F, coordinatesFEM):
-    """main_ALDIC_Quadtree_inc.m:166-205: a thin-plate RBF through the nodes
-    with a finite value, evaluated at every node, for u, v and each of the
-    four F components (the centres of U and of F are chosen separately)."""
+    """main_ALDIC_Quadtree_inc.m:166-222: a thin-plate RBF through the nodes
+    whose u is finite (notnanindex, :167), evaluated at every node, for u, v
+    and each of the four F components; the F fits take the same centres
+    (:192-222), so an F value at a node whose u is NaN is replaced by the fit
+    and a NaN F at a node whose u is finite makes that fit NaN."""
     c = np.asarray(coordinatesFEM, dtype=np.float64)
     N = c.shape[0]
     U = np.asarray(U, dtype=np.float64).reshape(N, 2)
     F = np.asarray(F, dtype=np.float64).reshape(N, 4)
     ok = np.isfinite(U[:, 0])
-    okF = np.isfinite(F[:, 0])
     Uo = thinplate_eval(c[ok], thinplate_fit(c[ok], U[ok]), c)
-    Fo = thinplate_eval(c[okF], thinplate_fit(c[okF], F[okF]), c)
+    Fo = thinplate_eval(c[ok], thinplate_fit(c[ok], F[ok]), c)
     return Uo.reshape(-1), Fo.reshape(-1)
 
 

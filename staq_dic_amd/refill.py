@@ -48,7 +48,8 @@ def thinplate_fit(centers, values):
     A[n + 1:, :n] = c.T
     b = np.zeros((n + 3, values.shape[1]))
     b[:n] = values
-    return sla.solve(A, b)
+    # NaN values give NaN coefficients, as MATLAB's backslash does (no error)
+    return sla.solve(A, b, check_finite=False)
 
 
 def thinplate_eval(centers, coeff, points):
@@ -74,8 +75,10 @@ def refill_bad(U, F, coordinatesFEM, mask):
     inside = (rc[:, 0] >= 1) & (rc[:, 0] <= H) & (rc[:, 1] >= 1) & (rc[:, 1] <= W)
     comp = np.zeros(N, dtype=np.int64)
     comp[inside] = lab[rc[inside, 0] - 1, rc[inside, 1] - 1]
+    # the centres of every fit are the nodes whose u is finite (notnanindex,
+    # LocalICGNQuadtree.m:180-181), for F as for U (:228-249)
     good_u = np.isfinite(U[:, 0])
-    good_f = good_u if Fm is None else np.isfinite(Fm[:, 0])
+    good_f = good_u
 
     # stage 1: thin-plate RBF per mask component (:189-255)
     for k in range(1, ncomp + 1):

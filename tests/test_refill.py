@@ -74,3 +74,40 @@ def test_refill_leftovers_outside_every_component(ref_frames):
     if hull.any():
         np.testing.assert_allclose(Uf.reshape(N, 2)[out][hull], U.reshape(N, 2)[out][hull],
                                    atol=1e-9)
+
+
+def test_f_takes_the_centres_of_u():
+    """Every fit of the refill and of the AL loop's refit takes the nodes whose
+    u is finite as centres, F's included (LocalICGNQuadtree.m:180-249,
+    main_ALDIC_Quadtree_inc.m:166-222): an F value at a node whose u is NaN is
+    replaced by the fit through the others, and an F of NaN at a node whose u
+    is finite makes that fit NaN (MATLAB's backslash returns NaN without an
+    error), so only the scattered-interpolation stage fills F there."""
+    from staq_dic_amd.aldic import rbf_refit
+    rng = np.random.default_rng(4)
+    c = np.unique(rng.uniform(10, 190, (80, 2)).round(), axis=0)
+    N = c.shape[0]
+    lin = lambda b0, bx, by: b0 + bx * c[:, 0] + by * c[:, 1]
+    U = np.stack([lin(0.3, 0.01, -0.02), lin(-1.0, 0.005, 0.01)], 1)
+    F = np.stack([lin(0.01, 1e-4, 0), lin(0.02, 0, 1e-4), lin(-0.01, 0, 0), lin(0.0, 2e-4, 0)], 1)
+    k = 7
+    U[k] = np.nan
+    F_planted = F.copy()
+    F_planted[k] = [5.0, 5.0, 5.0, 5.0]  # finite F where u is NaN
+    Uo, Fo = rbf_refit(U.ravel(), F_planted.ravel(), c)
+    # the F fit excludes node k (u is NaN there): the linear field comes back
+    np.testing.assert_allclose(Fo.reshape(N, 4)[k], F[k], atol=1e-8)
+    np.testing.assert_allclose(Uo.reshape(N, 2)[k], [0.3 + 0.01 * c[k, 0] - 0.02 * c[k, 1],
+                                                     -1.0 + 0.005 * c[k, 0] + 0.01 * c[k, 1]],
+                               atol=1e-8)
+    # F of NaN at a node whose u is finite: the F fit is NaN everywhere
+    F_nan = F.copy()
+    F_nan[3] = np.nan
+    _, Fo2 = rbf_refit(U.ravel(), F_nan.ravel(), c)
+    assert np.all(np.isnan(Fo2))
+    # the refill (one mask component): F at the node whose u is NaN is
+    # refitted through the other nodes, like u
+    mask = np.ones((200, 200))
+    U2, F2 = refill_bad(U.ravel(), F_planted.ravel(), c, mask)
+    np.testing.assert_allclose(F2.reshape(N, 4)[k], F[k], atol=1e-8)
+    np.testing.assert_allclose(U2.reshape(N, 2)[k], Uo.reshape(N, 2)[k], atol=1e-8)

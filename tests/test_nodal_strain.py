@@ -151,8 +151,10 @@ def test_al_driver_pieces():
     F = np.tile([0.01, 0.0, -0.02, 0.005], N).astype(float)
     Un = U.copy()
     Un[2 * 100:2 * 110] = np.nan
+    # F is NaN where u is (the fits' centres are the nodes with a finite u;
+    # test_refill.py covers F of NaN at a node whose u is finite)
     Fn = F.copy()
-    Fn[4 * 50:4 * 60] = np.nan
+    Fn[4 * 100:4 * 110] = np.nan
     Ur, Fr = rbf_refit(Un, Fn, c)
     np.testing.assert_allclose(Ur, U, atol=1e-8)
     np.testing.assert_allclose(Fr, F, atol=1e-10)

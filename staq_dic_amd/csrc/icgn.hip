@@ -28,6 +28,7 @@
 #include <type_traits>
 
 #include "kernels.h"
+#include "flood.h"
 
 namespace staq {
 
@@ -62,6 +63,7 @@ struct Ctl {
     int nanP;
     int sel0_all;
     int all_sel;
+    int lit;           // the exact path's statistics are degenerate: literal per-pixel passes
     long item;
 };
 
@@ -85,6 +87,22 @@ template <int BS>
 __device__ void flood_fill(const unsigned char *bw, unsigned char *out, int nr, int nc,
                            int seed_r, int seed_c, int conn, int npix, float rcp)
 {
+    if (nr <= 64 && nc <= 64) {
+        // the first wavefront: column j of the window as one 64-bit word in
+        // lane j, the bit-parallel fill, the words back to bytes
+        if (threadIdx.x < 64) {
+            const int lane = threadIdx.x;
+            unsigned long long w = 0ull;
+            if (lane < nc)
+                for (int i = 0; i < nr; ++i)
+                    w |= (unsigned long long)(bw[i + lane * nr] != 0) << i;
+            const unsigned long long sel = wave_flood(w, nc, seed_r, seed_c, conn);
+            if (lane < nc)
+                for (int i = 0; i < nr; ++i) out[i + lane * nr] = (unsigned char)((sel >> i) & 1ull);
+        }
+        __syncthreads();
+        return;
+    }
     for (int p = threadIdx.x; p < npix; p += BS) out[p] = 0;
     __syncthreads();
     if (threadIdx.x == 0 && seed_r >= 1 && seed_r <= nr && seed_c >= 1 && seed_c <= nc) {
@@ -822,6 +840,123 @@ __global__ __launch_bounds__(BS) void icgn_kernel(IcgnArgs a)
                         if (!sel[p]) gbuf[p] = 0.f;
                     __syncthreads();
                 }
+                // one pass: every FP64 sum of the iteration about c = cf
+                // (g statistics over the nonzero g of the region, T = sum J (g - c)
+                // over the f side; the 2-DOF region's f statistics, FJ, SJ and H
+                // when f is rebuilt on it); the literal passes below take over
+                // when the statistics are degenerate (NaN and infinity as the
+                // reference propagates them)
+                {
+                    constexpr int K = 4 + NJ + 10;
+                    double v[K];
+#pragma unroll
+                    for (int k = 0; k < K; ++k) v[k] = 0.0;
+                    const bool whole = MODE == 6 && a.plain;
+                    const bool all = c.sel0_all;
+                    const double cd = c.cf;
+                    for (int p = tid; p < npix; p += BS) {
+                        int i, j;
+                        pix_ij(p, nx, rcp, i, j);
+                        const int x = xa + i, y = ya + j;
+                        const double g = gbuf[p];
+                        const bool nz = whole || fabs(g) > 1e-10;
+                        const double e = g - cd;
+                        v[0] += nz ? 1.0 : 0.0;
+                        v[1] += nz ? e : 0.0;
+                        v[2] += nz ? e * e : 0.0;
+                        v[3] += masked ? (double)sel[p] : 0.0;
+                        const bool s = fsel ? (bool)sel[p] : (all || sel0[p]);
+                        if (s) {
+                            const size_t gi = (size_t)((x - 4) + (y - 4) * Hd);
+                            const double fx = gld(fr.dfdx, gi), fy = gld(fr.dfdy, gi);
+                            if (MODE == 6) {
+                                const double dx = i - (double)hx, dy = j - (double)hy;
+                                v[4] += dx * fx * e;
+                                v[5] += dx * fy * e;
+                                v[6] += dy * fx * e;
+                                v[7] += dy * fy * e;
+                                v[8] += fx * e;
+                                v[9] += fy * e;
+                            } else {
+                                v[4] += fx * e;
+                                v[5] += fy * e;
+                                if (fsel) {
+                                    const double fv = (double)fidx(fr.f, fr.H, x, y) *
+                                                      (double)fidx(fr.mask, fr.H, x, y);
+                                    const bool fnz = fabs(fv) > 1e-10;
+                                    const double ef = fv - cd;
+                                    v[6] += fnz ? 1.0 : 0.0;
+                                    v[7] += fnz ? ef : 0.0;
+                                    v[8] += fnz ? ef * ef : 0.0;
+                                    v[9] += fabs(fv);
+                                    v[10] += fx;
+                                    v[11] += fy;
+                                    v[12] += fx * fv;
+                                    v[13] += fy * fv;
+                                    v[14] += fx * fx;
+                                    v[15] += fx * fy;
+                                    v[16] += fy * fy;
+                                }
+                            }
+                        }
+                    }
+                    double o[K];
+                    block_sum<BS, K>(v, redd, o);
+                    if (tid == 0) {
+                        const double ng = o[0], dg = o[1] / ng, varg = o[2] - o[1] * dg;
+                        bool ok = ng >= 2 && varg > 0 && isfinite(varg) && isfinite(dg);
+                        double mf = c.meanf, bf = c.bottomf;
+                        double FJ[6], SJ[6];
+                        for (int k = 0; k < NJ; ++k) {
+                            FJ[k] = c.FJ[k];
+                            SJ[k] = c.SJ[k];
+                        }
+                        if (MODE == 2 && fsel) {
+                            const double nf = o[6], df = o[7] / nf, varf = o[8] - o[7] * df;
+                            ok = ok && nf >= 2 && varf > 0 && isfinite(varf) && isfinite(df);
+                            mf = cd + df;
+                            bf = sqrt(varf);
+                            SJ[0] = o[10];
+                            SJ[1] = o[11];
+                            FJ[0] = o[12] - mf * o[10];
+                            FJ[1] = o[13] - mf * o[11];
+                        }
+                        ok = ok && bf > 0 && isfinite(bf);
+                        c.lit = !ok;
+                        if (ok) {
+                            // pass A and A2's side effects
+                            c.ng = ng;
+                            c.gmean = cd + dg;
+                            c.bottomg = sqrt(varg);
+                            if (masked) c.sum_bw2 = (int)o[3];
+                            if (masked && MODE == 2) c.fsel = 1;
+                            if (MODE == 2 && fsel) {
+                                c.nf = o[6];
+                                c.meanf = mf;
+                                c.bottomf = bf;
+                                c.sabs = o[9];
+                                c.Hm[0] = o[14];
+                                c.Hm[1] = o[15];
+                                c.Hm[2] = o[16];
+                                const double s2 = 2.0 / (bf * bf);
+                                const double H2a = o[14] * s2 + a.mu, H2b = o[15] * s2,
+                                             H2d = o[16] * s2 + a.mu;
+                                const double dl = 1e-3 * (H2a > H2d ? H2a : H2d);
+                                c.tH[0] = H2a + dl;
+                                c.tH[1] = c.tH[2] = H2b;
+                                c.tH[3] = H2d + dl;
+                            }
+                            // b = bottomf sum J (fhat - ghat) = FJ - (bf/bg) (T - (gbar - c) SJ)
+                            const double ratio = bf / c.bottomg;
+                            for (int k = 0; k < NJ; ++k)
+                                c.bvec[k] = FJ[k] - ratio * (o[4 + k] - dg * SJ[k]);
+                            const double kp = c.bottomg / bf;
+                            if (isfinite(kp) && kp > 0) c.kap = kp;
+                        }
+                    }
+                    __syncthreads();
+                }
+                if (c.lit) {
                 // pass A: counts and sums (g; 2-DOF masked also f and H)
                 {
                     double v[9];
@@ -953,6 +1088,7 @@ __global__ __launch_bounds__(BS) void icgn_kernel(IcgnArgs a)
                     }
                     __syncthreads();
                 }
+                }  // literal passes
             }
 
             // ------------------------------- norm test, GN solve, IC update

@@ -433,6 +433,76 @@ def hole_leg(ctx, tm, steps, warmup):
     return res, (fr, coords, win, truth)
 
 
+def adjoint_bytes(w):
+    """Algorithmic bytes of one forward + backward pass over a subset of
+    window w: f ((w+1)^2 FP32) and the bicubic support of g ((w+4)^2 FP32)
+    read by each pass, P in (48 B) each pass, the cost out (8 B), the
+    upstream gradient in (8 B) and dC/dP out (48 B)."""
+    return 8.0 * ((w + 1.0) ** 2 + (w + 4.0) ** 2) + 160.0
+
+
+def adjoint_leg(ctx, tm, steps):
+    import torch
+    from staq_dic_amd import synth
+    from staq_dic_amd.autograd import znssd
+    fa, ca, wa = synth.config_adjoint()
+    ta = synth.truth_at(fa.field, ca)
+    N = len(ca)
+    dev = torch.device("cuda", 0)
+    stream = torch.cuda.ExternalStream(ctx.stream)
+    ctx.set_frame(fa.f, fa.g, fa.mask, slot=0)
+    wmax = int(wa.max())
+    with torch.cuda.stream(stream):
+        P = torch.tensor(ta, dtype=torch.float64, device=dev, requires_grad=True)
+        cc_ = torch.tensor(ca, device=dev)
+        ccm = cc_.t().contiguous()
+        ww = torch.tensor(wa, dtype=torch.int32, device=dev)
+        cost = torch.empty(N, dtype=torch.float64, device=dev)
+        status = torch.empty(N, dtype=torch.int32, device=dev)
+        gc = torch.ones(N, dtype=torch.float64, device=dev)
+        gP = torch.empty((N, 6), dtype=torch.float64, device=dev)
+        L, h, sp = ctx.L, ctx.h, ctx.stream
+
+        def kernels():
+            ctx.check(L.staq_znssd_fwd_dev(h, N, wmax, P.data_ptr(), ccm.data_ptr(), ww.data_ptr(),
+                                           None, cost.data_ptr(), status.data_ptr(), sp))
+            ctx.check(L.staq_znssd_bwd_dev(h, N, wmax, P.data_ptr(), ccm.data_ptr(), ww.data_ptr(),
+                                           None, gc.data_ptr(), gP.data_ptr(), sp))
+
+        for _ in range(3):
+            kernels()
+        torch.cuda.synchronize()
+        tm.e0.record(stream)
+        for _ in range(steps):
+            kernels()
+        tm.e1.record(stream)
+        torch.cuda.synchronize()
+        ms_k = tm.e0.elapsed_time(tm.e1) / steps
+        for _ in range(2):
+            c, _ = znssd(P, cc_, ww, ctx, wmax=wmax, coords_cm=ccm)
+            c.sum().backward()
+        torch.cuda.synchronize()
+        tm.e0.record(stream)
+        for _ in range(steps):
+            P.grad = None
+            c, _ = znssd(P, cc_, ww, ctx, wmax=wmax, coords_cm=ccm)
+            c.sum().backward()
+        tm.e1.record(stream)
+        torch.cuda.synchronize()
+    ms_a = tm.e0.elapsed_time(tm.e1) / steps
+    byts = float(np.sum(adjoint_bytes(wa.astype(float))))
+    ach = byts / (ms_k / 1e3) / 1e9
+    log(f"[bench] adjoint fwd+bwd kernels {ms_k:.3f} ms, autograd {ms_a:.3f} ms for {N} subsets, "
+        f"{ach:.0f} GB/s")
+    return {"workload": "2048x2048 sinusoidal field (u = 4 sin(2 pi x/512), v = 3 cos(2 pi y/768)), "
+                        "16-px grid, window 16",
+            "subsets": N, "ms_fwd_bwd_kernels": ms_k, "ms_fwd_bwd_autograd": ms_a,
+            "subsets_per_s": N / (ms_k / 1e3),
+            "roofline": {"bound": "hbm", "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                         "frac": ach / HBM_PEAK_GBS, "algorithmic_bytes": byts,
+                         "kernels": "znssd_wave_kernel<5,false> + znssd_wave_kernel<5,true>"}}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -441,6 +511,8 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-extras", action="store_true")
+    ap.add_argument("--leg", choices=["adjoint"], default=None,
+                    help="run one leg alone and print its JSON (a profiling aid)")
     ap.add_argument("--copy-calib", action="store_true",
                     help="first run a 256 MiB elementwise copy (read 256 MiB, write 256 MiB) "
                          "three times, the known-size kernel that scales the HBM counters")
@@ -461,6 +533,13 @@ def main():
             torch.mul(src, 1.0, out=dst)
         torch.cuda.synchronize()
         del src, dst
+
+    if args.leg == "adjoint":
+        ctx = Context(0)
+        print(json.dumps(adjoint_leg(ctx, Timer(torch.cuda.ExternalStream(ctx.stream)),
+                                     args.steps)))
+        ctx.close()
+        return
 
     ctx = Context(0)
     stream = torch.cuda.ExternalStream(ctx.stream)
@@ -516,33 +595,9 @@ def main():
 
         # adjoint ZNSSD (forward + vector-Jacobian product through the bicubic
         # warp) on the 2048^2 sinusoidal-field pair with its 16-px node grid,
-        # inputs resident, through the torch.autograd.Function
-        from staq_dic_amd.autograd import znssd
-        fa, ca, wa = synth.config_adjoint()
-        ta = synth.truth_at(fa.field, ca)
-        dev = torch.device("cuda", 0)
-        stream = torch.cuda.ExternalStream(ctx.stream)
-        with torch.cuda.stream(stream):
-            P = torch.tensor(ta, dtype=torch.float64, device=dev, requires_grad=True)
-            cc_ = torch.tensor(ca, device=dev)
-            ww = torch.tensor(wa, dtype=torch.int32, device=dev)
-            ctx.set_frame(fa.f, fa.g, fa.mask, slot=0)
-            for _ in range(2):
-                cost, _ = znssd(P, cc_, ww, ctx)
-                cost.sum().backward()
-            torch.cuda.synchronize()
-            tm.e0.record(stream)
-            for _ in range(args.steps):
-                P.grad = None
-                cost, _ = znssd(P, cc_, ww, ctx)
-                cost.sum().backward()
-            tm.e1.record(stream)
-            torch.cuda.synchronize()
-        msa = tm.e0.elapsed_time(tm.e1) / args.steps
-        result["adjoint_znssd"] = {"workload": "2048x2048 sinusoidal field, 16-px grid, window 16",
-                                   "subsets": int(len(ca)), "ms_fwd_bwd": msa,
-                                   "subsets_per_s": len(ca) / (msa / 1e3)}
-        log(f"[bench] adjoint fwd+bwd {msa:.3f} ms for {len(ca)} subsets")
+        # inputs resident: the two kernels called through the C ABI back to
+        # back (device time), and the torch.autograd.Function around them
+        result["adjoint_znssd"] = adjoint_leg(ctx, tm, args.steps)
         ctx.close()
 
         # multi-instance batch, in a fresh context

@@ -48,12 +48,18 @@ class _Znssd(torch.autograd.Function):
         return grad, None, None, None, None
 
 
-def znssd(P: torch.Tensor, coords: torch.Tensor, winsize: torch.Tensor, sctx: Context):
+def znssd(P: torch.Tensor, coords: torch.Tensor, winsize: torch.Tensor, sctx: Context,
+          wmax: int | None = None, coords_cm: torch.Tensor | None = None):
     """Per-subset ZNSSD mismatch (N,) and an out-of-image status (N,).
 
     coords: (N, 2) float64 CUDA tensor of node coordinates (1-based);
-    winsize: (N,) int32 CUDA tensor."""
-    coords_cm = coords.t().contiguous()  # N x 2 column-major = (2, N) row-major
-    ws = winsize.to(torch.int32).contiguous()
-    wmax = int(ws.max().item())
-    return _Znssd.apply(P, coords_cm, ws, sctx, wmax)
+    winsize: (N,) int32 CUDA tensor.  ``wmax`` (the largest window) and
+    ``coords_cm`` (coords as a (2, N) contiguous tensor) may be given to keep
+    a repeated call free of a device-to-host read and of an allocation."""
+    if coords_cm is None:
+        coords_cm = coords.t().contiguous()  # N x 2 column-major = (2, N) row-major
+    ws = winsize if winsize.dtype == torch.int32 and winsize.is_contiguous() else \
+        winsize.to(torch.int32).contiguous()
+    if wmax is None:
+        wmax = int(ws.max().item())
+    return _Znssd.apply(P, coords_cm, ws, sctx, int(wmax))

@@ -251,6 +251,183 @@ __global__ __launch_bounds__(BS) void znssd_kernel(ZnssdArgs a)
         for (int k = 0; k < 6; ++k) a.gradP[6 * idx + k] = up * go[k];
 }
 
+// The same cost and vector-Jacobian product with one wavefront per subset
+// (windows of up to 64 * PPT pixels): the warped window's support of g is
+// staged in an LDS tile with coalesced column loads, every lane gathers PPT
+// pixels from it (g and, for the backward pass, its two derivatives from the
+// same 16 taps), the five moments are FP32 per lane about c = f at the
+// centre (small terms, so FP32 keeps them to a few ulps of the window's
+// spread) and FP64 across the wavefront, and the gradient sums follow from
+// the pixels the lanes still hold.  A subset whose support outgrows the tile
+// gathers from HBM.
+template <int PPT, bool GRAD>
+__global__ __launch_bounds__(64) void znssd_wave_kernel(ZnssdArgs a, int tile_cap)
+{
+    extern __shared__ __align__(16) unsigned char smem[];
+    float *tile = reinterpret_cast<float *>(smem);
+    const long idx = blockIdx.x;
+    const int lane = threadIdx.x;
+    const FrameDev fr = load_frame(a.frames, a.slot ? gld(a.slot, (size_t)idx) : 0);
+    const int w = gld(a.winsize, (size_t)idx), nx = w + 1, npix = nx * nx;
+    const double x0 = round(gld(a.coords, (size_t)idx)), y0 = round(gld(a.coords, (size_t)(idx + a.N)));
+    double P[6];
+#pragma unroll
+    for (int k = 0; k < 6; ++k) P[k] = gld(a.P, (size_t)(6 * idx + k));
+    const double h = w * 0.5;
+    // same window rule as the IC-GN kernels (gradient array bounds), and the
+    // warped corners inside the image test (funICGNQuadtree.m:149-157)
+    bool ok = npix <= 64 * PPT && x0 - w / 2 >= 4 && y0 - w / 2 >= 4 && x0 + w / 2 <= fr.H - 3 &&
+              y0 + w / 2 <= fr.W - 3;
+    double umin = 1e300, umax = -1e300, vmin = 1e300, vmax = -1e300;
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        const double dx = (q & 1) ? h : -h, dy = (q & 2) ? h : -h;
+        const double u = (1 + P[0]) * dx + P[2] * dy + (x0 + P[4]);
+        const double v = P[1] * dx + (1 + P[3]) * dy + (y0 + P[5]);
+        ok = ok && u >= 3 && u <= fr.H - 2 && v >= 3 && v <= fr.W - 2;
+        umin = fmin(umin, u);
+        umax = fmax(umax, u);
+        vmin = fmin(vmin, v);
+        vmax = fmax(vmax, v);
+    }
+    if (!ok) {
+        if (lane == 0) {
+            if (a.cost) a.cost[idx] = 0.0;
+            if (a.status) a.status[idx] = 1;
+        }
+        if (GRAD && a.gradP && lane < 6) a.gradP[6 * idx + lane] = 0.0;
+        return;
+    }
+    const int xa = (int)x0 - w / 2, ya = (int)y0 - w / 2;
+    // local warp relative to an integer origin near the window keeps the FP32
+    // coordinates small
+    const int ou = (int)floor(x0 + P[4]), ov = (int)floor(y0 + P[5]);
+    const float a11 = (float)(1 + P[0]), a12 = (float)P[2], a21 = (float)P[1], a22 = (float)(1 + P[3]);
+    const float cu = (float)((x0 + P[4]) - ou), cv = (float)((y0 + P[5]) - ov);
+    // the taps' support: rows floor(u) - 1 .. floor(u) + 2 (one row of slack
+    // for the FP32 local coordinates)
+    const int tu0 = (int)floor(umin) - 2, tv0 = (int)floor(vmin) - 2;
+    const int tnx = (int)floor(umax) + 4 - tu0, tny = (int)floor(vmax) + 4 - tv0;
+    const bool use_tile = tnx * tny <= tile_cap;
+    if (use_tile) {
+        for (int t = lane; t < tnx * tny; t += 64) {
+            const int sj = t / tnx, ri = t - sj * tnx;
+            const int y = min(max(tv0 + sj, 1), fr.W), x = min(max(tu0 + ri, 1), fr.H);
+            tile[t] = gld(fr.g, (size_t)((x - 1) + (y - 1) * fr.H));
+        }
+        __syncthreads();
+    }
+    const float rcp = 1.0f / nx, hf = w * 0.5f;
+    const float c0 = fidx(fr.f, fr.H, xa + nx / 2, ya + nx / 2);
+    float fv[PPT], gv[PPT], gu[PPT], gw[PPT];
+    float s[5] = {0.f, 0.f, 0.f, 0.f, 0.f};  // about c0: f, f^2, g, g^2, f g
+#pragma unroll
+    for (int k = 0; k < PPT; ++k) {
+        const int p = lane + 64 * k;
+        fv[k] = gv[k] = gu[k] = gw[k] = 0.f;
+        if (64 * k >= npix) continue;  // no lane holds a pixel of this slot
+        if (p < npix) {
+            const int j = (int)(((float)p + 0.5f) * rcp), i = p - j * nx;
+            const float dx = (float)i - hf, dy = (float)j - hf;
+            const float lu = fmaf(a11, dx, fmaf(a12, dy, cu));
+            const float lv = fmaf(a21, dx, fmaf(a22, dy, cv));
+            const float fu = floorf(lu), fvv = floorf(lv);
+            float wu[4], wv[4], du[4], dv[4];
+            keys_w(lu - fu, wu);
+            keys_w(lv - fvv, wv);
+            if (GRAD) {
+                keys_dw(lu - fu, du);
+                keys_dw(lv - fvv, dv);
+            }
+            const int r0 = ou + (int)fu - 1, cc0 = ov + (int)fvv - 1;  // 1-based first tap
+            float g = 0.f, gdu = 0.f, gdv = 0.f;
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                float t4[4];
+                if (use_tile) {
+                    const float *row = tile + (r0 - tu0) + (cc0 + q - tv0) * tnx;
+#pragma unroll
+                    for (int r = 0; r < 4; ++r) t4[r] = row[r];
+                } else {
+                    const int y = min(max(cc0 + q, 1), fr.W);
+                    const float *col = fr.g + (size_t)(y - 1) * fr.H;
+#pragma unroll
+                    for (int r = 0; r < 4; ++r) t4[r] = gld(col, (size_t)(min(max(r0 + r, 1), fr.H) - 1));
+                }
+                float sv = 0.f, sd = 0.f;
+#pragma unroll
+                for (int r = 0; r < 4; ++r) {
+                    sv = fmaf(wu[r], t4[r], sv);
+                    if (GRAD) sd = fmaf(du[r], t4[r], sd);
+                }
+                g = fmaf(wv[q], sv, g);
+                if (GRAD) {
+                    gdu = fmaf(wv[q], sd, gdu);
+                    gdv = fmaf(dv[q], sv, gdv);
+                }
+            }
+            const float f = fidx(fr.f, fr.H, xa + i, ya + j);
+            fv[k] = f;
+            gv[k] = g;
+            gu[k] = gdu;
+            gw[k] = gdv;
+            const float ef = f - c0, eg = g - c0;
+            s[0] += ef;
+            s[1] = fmaf(ef, ef, s[1]);
+            s[2] += eg;
+            s[3] = fmaf(eg, eg, s[3]);
+            s[4] = fmaf(ef, eg, s[4]);
+        }
+    }
+    double m[5];
+#pragma unroll
+    for (int q = 0; q < 5; ++q) m[q] = wave_sum_dpp_f64((double)s[q]);
+    const double n = npix;
+    const double df = m[0] / n, dg = m[2] / n;
+    const double bf = sqrt(fmax(m[1] - m[0] * df, 0.0));
+    const double bg = sqrt(fmax(m[3] - m[2] * dg, 0.0));
+    const double zncc = (m[4] - m[0] * dg) / (bf * bg);
+    if (!GRAD) {
+        if (lane == 0) {
+            if (a.cost) a.cost[idx] = 2.0 - 2.0 * zncc;
+            if (a.status) a.status[idx] = 0;
+        }
+        return;
+    }
+    // dC/dg_p = -(2/bg) (fhat_p - ZNCC ghat_p)
+    const float mf = (float)((double)c0 + df), mg = (float)((double)c0 + dg);
+    const float ibf = (float)(1.0 / bf), ibg = (float)(1.0 / bg), z = (float)zncc;
+    const float k2 = (float)(-2.0 / bg);
+    float gs[6] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+    for (int k = 0; k < PPT; ++k) {
+        const int p = lane + 64 * k;
+        if (64 * k >= npix) continue;
+        if (p < npix) {
+            const int j = (int)(((float)p + 0.5f) * rcp), i = p - j * nx;
+            const float dx = (float)i - hf, dy = (float)j - hf;
+            const float fh = (fv[k] - mf) * ibf, gh = (gv[k] - mg) * ibg;
+            const float dc = k2 * fmaf(-z, gh, fh);
+            const float au = dc * gu[k], av = dc * gw[k];
+            gs[0] = fmaf(au, dx, gs[0]);
+            gs[1] = fmaf(av, dx, gs[1]);
+            gs[2] = fmaf(au, dy, gs[2]);
+            gs[3] = fmaf(av, dy, gs[3]);
+            gs[4] += au;
+            gs[5] += av;
+        }
+    }
+    const double up = a.gcost ? gld(a.gcost, (size_t)idx) : 1.0;
+    double go[6];
+#pragma unroll
+    for (int q = 0; q < 6; ++q) go[q] = wave_sum_dpp_f64((double)gs[q]);
+    if (lane == 0) {
+        for (int q = 0; q < 6; ++q) a.gradP[6 * idx + q] = up * go[q];
+        if (a.cost) a.cost[idx] = 2.0 - 2.0 * zncc;
+        if (a.status) a.status[idx] = 0;
+    }
+}
+
 // Zeros of f * mask in each subset's window (LocalICGNQuadtree.m:71-82): a
 // window with any grows at set-up (funICGNQuadtree.m:66-70), so the batch
 // bins such subsets by their grown size, apart from the rest.  One wavefront
@@ -288,9 +465,31 @@ hipError_t mask_class_launch(const FrameDev *frames, const int *slot, const doub
     return hipGetLastError();
 }
 
+// windows of up to 32 px: one wavefront per subset; larger: one workgroup
+template <int PPT>
+static hipError_t znssd_wave(const ZnssdArgs &a, int wmax, hipStream_t s)
+{
+    // the warped support of the largest window with a 12 % stretch, plus
+    // the taps and their slack
+    const int side = (int)ceil((wmax + 1) * 1.12) + 8;
+    const int tile_cap = side * side;
+    const size_t lds = sizeof(float) * (size_t)tile_cap;
+    if (a.gradP)
+        hipLaunchKernelGGL((znssd_wave_kernel<PPT, true>), dim3((unsigned)a.N), dim3(64), lds, s,
+                           a, tile_cap);
+    else
+        hipLaunchKernelGGL((znssd_wave_kernel<PPT, false>), dim3((unsigned)a.N), dim3(64), lds, s,
+                           a, tile_cap);
+    return hipGetLastError();
+}
+
 hipError_t znssd_launch(const ZnssdArgs &a, hipStream_t s)
 {
     if (a.N <= 0) return hipSuccess;
+    const int npix = a.npix_cap;
+    if (npix <= 64 * 5) return znssd_wave<5>(a, (int)std::sqrt((double)npix) - 1, s);
+    if (npix <= 64 * 9) return znssd_wave<9>(a, (int)std::sqrt((double)npix) - 1, s);
+    if (npix <= 64 * 18) return znssd_wave<18>(a, (int)std::sqrt((double)npix) - 1, s);
     const size_t lds = 3 * sizeof(float) * (size_t)a.npix_cap;
     static bool attr = false;
     if (!attr) {

@@ -433,6 +433,54 @@ def hole_leg(ctx, tm, steps, warmup):
     return res, (fr, coords, win, truth)
 
 
+def al_loop_hole_leg(ctx, al_steps=5):
+    """One frame pair of the quadtree-with-hole workload (1024^2, Kirsch
+    field) through the incremental driver's AL loop with 5 AL steps: the
+    reference's quadtree mesh generated from the pair's mask (MeshSetUp,
+    GenerateQuadtreeMesh), the rounded analytic field as the initial guess,
+    local steps on a device-resident batch (captured graphs), thin-plate
+    refit, beta tuning, global steps with the FE nodal strain on the host;
+    seconds per kind of step and the RMS error of the result."""
+    from staq_dic_amd import synth
+    from staq_dic_amd.aldic import ALDICQuadtreeFrame, ResidentLocalSteps
+    from staq_dic_amd.mesh import GenerateQuadtreeMesh, MeshSetUp, uniform_nodes
+    fr, _, _ = synth.config_hole()
+    H, W = fr.f.shape
+    t = time.perf_counter()
+    xn, yn = uniform_nodes(H, W, 32, 16)
+    c0, e0 = MeshSetUp(xn, yn)
+    coords, elems, _, mark = GenerateQuadtreeMesh(c0, e0, fr.mask, 16, 32)
+    t_mesh = time.perf_counter() - t
+    mesh = {"coordinatesFEM": coords, "elementsFEM": elems, "markCoordHoleEdge": mark,
+            "dirichlet": mark}
+    truth = synth.truth_at(fr.field, coords)
+    U0 = np.round(truth[:, 4:6]).reshape(-1)
+    t = time.perf_counter()
+    res = ResidentLocalSteps(ctx, coords, 32, fr.f, fr.g, fr.mask)
+    t_res = time.perf_counter() - t
+    try:
+        t = time.perf_counter()
+        out = ALDICQuadtreeFrame(U0, mesh, {"ImgRefMask": fr.mask}, fr.f, fr.g,
+                                 {"winsize": 32, "winstepsize": 16}, tol=1e-2, ctx=ctx,
+                                 al_steps=al_steps, resident=res)
+        wall = time.perf_counter() - t
+    finally:
+        res.close()
+    U2 = out["USubpb2"]
+    inside = fr.mask[np.round(coords[:, 0]).astype(int) - 1,
+                     np.round(coords[:, 1]).astype(int) - 1] != 0
+    err = np.hypot(U2[0::2] - truth[:, 4], U2[1::2] - truth[:, 5])[inside]
+    r = {"workload": "hole-quadtree-1024 pair (seed 1002), reference quadtree mesh from its mask "
+                     f"({len(coords)} nodes), {al_steps} AL steps, beta tuned over 3 values, "
+                     "local steps on a resident batch (graphs), host FP64 global steps",
+         "nodes": int(len(coords)), "seconds": wall, "seconds_by_step": out["seconds"],
+         "seconds_mesh": t_mesh, "seconds_resident_setup": t_res, "beta": out["beta"],
+         "ALSolveStep": out["ALSolveStep"], "bad_subsets_per_local_step": out["bad"],
+         "rms_px_off_hole": float(np.sqrt(np.mean(err[np.isfinite(err)] ** 2)))}
+    log(f"[bench] AL loop hole 1024: {wall:.2f} s {out['seconds']}, rms {r['rms_px_off_hole']:.4f} px")
+    return r
+
+
 def adjoint_bytes(w):
     """Algorithmic bytes of one forward + backward pass over a subset of
     window w: f ((w+1)^2 FP32) and the bicubic support of g ((w+4)^2 FP32)
@@ -511,7 +559,7 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-extras", action="store_true")
-    ap.add_argument("--leg", choices=["adjoint"], default=None,
+    ap.add_argument("--leg", choices=["adjoint", "al"], default=None,
                     help="run one leg alone and print its JSON (a profiling aid)")
     ap.add_argument("--copy-calib", action="store_true",
                     help="first run a 256 MiB elementwise copy (read 256 MiB, write 256 MiB) "
@@ -534,10 +582,13 @@ def main():
         torch.cuda.synchronize()
         del src, dst
 
-    if args.leg == "adjoint":
+    if args.leg is not None:
         ctx = Context(0)
-        print(json.dumps(adjoint_leg(ctx, Timer(torch.cuda.ExternalStream(ctx.stream)),
-                                     args.steps)))
+        if args.leg == "adjoint":
+            out = adjoint_leg(ctx, Timer(torch.cuda.ExternalStream(ctx.stream)), args.steps)
+        else:
+            out = al_loop_hole_leg(ctx)
+        print(json.dumps(out))
         ctx.close()
         return
 
@@ -610,6 +661,7 @@ def main():
         # on the GPU, global step and FE nodal strain on the host
         ctx = Context(0)
         result["al_frame"] = al_frame_leg(ctx)
+        result["al_loop_hole_1024"] = al_loop_hole_leg(ctx)
         ctx.close()
 
     if not args.no_cpu:

@@ -34,7 +34,7 @@ import numpy as np
 
 from .api import LocalICGNQuadtree, Subpb1Quadtree
 from .global_step import Subpb2Quadtree, funGlobalNodalStrainQuadtree
-from .refill import thinplate_eval, thinplate_fit
+from .refill import DENSE_MAX, thinplate_eval, thinplate_fill_local, thinplate_fit
 
 __all__ = ["rbf_refit", "tune_beta", "ALDICQuadtreeFrame"]
 
@@ -44,12 +44,23 @@ def rbf_refit(U, F, coordinatesFEM):
     whose u is finite (notnanindex, :167), evaluated at every node, for u, v
     and each of the four F components; the F fits take the same centres
     (:192-222), so an F value at a node whose u is NaN is replaced by the fit
-    and a NaN F at a node whose u is finite makes that fit NaN."""
+    and a NaN F at a node whose u is finite makes that fit NaN.  Past
+    refill.DENSE_MAX centres the fit is the local-support form
+    (refill.thinplate_fill_local) at the nodes whose u is NaN."""
     c = np.asarray(coordinatesFEM, dtype=np.float64)
     N = c.shape[0]
     U = np.asarray(U, dtype=np.float64).reshape(N, 2)
     F = np.asarray(F, dtype=np.float64).reshape(N, 4)
     ok = np.isfinite(U[:, 0])
+    if ok.sum() > DENSE_MAX:
+        # local support past the reference's mesh sizes: the nodes with a
+        # finite u keep their values (the dense interpolant returns them), the
+        # others take the thin-plate RBF through their nearest such nodes
+        Uo, Fo = U.copy(), F.copy()
+        miss = np.flatnonzero(~ok)
+        Uo[miss] = thinplate_fill_local(c[ok], U[ok], c[miss])
+        Fo[miss] = thinplate_fill_local(c[ok], F[ok], c[miss])
+        return Uo.reshape(-1), Fo.reshape(-1)
     Uo = thinplate_eval(c[ok], thinplate_fit(c[ok], U[ok]), c)
     Fo = thinplate_eval(c[ok], thinplate_fit(c[ok], F[ok]), c)
     return Uo.reshape(-1), Fo.reshape(-1)
@@ -78,8 +89,51 @@ def _resets(U2, F2, U1, F1, hole_edge, hole_strain):
     return U2, F2
 
 
+class ResidentLocalSteps:
+    """The local steps of one frame pair on a device-resident batch: the pair
+    in frame slot 0 and the subset list (every node, window DICpara.winsize)
+    stay in HBM, each kind of step is captured once into a hipGraph and
+    replayed; only the step's inputs and outputs cross PCIe."""
+
+    def __init__(self, ctx, coords, winsize, f, g, mask=None):
+        from ._lib import Batch
+        self.ctx = ctx
+        ctx.set_frame(f, g, mask, slot=0)
+        N = np.asarray(coords).shape[0]
+        w = np.full(N, int(winsize), dtype=np.int64)
+        self.b = Batch(ctx, np.asarray(coords, dtype=np.float64), w)
+        self.graphs = {}
+
+    def _run(self, mode, p):
+        key = (mode, p.tol, p.mu)
+        if self.graphs.get(mode) != key:
+            (self.b.icgn6 if mode == 6 else self.b.icgn2)(p)  # a first eager step
+            self.b.reorder()
+            self.b.graph_capture(p, mode)
+            self.graphs[mode] = key
+        self.b.graph_launch()
+        return self.b.outputs()
+
+    def icgn6(self, U0, tol):
+        from ._lib import IN_U0, params
+        self.b.set_input(IN_U0, np.asarray(U0, dtype=np.float64))
+        return self._run(6, params(tol=tol))
+
+    def icgn2(self, UOld, FOld, vdual, mu, tol):
+        from ._lib import IN_FOLD, IN_UOLD, IN_VDUAL, params
+        self.b.set_input(IN_UOLD, np.asarray(UOld, dtype=np.float64))
+        self.b.set_input(IN_FOLD, np.asarray(FOld, dtype=np.float64))
+        self.b.set_input(IN_VDUAL, np.asarray(vdual, dtype=np.float64))
+        U, _, _, st = self._run(2, params(tol=tol, mu=mu))
+        return U, st
+
+    def close(self):
+        self.b.close()
+
+
 def ALDICQuadtreeFrame(U0, DICmesh, Df, fNormalized, gNormalized, DICpara, tol=1e-2,
-                       first_pair=True, measure_update=False, ctx=None):
+                       first_pair=True, measure_update=False, ctx=None, al_steps=3,
+                       resident=None):
     """One frame pair through sections 4-6 of main_ALDIC_Quadtree_inc.m.
 
     DICmesh: coordinatesFEM, elementsFEM (E x 8), markCoordHoleEdge (also the
@@ -88,7 +142,9 @@ def ALDICQuadtreeFrame(U0, DICmesh, Df, fNormalized, gNormalized, DICpara, tol=1
     between rounds for this pair (:402), which allows the early stop.
     Returns a dict with USubpb2, FSubpb2 (the pair's result), USubpb1,
     FSubpb1, beta, ALSolveStep, the local steps' status codes and bad counts,
-    and the seconds spent in each kind of step."""
+    and the seconds spent in each kind of step.  al_steps: the loop's bound
+    (ALSolveStep < al_steps, 3 in the reference's drivers, :337).  resident:
+    a ResidentLocalSteps of the pair, whose graphs then run the local steps."""
     coords = np.asarray(DICmesh["coordinatesFEM"], dtype=np.float64)
     N = coords.shape[0]
     mesh = {"coordinatesFEM": coords, "elementsFEM": DICmesh["elementsFEM"],
@@ -102,8 +158,9 @@ def ALDICQuadtreeFrame(U0, DICmesh, Df, fNormalized, gNormalized, DICpara, tol=1
 
     # section 4
     t = time.perf_counter()
+    raw = None if resident is None else resident.icgn6(U0, tol)
     U1, F1, Hp, _, conv1, nbad1, hole_strain = LocalICGNQuadtree(
-        U0, coords, Df, fNormalized, gNormalized, P, "GaussNewton", tol, ctx=ctx)
+        U0, coords, Df, fNormalized, gNormalized, P, "GaussNewton", tol, ctx=ctx, raw=raw)
     times["local"] += time.perf_counter() - t
     t = time.perf_counter()
     U1, F1 = rbf_refit(U1, F1, coords)
@@ -145,12 +202,13 @@ def ALDICQuadtreeFrame(U0, DICmesh, Df, fNormalized, gNormalized, DICpara, tol=1
     tol2 = 1e-2
     U2_prev, U1_prev = U2, U1
     P["winsize_List"] = np.full((N, 2), int(P["winsize"]), dtype=np.int64)
-    while al < 3:
+    while al < al_steps:
         al += 1
         t = time.perf_counter()
+        raw = None if resident is None else resident.icgn2(U2, F2, vdual, mu, tol)
         U1, _, _, conv, nb = Subpb1Quadtree(U2, F2, udual, vdual, coords, Df, fNormalized,
                                             gNormalized, mu, beta, None, al, P, "GaussNewton",
-                                            tol, ctx=ctx)
+                                            tol, ctx=ctx, raw=raw)
         times["local"] += time.perf_counter() - t
         F1 = F2
         status.append(conv)

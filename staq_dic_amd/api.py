@@ -268,10 +268,12 @@ def _hole_nodes(coords, winsize, f, mask):
 
 
 def LocalICGNQuadtree(U0, coordinatesFEM, Df, ImgRef, ImgDef, DICpara, ICGNmethod="GaussNewton",
-                      tol=1e-2, ctx=None, refill=True):
+                      tol=1e-2, ctx=None, refill=True, raw=None):
     """[U,F,HtempPar,LocalTime,ConvItPerEle,LocalICGNBadPtNum,markCoordHoleStrain] =
     LocalICGNQuadtree(U0,coordinatesFEM,Df,ImgRef,ImgDef,DICpara,ICGNmethod,tol).
-    Bad subsets are refilled as the reference does (refill=False leaves them NaN)."""
+    Bad subsets are refilled as the reference does (refill=False leaves them NaN).
+    raw: the GPU step's (U, F, H, status) when the caller ran it already (a
+    resident batch), else it runs here."""
     if ICGNmethod != "GaussNewton":
         raise NotImplementedError("only the Gauss-Newton scheme runs on this path")
     coords = np.asarray(coordinatesFEM, dtype=np.float64)
@@ -279,7 +281,10 @@ def LocalICGNQuadtree(U0, coordinatesFEM, Df, ImgRef, ImgDef, DICpara, ICGNmetho
     mask = Df.get("ImgRefMask") if isinstance(Df, dict) else None
     win = _winsizes(DICpara, N)
     t0 = time.perf_counter()
-    U, F, H, st = local_icgn_raw(U0, coords, win, ImgRef, ImgDef, mask, tol=tol, ctx=ctx)
+    if raw is None:
+        U, F, H, st = local_icgn_raw(U0, coords, win, ImgRef, ImgDef, mask, tol=tol, ctx=ctx)
+    else:
+        U, F, H, st = raw
     LocalTime = time.perf_counter() - t0
     bad, nbad = classify_local(st)
     U = U.copy()
@@ -321,9 +326,11 @@ def LocalICGN(U0, coordinatesFEM, Df, ImgRef, ImgDef, DICpara, ICGNmethod="Gauss
 
 
 def Subpb1Quadtree(UOld, FOld, udual, vdual, coordinatesFEM, Df, ImgRef, ImgDef, mu, beta, HPar,
-                   ALSolveStep, DICpara, ICGNmethod="GaussNewton", tol=1e-2, ctx=None, refill=True):
+                   ALSolveStep, DICpara, ICGNmethod="GaussNewton", tol=1e-2, ctx=None, refill=True,
+                   raw=None):
     """[U,HPar,ALSub1Time,ConvItPerEle,LocalICGNBadPtNum] = Subpb1Quadtree(...).
-    Window sizes come from DICpara['winsize_List'] (N x 2), else winsize."""
+    Window sizes come from DICpara['winsize_List'] (N x 2), else winsize.
+    raw: the GPU step's (U, status) when the caller ran it already."""
     if ICGNmethod != "GaussNewton":
         raise NotImplementedError("only the Gauss-Newton scheme runs on this path")
     coords = np.asarray(coordinatesFEM, dtype=np.float64)
@@ -336,8 +343,11 @@ def Subpb1Quadtree(UOld, FOld, udual, vdual, coordinatesFEM, Df, ImgRef, ImgDef,
         wx, wy = wl[:, 0].copy(), wl[:, 1].copy()
     mask = Df.get("ImgRefMask") if isinstance(Df, dict) else None
     t0 = time.perf_counter()
-    U, st = subpb1_raw(UOld, FOld, vdual, coords, wx, wy, mu, ImgRef, ImgDef, mask, tol=tol,
-                       ctx=ctx)
+    if raw is None:
+        U, st = subpb1_raw(UOld, FOld, vdual, coords, wx, wy, mu, ImgRef, ImgDef, mask, tol=tol,
+                           ctx=ctx)
+    else:
+        U, st = raw
     ALSub1Time = time.perf_counter() - t0
     # a caught error leaves ConvItPerEle at its initial 0 here (Subpb1Quadtree.m:146-149)
     st = np.where(st == -1, 0, st)

@@ -233,3 +233,28 @@ def test_mask_nan_nodes_try_catch_rule():
     np.testing.assert_array_equal(hole, expect)
     np.testing.assert_array_equal(mask_nan_nodes(X, Y, u, v, fm, None),
                                   np.array([[True, False, False], [False] * 3, [False] * 3]))
+
+
+@pytest.mark.gpu
+def test_al_frame_with_resident_local_steps(ctx, ref_frames):
+    """The same AL frame with the local steps on a device-resident batch
+    (captured graphs): the result equals the one-shot entry points' up to the
+    FP32 summation order of the kernel forms, and stays as close to the
+    stored answer."""
+    from staq_dic_amd.aldic import ALDICQuadtreeFrame, ResidentLocalSteps
+    r = ref_frames
+    args = (np.round(r["U1"]), r["mesh"], {"ImgRefMask": r["mask"]}, r["f"], r["g"],
+            {"winsize": 32, "winstepsize": 16})
+    one = ALDICQuadtreeFrame(*args, tol=1e-2, ctx=ctx)
+    res = ResidentLocalSteps(ctx, r["coords"], 32, r["f"], r["g"], r["mask"])
+    try:
+        out = ALDICQuadtreeFrame(*args, tol=1e-2, ctx=ctx, resident=res)
+    finally:
+        res.close()
+    U2, U2o = out["USubpb2"], one["USubpb2"]
+    d = np.hypot(U2[0::2] - U2o[0::2], U2[1::2] - U2o[1::2])
+    print(f"resident vs one-shot local steps: median {np.median(d):.2e} px, max {d.max():.2e} px",
+          flush=True)
+    assert np.median(d) < 1e-4 and d.max() < 1e-2
+    ds = np.hypot(U2[0::2] - r["U1"][0::2], U2[1::2] - r["U1"][1::2])
+    assert np.median(ds) < 0.05 and np.percentile(ds, 90) < 0.15

@@ -149,6 +149,19 @@ int staq_natural_interp(long n_sites, const double *sites, long n_tri, const int
                         const double *values, long n_q, const double *queries, const int *qtri,
                         double *out);
 
+/* staq_natural_weights (host only): the Sibson coordinates behind
+ * staq_natural_interp, for callers that interpolate many value sets over one
+ * set of sites and queries (the nodal strain of every global step of an AL
+ * loop over one mesh, funGlobalNodalStrainQuadtree.m:285-292).  Same inputs
+ * without the values; per query k, count[k] natural neighbours with positive
+ * weight, their site indices idx[k * cap + i] and weights w[k * cap + i]
+ * (i < min(count[k], cap)), in the order staq_natural_interp sums them.
+ * Returns STAQ_EINVAL if any count exceeds cap (the counts are still set). */
+int staq_natural_weights(long n_sites, const double *sites, long n_tri, const int *tri,
+                         const int *nbr, const long *vptr, const int *vadj, long n_q,
+                         const double *queries, const int *qtri, long cap, int *idx, double *w,
+                         int *count);
+
 /* Raw subset outputs of LocalICGNQuadtree (func_quadtree/LocalICGNQuadtree.m:63-147)
  * or, with params->plain, LocalICGN (func/LocalICGN.m:60-115), before the
  * bad-subset classification: U (2N), F (4N), Hpar (N x 21), status (N).

@@ -56,6 +56,8 @@ SIGNATURES = {
     "staq_xcorr_boxes": (C.c_int, [_vp, C.c_int, C.c_long, _lp, _lp, C.c_long, _dp, _ip]),
     "staq_natural_interp": (C.c_int, [C.c_long, _dp, C.c_long, _ip, _ip, _lp, _ip, C.c_long, _dp,
                                       C.c_long, _dp, _ip, _dp]),
+    "staq_natural_weights": (C.c_int, [C.c_long, _dp, C.c_long, _ip, _ip, _lp, _ip, C.c_long, _dp,
+                                       _ip, C.c_long, _ip, _dp, _ip]),
     "staq_local_icgn": (C.c_int, [_vp, C.POINTER(StaqParams), C.c_long, _dp, _dp, _lp, _ip, _dp,
                                   _dp, _dp, _ip]),
     "staq_subpb1": (C.c_int, [_vp, C.POINTER(StaqParams), C.c_long, _dp, _dp, _dp, _dp, _dp, _lp,

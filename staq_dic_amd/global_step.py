@@ -17,9 +17,17 @@ The restatement keeps the reference's discretisation exactly, including:
     mid-node (:273-276);
   * missing mid-nodes point at a padding node N+1 whose DOFs stay zero (:128-136).
 Element matrices are built for all elements and Gauss points at once with
-NumPy and assembled with scipy.sparse; the solve is scipy's sparse LU.
+NumPy and assembled with scipy.sparse; the solve is scipy's sparse LU.  An AL
+loop calls the global step and the nodal strain many times on one mesh, so
+the parts that depend on the mesh and the weights alone (the factorised
+matrix, the natural-neighbour map of the nodal strain) are kept in a small
+cache keyed by their inputs' bytes; every call still assembles its own
+right-hand side.
 """
 from __future__ import annotations
+
+import hashlib
+from collections import OrderedDict
 
 import numpy as np
 import scipy.sparse as sp
@@ -27,6 +35,31 @@ import scipy.sparse.linalg as spla
 
 __all__ = ["Subpb2Quadtree", "gauss_points", "assemble_subpb2", "gauss_point_strain",
            "funGlobalNodalStrainQuadtree"]
+
+_CACHE: "OrderedDict[bytes, object]" = OrderedDict()
+_CACHE_SIZE = 8
+
+
+def _digest(*parts):
+    h = hashlib.blake2b(digest_size=20)
+    for p in parts:
+        if isinstance(p, np.ndarray):
+            h.update(str((p.dtype.str, p.shape)).encode())
+            h.update(np.ascontiguousarray(p).tobytes())
+        else:
+            h.update(repr(p).encode())
+    return h.digest()
+
+
+def _cached(key, build):
+    if key in _CACHE:
+        _CACHE.move_to_end(key)
+        return _CACHE[key]
+    val = build()
+    _CACHE[key] = val
+    while len(_CACHE) > _CACHE_SIZE:
+        _CACHE.popitem(last=False)
+    return val
 
 
 def gauss_points(order):
@@ -73,9 +106,11 @@ def _shape(ksi, eta, dl):
     return N, Dk, De
 
 
-def assemble_subpb2(coordinatesFEM, elementsFEM, GaussPtOrder, beta, mu, U, F, alpha=0.0):
+def assemble_subpb2(coordinatesFEM, elementsFEM, GaussPtOrder, beta, mu, U, F, alpha=0.0,
+                    need_A=True):
     """Global matrix A and right-hand side b of Subpb2Quadtree.m:70-264 (size
-    2N+8, the last 8 entries being the padding DOFs)."""
+    2N+8, the last 8 entries being the padding DOFs); A is None when
+    need_A is false."""
     coords = np.asarray(coordinatesFEM, dtype=np.float64)
     el = np.asarray(elementsFEM, dtype=np.int64)
     N = coords.shape[0]
@@ -115,14 +150,19 @@ def assemble_subpb2(coordinatesFEM, elementsFEM, GaussPtOrder, beta, mu, U, F, a
         s = (det * wk * we)[:, None, None]
         K = Dx[:, :, None] * Dx[:, None, :] + Dy[:, :, None] * Dy[:, None, :]
         M = Nn[:, :, None] * Nn[:, None, :]
-        blk = s * ((beta + alpha) * K + mu * M)
-        Ae[..., 0] += blk
-        Ae[..., 1] += blk
+        if need_A:
+            blk = s * ((beta + alpha) * K + mu * M)
+            Ae[..., 0] += blk
+            Ae[..., 1] += blk
         for c in range(2):
             rhs = beta * (Dx * Fx[:, :, c] + Dy * Fy[:, :, c])
             rhs = rhs + mu * np.einsum("eab,eb->ea", M, Ue[:, :, c])
             rhs = rhs + alpha * np.einsum("eab,eb->ea", K, Ue[:, :, c])
             be[:, :, c] += s[:, :, 0] * rhs
+    b = np.zeros(n_dof)
+    np.add.at(b, dof.ravel(), be.reshape(E, 16).ravel())
+    if not need_A:
+        return None, b
     # interleave the two component blocks into 16 x 16 element matrices
     A16 = np.zeros((E, 8, 2, 8, 2))
     A16[:, :, 0, :, 0] = Ae[..., 0]
@@ -131,8 +171,6 @@ def assemble_subpb2(coordinatesFEM, elementsFEM, GaussPtOrder, beta, mu, U, F, a
     rows = np.repeat(dof[:, :, None], 16, axis=2).ravel()
     cols = np.repeat(dof[:, None, :], 16, axis=1).ravel()
     A = sp.coo_matrix((A16.ravel(), (rows, cols)), shape=(n_dof, n_dof)).tocsr()
-    b = np.zeros(n_dof)
-    np.add.at(b, dof.ravel(), be.reshape(E, 16).ravel())
     return A, b
 
 
@@ -148,15 +186,23 @@ def Subpb2Quadtree(DICmesh, GaussPtOrder, beta, mu, U, F, udual, vdual, alpha, w
     n_dof = 2 * N + 8
     U = np.asarray(U, dtype=np.float64).ravel()
     F = np.asarray(F, dtype=np.float64).ravel()
-    A, b = assemble_subpb2(coords, el, GaussPtOrder, beta, mu, U, F, alpha)
-
-    # DOFs of the nodes the elements use, without the smallest index (:273-276)
-    involved = np.unique(el[:, :8])[1:]
-    uidx = np.stack([2 * involved - 2, 2 * involved - 1], axis=1).ravel()
     dirichlet = np.unique(np.asarray(DICmesh.get("dirichlet", []), dtype=np.int64).ravel())
-    dir_dof = np.concatenate([2 * dirichlet - 1, 2 * dirichlet - 2]) if dirichlet.size else \
-        np.zeros(0, dtype=np.int64)
-    free = np.setdiff1d(uidx, dir_dof)
+
+    def build():
+        A, _ = assemble_subpb2(coords, el, GaussPtOrder, beta, mu, np.zeros(2 * N),
+                               np.zeros(4 * N), alpha)
+        # DOFs of the nodes the elements use, without the smallest index (:273-276)
+        involved = np.unique(el[:, :8])[1:]
+        uidx = np.stack([2 * involved - 2, 2 * involved - 1], axis=1).ravel()
+        dir_dof = np.concatenate([2 * dirichlet - 1, 2 * dirichlet - 2]) if dirichlet.size else \
+            np.zeros(0, dtype=np.int64)
+        free = np.setdiff1d(uidx, dir_dof)
+        lu = spla.splu(A[free][:, free].tocsc()) if free.size else None
+        return A, free, lu
+
+    A, free, lu = _cached(_digest("subpb2", coords, el, dirichlet, GaussPtOrder, float(beta),
+                                  float(mu), float(alpha)), build)
+    _, b = assemble_subpb2(coords, el, GaussPtOrder, beta, mu, U, F, alpha, need_A=False)
 
     # Neumann edges with traction -F / winstepsize (:279-287)
     neumann = np.asarray(DICmesh.get("neumann", np.zeros((0, 4))), dtype=np.float64)
@@ -177,8 +223,7 @@ def Subpb2Quadtree(DICmesh, GaussPtOrder, beta, mu, U, F, udual, vdual, alpha, w
         Uhat[2 * dirichlet - 2] = U[2 * dirichlet - 2]
     b = b - A @ Uhat
     if free.size:
-        Aff = A[free][:, free].tocsc()
-        Uhat[free] = spla.spsolve(Aff, b[free])
+        Uhat[free] = lu.solve(b[free])
     return Uhat[:2 * N]
 
 
@@ -247,8 +292,9 @@ def funGlobalNodalStrainQuadtree(DICmesh, U, GaussPtOrder, waitBarDisplayOrNot=1
     opposite order to the F of the local steps ([u_x, v_x, u_y, v_y]); kept
     as the reference returns it.  The smoothing branch (:296-330) is not
     reached: the reference fixes smoothness = 0 (:40)."""
-    from .natural import natural_nearest
+    from .natural import natural_plan
     coords = np.asarray(DICmesh["coordinatesFEM"], dtype=np.float64)
     sg, cg = gauss_point_strain(coords, DICmesh["elementsFEM"], U, GaussPtOrder)
-    nodal = natural_nearest(cg, sg, coords)
+    W = _cached(_digest("natural", cg, coords), lambda: natural_plan(cg, coords))
+    nodal = W @ sg
     return nodal.reshape(-1), sg, cg

@@ -31,9 +31,10 @@ several are equally near (MATLAB documents no tie rule).
 from __future__ import annotations
 
 import numpy as np
+import scipy.sparse as sp
 from scipy.spatial import Delaunay, cKDTree
 
-__all__ = ["natural_nearest", "sibson_weights"]
+__all__ = ["natural_nearest", "natural_plan", "sibson_weights"]
 
 # ``_sibson`` below is the restatement in Python; the same steps run natively
 # (staq_natural_interp) for the interior queries of ``natural_nearest``, and
@@ -208,6 +209,90 @@ def _native_interior(T, V, Q, simp):
     return out
 
 
+def _classify(T, Q):
+    """Each query's class: ("site", i), ("out", i), ("edge", i, j, t) or
+    ("in", triangle), by the rules of ``natural_nearest``."""
+    simp = T.tri.find_simplex(Q)
+    dist, near = T.kd.query(Q, k=1)
+    out = []
+    for k in range(Q.shape[0]):
+        q = Q[k]
+        if dist[k] <= 1e-12 * T.span:
+            out.append(("site", int(near[k])))
+            continue
+        if simp[k] < 0:
+            # outside the hull: the nearest site, the lowest index on a tie
+            cand = T.kd.query_ball_point(q, dist[k] * (1 + 1e-12) + 1e-300)
+            out.append(("out", int(min(cand)) if cand else int(near[k])))
+            continue
+        edge = T.on_hull(q, int(simp[k]))
+        if edge is not None:
+            out.append(("edge",) + tuple(edge))
+            continue
+        out.append(("in", int(simp[k])))
+    return out
+
+
+def natural_plan(sites, queries, cap=64):
+    """The linear map of ``natural_nearest`` from site values to query values
+    as a CSR matrix (queries x sites): one set of sites and queries, any
+    number of value sets.  Each row holds its terms in the order
+    ``natural_nearest`` adds them, so ``natural_plan(S, Q) @ V`` equals
+    ``natural_nearest(S, V, Q)`` bit for bit."""
+    import ctypes as C
+
+    from . import _lib
+    sites = np.asarray(sites, dtype=np.float64)
+    Q = np.asarray(queries, dtype=np.float64)
+    if sites.shape[0] < 3:
+        raise ValueError("natural-neighbour interpolation needs at least three sites")
+    T = _Triangulation(sites)
+    cls = _classify(T, Q)
+    nq = Q.shape[0]
+    rows = [None] * nq
+    interior = [k for k, c in enumerate(cls) if c[0] == "in"]
+    if interior:
+        L = _lib.load()
+        ki = np.asarray(interior)
+        tri = np.ascontiguousarray(T.tri.simplices, dtype=np.int32)
+        nbr = np.ascontiguousarray(T.tri.neighbors, dtype=np.int32)
+        ptr, adj = T.tri.vertex_neighbor_vertices
+        ptr = np.ascontiguousarray(ptr, dtype=np.int64)
+        adj = np.ascontiguousarray(adj, dtype=np.int32)
+        P = np.ascontiguousarray(T.p)
+        Qi = np.ascontiguousarray(Q[ki])
+        st = np.ascontiguousarray([cls[k][1] for k in interior], dtype=np.int32)
+        dp = lambda a: a.ctypes.data_as(C.POINTER(C.c_double))
+        ip = lambda a: a.ctypes.data_as(C.POINTER(C.c_int))
+        cnt = np.zeros(ki.size, dtype=np.int32)
+        for _ in range(2):
+            # a query inside a hole of the sites has the whole rim as
+            # neighbours: a second call with room for the most of them
+            idx = np.zeros((ki.size, cap), dtype=np.int32)
+            w = np.zeros((ki.size, cap))
+            rc = L.staq_natural_weights(P.shape[0], dp(P), tri.shape[0], ip(tri), ip(nbr),
+                                        ptr.ctypes.data_as(C.POINTER(C.c_long)), ip(adj),
+                                        ki.size, dp(Qi), ip(st), cap, ip(idx), dp(w), ip(cnt))
+            if rc == 0 or int(cnt.max()) <= cap:
+                break
+            cap = int(cnt.max())
+        if rc != 0:
+            raise _lib.StaqError(f"staq_natural_weights failed ({rc})")
+        for r, k in enumerate(interior):
+            rows[k] = (idx[r, :cnt[r]], w[r, :cnt[r]])
+    for k, c in enumerate(cls):
+        if c[0] in ("site", "out"):
+            rows[k] = (np.array([c[1]]), np.array([1.0]))
+        elif c[0] == "edge":
+            _, i, j, t = c
+            rows[k] = (np.array([i, j]), np.array([1.0 - t, t]))
+    indptr = np.zeros(nq + 1, dtype=np.int64)
+    indptr[1:] = np.cumsum([r[0].size for r in rows])
+    ind = np.concatenate([r[0] for r in rows]).astype(np.int64) if nq else np.zeros(0, np.int64)
+    dat = np.concatenate([r[1] for r in rows]) if nq else np.zeros(0)
+    return sp.csr_matrix((dat, ind, indptr), shape=(nq, sites.shape[0]))
+
+
 def natural_nearest(sites, values, queries):
     """scatteredInterpolant(sites, values, 'natural', 'nearest') evaluated at
     queries: sites (n, 2), values (n,) or (n, m), queries (k, 2).  Interior
@@ -222,27 +307,18 @@ def natural_nearest(sites, values, queries):
     if sites.shape[0] < 3:
         raise ValueError("natural-neighbour interpolation needs at least three sites")
     T = _Triangulation(sites)
-    simp = T.tri.find_simplex(Q)
-    dist, near = T.kd.query(Q, k=1)
-    interior = []
-    for k in range(Q.shape[0]):
-        q = Q[k]
-        if dist[k] <= 1e-12 * T.span:
-            out[k] = V[near[k]]
-            continue
-        if simp[k] < 0:
-            # outside the hull: the nearest site, the lowest index on a tie
-            cand = T.kd.query_ball_point(q, dist[k] * (1 + 1e-12) + 1e-300)
-            out[k] = V[min(cand)] if cand else V[near[k]]
-            continue
-        edge = T.on_hull(q, int(simp[k]))
-        if edge is not None:
-            i, j, t = edge
+    interior, simp = [], []
+    for k, c in enumerate(_classify(T, Q)):
+        if c[0] in ("site", "out"):
+            out[k] = V[c[1]]
+        elif c[0] == "edge":
+            _, i, j, t = c
             out[k] = (1.0 - t) * V[i] + t * V[j]
-            continue
-        interior.append(k)
+        else:
+            interior.append(k)
+            simp.append(c[1])
     if interior:
         ki = np.asarray(interior)
-        out[ki] = _native_interior(T, V, Q[ki], simp[ki])
+        out[ki] = _native_interior(T, V, Q[ki], np.asarray(simp))
     return out[:, 0] if squeeze else out
 

@@ -158,3 +158,21 @@ def test_bad_arguments_without_a_device(lib):
     assert L.staq_natural_interp(3, dp(sites), 1, ip(tri), ip(nbr), lp, ip(adj), 1, dp(vals), 1,
                                  dp(q), ip(np.array([0], dtype=np.int32)), dp(out)) == 0
     assert abs(out[0] - (1.0 + 0.25 + 2 * 0.25)) < 1e-9
+    # the weights form: a bad triangle index and too small a cap are
+    # rejected (the count is still reported); with room, the weights are the
+    # barycentric ones and reproduce the value above
+    idx = np.zeros(3, dtype=np.int32)
+    w = np.zeros(3)
+    cnt = np.zeros(1, dtype=np.int32)
+    assert L.staq_natural_weights(3, dp(sites), 1, ip(tri), ip(nbr), lp, ip(adj), 1, dp(q),
+                                  ip(np.array([5], dtype=np.int32)), 3, ip(idx), dp(w),
+                                  ip(cnt)) == -1
+    assert L.staq_natural_weights(3, dp(sites), 1, ip(tri), ip(nbr), lp, ip(adj), 1, dp(q),
+                                  ip(np.array([0], dtype=np.int32)), 2, ip(idx), dp(w),
+                                  ip(cnt)) == -1
+    assert cnt[0] == 3
+    assert L.staq_natural_weights(3, dp(sites), 1, ip(tri), ip(nbr), lp, ip(adj), 1, dp(q),
+                                  ip(np.array([0], dtype=np.int32)), 3, ip(idx), dp(w),
+                                  ip(cnt)) == 0
+    assert cnt[0] == 3 and abs(w.sum() - 1.0) < 1e-12
+    assert abs(w @ vals[idx] - out[0]) < 1e-12

@@ -192,3 +192,58 @@ def test_native_sibson_matches_restatement():
     q = np.array([[2.5, 3.5], [4.25, 1.75]])
     out = natural._native_interior(TL, (L[:, :1] * 2 - L[:, 1:]), q, TL.tri.find_simplex(q))
     np.testing.assert_allclose(out[:, 0], 2 * q[:, 0] - q[:, 1], atol=1e-9)
+
+
+def test_natural_plan_equals_natural_nearest():
+    """The cached linear map (natural_plan) against natural_nearest over the
+    four kinds of query (on a site, outside the hull, on a hull edge,
+    interior, on a lattice with co-circular ties too): equal bit for bit,
+    since each row adds its terms in natural_nearest's order."""
+    from staq_dic_amd.natural import natural_plan
+    rng = np.random.default_rng(11)
+    S = rng.uniform(0, 10, (400, 2))
+    V = rng.normal(size=(400, 4))
+    Q = np.concatenate([S[:7], rng.uniform(-2, 12, (200, 2)), rng.uniform(1, 9, (200, 2))])
+    W = natural_plan(S, Q)
+    np.testing.assert_array_equal(W @ V, natural_nearest(S, V, Q))
+    g = np.arange(6.0)
+    X, Y = np.meshgrid(g, g, indexing="ij")
+    L = np.stack([X.ravel(), Y.ravel()], axis=1)
+    q = np.array([[2.5, 3.5], [4.25, 1.75], [0.0, 2.5], [5.0, 5.0], [7.0, -1.0]])
+    VL = rng.normal(size=(L.shape[0], 2))
+    np.testing.assert_array_equal(natural_plan(L, q) @ VL, natural_nearest(L, VL, q))
+
+
+def test_cached_global_step_and_strain_on_reference_mesh(ref_frames):
+    """Subpb2Quadtree's cached factorisation against a direct sparse solve of
+    the same system, and a second call of each (cache hit) against the
+    first."""
+    import scipy.sparse.linalg as spla
+    from staq_dic_amd.global_step import Subpb2Quadtree, assemble_subpb2
+    r = ref_frames
+    mesh = dict(r["mesh"])
+    c = mesh["coordinatesFEM"]
+    N = c.shape[0]
+    rng = np.random.default_rng(2)
+    U = r["U1"] + rng.normal(0, 0.05, 2 * N)
+    F = rng.normal(0, 0.01, 4 * N)
+    Uh = Subpb2Quadtree(mesh, 2, 0.3, 1e-3, U, F, 0 * F, 0 * U, 0.0, 16.0, 0)
+    A, b = assemble_subpb2(c, mesh["elementsFEM"], 2, 0.3, 1e-3, U, F)
+    el = np.asarray(mesh["elementsFEM"])
+    inv = np.unique(el[:, :8])[1:]
+    dirichlet = np.unique(np.asarray(mesh["dirichlet"]).ravel())
+    free = np.setdiff1d(np.stack([2 * inv - 2, 2 * inv - 1], 1).ravel(),
+                        np.concatenate([2 * dirichlet - 1, 2 * dirichlet - 2]))
+    want = np.zeros(2 * N + 8)
+    want[2 * dirichlet - 1] = U[2 * dirichlet - 1]
+    want[2 * dirichlet - 2] = U[2 * dirichlet - 2]
+    rhs = b - A @ want
+    want[free] = spla.spsolve(A[free][:, free].tocsc(), rhs[free])
+    np.testing.assert_allclose(Uh, want[:2 * N], rtol=0, atol=1e-10)
+    np.testing.assert_array_equal(Subpb2Quadtree(mesh, 2, 0.3, 1e-3, U, F, 0 * F, 0 * U, 0.0,
+                                                 16.0, 0), Uh)
+    F1 = funGlobalNodalStrainQuadtree(mesh, Uh, 2, 0)[0]
+    np.testing.assert_array_equal(funGlobalNodalStrainQuadtree(mesh, Uh, 2, 0)[0], F1)
+    from staq_dic_amd.global_step import gauss_point_strain
+    sg, cg = gauss_point_strain(c, mesh["elementsFEM"], Uh, 2)
+    np.testing.assert_array_equal(F1, natural_nearest(cg, sg, c).reshape(-1))

@@ -10,8 +10,8 @@ What stays on the host is the reference's bookkeeping around the per-subset
 solver: bad-subset classification (LocalICGNQuadtree.m:151-177,
 Subpb1Quadtree.m:168-187) and the refill of bad subsets (refill.py), the hole
 mask of the integer search (IntegerSearch.m:69-77) and timing.  The plain
-twins' refills (plane fit and rmoutliers, gridfit) are not restated: their bad
-subsets come back as NaN.
+twins' refills (LocalICGN's plane fit, rmoutliers and nearest fill, Subpb1's
+gridfit) are in refill_plain.py.
 """
 from __future__ import annotations
 
@@ -22,6 +22,7 @@ import numpy as np
 from . import _lib
 from .outliers import funRemoveOutliers
 from .refill import refill_bad
+from .refill_plain import local_icgn_refill, subpb1_gridfit
 from ._lib import Batch, Context, IN_FOLD, IN_U0, IN_UOLD, IN_VDUAL, d, ip, lp
 
 __all__ = [
@@ -301,9 +302,11 @@ def LocalICGNQuadtree(U0, coordinatesFEM, Df, ImgRef, ImgDef, DICpara, ICGNmetho
 
 
 def LocalICGN(U0, coordinatesFEM, Df, ImgRef, ImgDef, DICpara, ICGNmethod="GaussNewton", tol=1e-2,
-              ctx=None):
+              ctx=None, refill=True):
     """[U,F,HtempPar,LocalTime,ConvItPerEle,LocalICGNBadPtNum] = LocalICGN(...) —
-    the uniform-grid twin (func/LocalICGN.m)."""
+    the uniform-grid twin (func/LocalICGN.m): bad subsets NaN, then the
+    plane fit per mask component, the F outlier removal and the nearest fill
+    of LocalICGN.m:143-287 (refill_plain.local_icgn_refill)."""
     if ICGNmethod != "GaussNewton":
         raise NotImplementedError("only the Gauss-Newton scheme runs on this path")
     coords = np.asarray(coordinatesFEM, dtype=np.float64)
@@ -322,6 +325,9 @@ def LocalICGN(U0, coordinatesFEM, Df, ImgRef, ImgDef, DICpara, ICGNmethod="Gauss
     U[2 * idx + 1] = np.nan
     for k in range(4):
         F[4 * idx + k] = np.nan
+    if refill:
+        m = np.ones(np.shape(ImgRef)) if mask is None else np.asarray(mask)
+        U, F = local_icgn_refill(U, F, coords, m, DICpara["winstepsize"])
     return U, F, H, LocalTime, st.astype(np.float64), nbad
 
 
@@ -363,9 +369,10 @@ def Subpb1Quadtree(UOld, FOld, udual, vdual, coordinatesFEM, Df, ImgRef, ImgDef,
 
 
 def Subpb1(UOld, FOld, udual, vdual, coordinatesFEM, Df, ImgRef, ImgDef, mu, beta, HPar,
-           ALSolveStep, DICpara, ICGNmethod="GaussNewton", tol=1e-2, ctx=None):
+           ALSolveStep, DICpara, ICGNmethod="GaussNewton", tol=1e-2, ctx=None, refill=True):
     """[U,HPar,ALSub1Time,ConvItPerEle,LocalICGNBadPtNum] = Subpb1(...) — the
-    uniform-grid twin (func/Subpb1.m, errors are -1)."""
+    uniform-grid twin (func/Subpb1.m, errors are -1): bad subsets NaN, then
+    the gridfit of Subpb1.m:185-199 (refill_plain.subpb1_gridfit)."""
     coords = np.asarray(coordinatesFEM, dtype=np.float64)
     N = coords.shape[0]
     w = _winsizes(DICpara, N)
@@ -379,4 +386,6 @@ def Subpb1(UOld, FOld, udual, vdual, coordinatesFEM, Df, ImgRef, ImgDef, mu, bet
     idx = np.flatnonzero(bad)
     U[2 * idx] = np.nan
     U[2 * idx + 1] = np.nan
+    if refill:
+        U = subpb1_gridfit(U, coords)
     return U, HPar, ALSub1Time, st.astype(np.float64), nbad

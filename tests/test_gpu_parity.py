@@ -161,6 +161,44 @@ def test_icgn2_al(ctx, oracle, plain_cfg):
         _cmp_icgn(sg, so, Ug, Uo)
 
 
+def test_icgn2_plain_and_plain_refills(ctx, oracle, plain_cfg):
+    """The uniform-grid 2-DOF step (func/Subpb1.m calling funICGN_Subpb1.m with
+    the node coordinates as given) against the oracle's plain mode; then the
+    plain twins end to end: Subpb1 with its gridfit and LocalICGN with its
+    plane fit, rmoutliers and nearest fill, against the analytic field."""
+    from staq_dic_amd import synth
+    from staq_dic_amd.api import LocalICGN, Subpb1, subpb1_raw
+    fr, coords, win = plain_cfg
+    N = coords.shape[0]
+    truth = synth.truth_at(fr.field, coords)
+    rng = np.random.default_rng(6)
+    UOld = (truth[:, 4:6] + rng.uniform(-0.3, 0.3, (N, 2))).reshape(-1)
+    FOld = truth[:, :4].reshape(-1)
+    vdual = rng.uniform(-0.05, 0.05, 2 * N)
+    of = oracle.Frame(fr.f, fr.g, fr.mask)
+    for mu in (1e-3, 0.0):
+        _print("icgn2 plain mu", mu)
+        Ug, sg = subpb1_raw(UOld, FOld, vdual, coords, win, win, mu, fr.f, fr.g, fr.mask,
+                            tol=1e-3, plain=True, ctx=ctx)
+        Uo, so = oracle.subpb1(of, UOld, FOld, vdual, coords, win, win, mu, tol=1e-3, plain=True)
+        _cmp_icgn(sg, so, Ug, Uo, name="icgn2 plain")
+    para = {"winsize": int(win[0]), "winstepsize": 16}
+    Df = {"ImgRefMask": fr.mask}
+    U1, _, _, st, nb = Subpb1(UOld, FOld, 0 * FOld, vdual, coords, Df, fr.f, fr.g, 1e-3, 0.0,
+                              None, 2, para, tol=1e-3, ctx=ctx)
+    e = np.hypot(U1[0::2] - truth[:, 4], U1[1::2] - truth[:, 5])
+    _print(f"Subpb1 + gridfit: {nb} bad, median {np.median(e):.4f} px, max {e.max():.4f} px")
+    assert np.isfinite(U1).all() and np.median(e) < 0.05
+    U0 = np.round(truth[:, 4:6]).reshape(-1)
+    U6, F6, _, _, st6, nb6 = LocalICGN(U0, coords, Df, fr.f, fr.g, para, tol=1e-3, ctx=ctx)
+    e6 = np.hypot(U6[0::2] - truth[:, 4], U6[1::2] - truth[:, 5])
+    eF = np.abs(F6.reshape(-1, 4) - truth[:, :4]).max(axis=1)
+    _print(f"LocalICGN + plane fit: {nb6} bad, median {np.median(e6):.4f} px, "
+           f"median |dF| {np.median(eF):.2e}")
+    assert np.isfinite(U6).all() and np.isfinite(F6).all()
+    assert np.median(e6) < 0.02 and np.median(eF) < 2e-3
+
+
 def test_hole_masked_path(ctx, oracle):
     """Quadtree mesh around a hole: masked windows, window growth, bwselect
     regions, the -1 size-mismatch quirk and 102 all go through the device."""

@@ -136,6 +136,19 @@ int staq_xcorr_full_peak(staq_ctx *ctx, int slot, const long box_f[4], const lon
 int staq_xcorr_boxes(staq_ctx *ctx, int slot, long nbox, const long *box_f, const long *box_g,
                      long pad, double *out, int *ok);
 
+/* staq_xcorr_qfactors: the correlation quality factors of compute_qFactor
+ * (funIntegerSearch.m:211-264) for nbox boxes (4 longs each, x0 x1 y0 y1,
+ * 1-based, inclusive): C = f(box_f), D = g(box_g), the whole normxcorr2(C, D)
+ * map ((n1 + d1 - 1) x (n2 + d2 - 1)), cc_min = map - min(map); q = [pce ppe]
+ * per box: pce = max(cc_min)^2 / mean(cc_min.^2), ppe = 1 / the entropy of
+ * cc_min's histogram in 30 equal bins over [0, max(cc_min)].  ok = 1, or
+ * ok = 0 and NaN where the reference's try fails (a box outside the image, a
+ * g box smaller than the template, a constant template).  The callers are the
+ * whole-field search (:123-125, boxes ii:ii+w by ii-r:ii+w+r) and the last
+ * level of the multigrid search (funIntegerSearchMg.m:164-173, g box +- 5). */
+int staq_xcorr_qfactors(staq_ctx *ctx, int slot, long nbox, const long *box_f, const long *box_g,
+                        double *q, int *ok);
+
 /* staq_natural_interp (host only, no device work): Sibson natural-neighbour
  * interpolation at interior query points, the 'natural' method of MATLAB's
  * scatteredInterpolant as funGlobalNodalStrainQuadtree.m:285-292 and

@@ -144,6 +144,57 @@ def normxcorr2_full(Cm, D):
     return out
 
 
+QF_BINS = 30
+
+
+def qfactor_stats(cc):
+    """compute_qFactor (func/funIntegerSearch.m:211-264) of one normxcorr2 map:
+    cc_min = cc - min(cc) (cc_max_find, :241-250), pce = max(cc_min)^2 /
+    mean(cc_min.^2) (:222), ppe = 1 / the entropy (natural log) of cc_min's
+    30-bin histogram (q_entropy, :252-264).  The bins are 30 equal ones over
+    [0, max(cc_min)], bin k holding k w <= x < (k+1) w and the last one closed;
+    MATLAB's histcounts(x, 30) places its edges at rounded positions by a rule
+    of its own that the reference's files do not fix, so ppe's binning is
+    parity unpinned against MATLAB (pce is not affected)."""
+    x = np.asarray(cc, dtype=np.float64).ravel()
+    x = x - x.min()
+    peak = x.max()
+    n = x.size
+    pce = peak * peak / (np.sum(x * x) / n)
+    width = peak / QF_BINS
+    if width > 0:
+        edges = np.arange(QF_BINS + 1, dtype=np.float64) * width
+        k = np.clip(np.searchsorted(edges, x, side="right") - 1, 0, QF_BINS - 1)
+    else:
+        k = np.zeros(n, dtype=np.int64)
+    p = np.bincount(k, minlength=QF_BINS) / n
+    p = p[p > 0]
+    ent = float(np.sum(p * np.log(1.0 / p)))
+    ppe = 1.0 / ent if ent > 0 else np.inf
+    return pce, ppe
+
+
+def xcorr_qfactors(f, g, box_f, box_g):
+    """[pce ppe] of normxcorr2(f(box_f), g(box_g)) per box (4 ints each,
+    1-based inclusive) and ok (0 where normxcorr2 fails: a constant template
+    or a box outside the image, the reference's catch)."""
+    box_f = np.asarray(box_f, dtype=np.int64).reshape(-1, 4)
+    box_g = np.asarray(box_g, dtype=np.int64).reshape(-1, 4)
+    q = np.full((box_f.shape[0], 2), np.nan)
+    ok = np.zeros(box_f.shape[0], dtype=np.int32)
+    H, W = f.shape
+    for b, (F, G) in enumerate(zip(box_f, box_g)):
+        if min(F[0], F[2], G[0], G[2]) < 1 or max(F[1], G[1]) > H or max(F[3], G[3]) > W:
+            continue
+        Cm = f[F[0] - 1:F[1], F[2] - 1:F[3]]
+        D = g[G[0] - 1:G[1], G[2] - 1:G[3]]
+        if Cm.max() == Cm.min() or D.shape[0] < Cm.shape[0] or D.shape[1] < Cm.shape[1]:
+            continue
+        q[b] = qfactor_stats(normxcorr2_full(Cm, D))
+        ok[b] = 1
+    return q, ok
+
+
 def integer_search_mg(f, g, gridx, gridy, winsize, winstepsize, trace=None):
     """funIntegerSearchMg (func/funIntegerSearchMg.m:3-285) with direct FP64
     normxcorr2 maps: returns the scattered box centres and shifts before the

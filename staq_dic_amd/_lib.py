@@ -54,6 +54,7 @@ SIGNATURES = {
                                       C.c_long, C.c_long, _lp, _lp, _dp, _dp, _dp, _dp, _dp]),
     "staq_xcorr_full_peak": (C.c_int, [_vp, C.c_int, _lp, _lp, _dp]),
     "staq_xcorr_boxes": (C.c_int, [_vp, C.c_int, C.c_long, _lp, _lp, C.c_long, _dp, _ip]),
+    "staq_xcorr_qfactors": (C.c_int, [_vp, C.c_int, C.c_long, _lp, _lp, _dp, _ip]),
     "staq_natural_interp": (C.c_int, [C.c_long, _dp, C.c_long, _ip, _ip, _lp, _ip, C.c_long, _dp,
                                       C.c_long, _dp, _ip, _dp]),
     "staq_natural_weights": (C.c_int, [C.c_long, _dp, C.c_long, _ip, _ip, _lp, _ip, C.c_long, _dp,

@@ -102,20 +102,49 @@ def funImgGradient(ImgRef, ImgDef=None, ImgRefMask=None, ctx=None):
 # func/funIntegerSearch.m, func/IntegerSearch.m
 # ---------------------------------------------------------------------------
 def funIntegerSearch(f, g, tempSizeOfSearchRegion, gridx, gridy, winsize, winstepsize,
-                     tempNoOfInitPt=0, *varargin, ctx=None, slot=0):
+                     tempNoOfInitPt=0, *varargin, ctx=None, slot=0, qfactors=True):
     """[x,y,u,v,cc] = funIntegerSearch(f, g, r, gridx, gridy, winsize, winstepsize, 0).
-    Whole-field mode (tempNoOfInitPt = 0).  cc is a dict with 'max' (ny x nx)."""
+    Whole-field mode (tempNoOfInitPt = 0).  cc is a dict with 'max' (ny x nx)
+    and 'qfactors' (one row per search point, x fastest)."""
     if tempNoOfInitPt != 0:
         raise NotImplementedError("seed-point mode needs interactive input (funIntegerSearch.m:37-74)")
     c = _c(ctx)
     c.set_frame(f, g, None, slot=slot)
     return integer_search_slot(c, slot, tempSizeOfSearchRegion, gridx, gridy, winsize,
-                               winstepsize)
+                               winstepsize, qfactors=qfactors)
 
 
-def integer_search_slot(c, slot, tempSizeOfSearchRegion, gridx, gridy, winsize, winstepsize):
+def xcorr_qfactors(c, slot, box_f, box_g, fail=np.inf, normalise=True):
+    """compute_qFactor (funIntegerSearch.m:211-264) for each box on the frame
+    pair in slot ``slot``: C = f(box_f), D = g(box_g), boxes (n, 4) 1-based
+    inclusive [x0 x1 y0 y1].  A box the reference's try rejects takes
+    ``fail`` (Inf in funIntegerSearch, :143; NaN in funIntegerSearchMg,
+    :185).  With ``normalise``, each column becomes (q - min q) / max(q -
+    min q), NaN left out of min and max, as :151-154 and
+    funIntegerSearchMg.m:192-196 do."""
+    bf = np.ascontiguousarray(np.asarray(box_f, dtype=np.int64).reshape(-1, 4))
+    bg = np.ascontiguousarray(np.asarray(box_g, dtype=np.int64).reshape(-1, 4))
+    n = bf.shape[0]
+    q = np.zeros(2 * n)
+    ok = np.zeros(n, dtype=np.int32)
+    c.check(c.L.staq_xcorr_qfactors(c.h, slot, n, lp(bf), lp(bg), d(q), ip(ok)),
+            "compute_qFactor")
+    q = q.reshape(n, 2)
+    q[ok == 0] = fail
+    if normalise and n:
+        with np.errstate(invalid="ignore", divide="ignore"):
+            for k in range(2):
+                qf = q[:, k] - np.nanmin(q[:, k])
+                q[:, k] = qf / np.nanmax(qf)
+    return q
+
+
+def integer_search_slot(c, slot, tempSizeOfSearchRegion, gridx, gridy, winsize, winstepsize,
+                        qfactors=False):
     """funIntegerSearch on the frame pair already resident in context slot
-    ``slot`` (f = its reference image, g = its deformed image)."""
+    ``slot`` (f = its reference image, g = its deformed image).  With
+    ``qfactors`` cc['qfactors'] holds the normalised [pce ppe] of every
+    search point in the reference's order (x fastest)."""
     r = np.broadcast_to(np.asarray(tempSizeOfSearchRegion, dtype=np.int64), (2,))
     w = int(np.asarray(winsize).ravel()[0])
     step = int(np.asarray(winstepsize).ravel()[0])
@@ -131,7 +160,15 @@ def integer_search_slot(c, slot, tempSizeOfSearchRegion, gridx, gridy, winsize, 
             "funIntegerSearch")
     n = nx.value * ny.value
     x, y, u, v, cmax = [o[:n].reshape((ny.value, nx.value), order="F") for o in outs]
-    return x, y, u, v, {"max": cmax, "A": []}
+    cc = {"max": cmax, "A": []}
+    if qfactors:
+        # point tempi = ci + nx (cj - 1) sits at x(cj, ci) = ii + w/2 (:194-195)
+        ii = np.rint(x.ravel() - 0.5 * w).astype(np.int64)
+        jj = np.rint(y.ravel() - 0.5 * w).astype(np.int64)
+        bf = np.stack([ii, ii + w, jj, jj + w], axis=1)
+        bg = np.stack([ii - r[0], ii + w + r[0], jj - r[1], jj + w + r[1]], axis=1)
+        cc["qfactors"] = xcorr_qfactors(c, slot, bf, bg, fail=np.inf)
+    return x, y, u, v, cc
 
 
 def IntegerSearch(ImgRef, ImgDef, file_name=None, DICpara=None, ctx=None):

@@ -10,7 +10,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libstaqdic.so")
 SOURCES = ["capi.hip", "icgn.hip", "icgn_wave.hip", "icgn_lean.hip", "gather.hip", "search.hip",
-           "mgsearch.hip", "natural.hip"]
+           "mgsearch.hip", "natural.hip", "qfactor.hip"]
 ARCH = os.environ.get("STAQ_OFFLOAD_ARCH", "gfx950")
 
 HIPCC_FLAGS = [

@@ -608,6 +608,96 @@ int staq_xcorr_boxes(staq_ctx *ctx, int slot, long nbox, const long *box_f, cons
     return STAQ_OK;
 }
 
+int staq_xcorr_qfactors(staq_ctx *ctx, int slot, long nbox, const long *box_f, const long *box_g,
+                        double *q, int *ok)
+{
+    if (!ctx || nbox < 0 || (nbox > 0 && (!box_f || !box_g || !q || !ok)))
+        return fail(ctx, STAQ_EINVAL, "null argument");
+    FrameDev fr;
+    if (!frame_desc(ctx, slot, fr)) return fail(ctx, STAQ_ENOFRAME, "frame slot not set");
+    if (nbox == 0) return STAQ_OK;
+    hipSetDevice(ctx->device);
+    // boxes that leave the image or whose g box is smaller than the template
+    // are the reference's catch; the others are computed in chunks whose maps
+    // fit the scratch budget
+    constexpr size_t kMapBudget = (size_t)16 << 20;  // placements per chunk (256 MiB)
+    constexpr size_t kLdsMax = 150 * 1024;
+    std::vector<long> good;
+    for (long b = 0; b < nbox; ++b) {
+        const long *F = box_f + 4 * b, *G = box_g + 4 * b;
+        const long n1 = F[1] - F[0] + 1, n2 = F[3] - F[2] + 1;
+        const long d1 = G[1] - G[0] + 1, d2 = G[3] - G[2] + 1;
+        ok[b] = 0;
+        q[2 * b] = q[2 * b + 1] = NAN;
+        const bool in = n1 >= 1 && n2 >= 1 && d1 >= n1 && d2 >= n2 && F[0] >= 1 && F[2] >= 1 &&
+                        F[1] <= fr.H && F[3] <= fr.W && G[0] >= 1 && G[2] >= 1 && G[1] <= fr.H &&
+                        G[3] <= fr.W;
+        if (in && (size_t)(n1 + d1 - 1) * (size_t)(n2 + d2 - 1) > kMapBudget)
+            return fail(ctx, STAQ_EINVAL, "a box's correlation map exceeds 16M placements");
+        if (in) good.push_back(b);
+    }
+    hipStream_t s = ctx->stream;
+    size_t pos = 0;
+    while (pos < good.size()) {
+        // one chunk: boxes in order while their maps fit
+        std::vector<long> bf, bg, map0, lds_ix, hbm_ix;
+        size_t np = 0, lds = 0;
+        const size_t first = pos;
+        while (pos < good.size()) {
+            const long b = good[pos];
+            const long *F = box_f + 4 * b, *G = box_g + 4 * b;
+            const long n1 = F[1] - F[0] + 1, n2 = F[3] - F[2] + 1;
+            const long d1 = G[1] - G[0] + 1, d2 = G[3] - G[2] + 1;
+            const size_t nb = (size_t)(n1 + d1 - 1) * (size_t)(n2 + d2 - 1);
+            if (np + nb > kMapBudget && pos > first) break;
+            const long k = (long)(pos - first);
+            bf.insert(bf.end(), F, F + 4);
+            bg.insert(bg.end(), G, G + 4);
+            map0.push_back((long)np);
+            const size_t need = qfactor_lds_bytes(n1, n2, d1, d2);
+            if (need <= kLdsMax) {
+                lds_ix.push_back(k);
+                lds = std::max(lds, need);
+            } else {
+                hbm_ix.push_back(k);
+            }
+            np += nb;
+            ++pos;
+        }
+        const long nb = (long)(pos - first);
+        const size_t ib = (size_t)nb * 10 * sizeof(long);
+        HIPCHK(ctx, ctx->s[6].ensure(ib + (size_t)nb * (2 * sizeof(double) + sizeof(int))));
+        HIPCHK(ctx, ctx->s[7].ensure(np * 2 * sizeof(double)));
+        long *dbf = ctx->s[6].as<long>(), *dbg = dbf + 4 * nb, *dm0 = dbg + 4 * nb;
+        long *dix = dm0 + nb;  // lds boxes, then hbm boxes
+        double *dq = reinterpret_cast<double *>(dix + nb);
+        int *dok = reinterpret_cast<int *>(dq + 2 * nb);
+        std::vector<long> ix(lds_ix);
+        ix.insert(ix.end(), hbm_ix.begin(), hbm_ix.end());
+        HIPCHK(ctx, hipMemcpyAsync(dbf, bf.data(), nb * 32, hipMemcpyHostToDevice, s));
+        HIPCHK(ctx, hipMemcpyAsync(dbg, bg.data(), nb * 32, hipMemcpyHostToDevice, s));
+        HIPCHK(ctx, hipMemcpyAsync(dm0, map0.data(), nb * 8, hipMemcpyHostToDevice, s));
+        HIPCHK(ctx, hipMemcpyAsync(dix, ix.data(), nb * 8, hipMemcpyHostToDevice, s));
+        HIPCHK(ctx, qfactor_launch(fr, dbf, dbg, dm0, ctx->s[7].as<double>(), dq, dok, dix,
+                                   (long)lds_ix.size(), lds, dix + lds_ix.size(), (long)hbm_ix.size(),
+                                   s));
+        std::vector<double> hq(2 * nb);
+        std::vector<int> hk(nb);
+        HIPCHK(ctx, hipMemcpyAsync(hq.data(), dq, nb * 16, hipMemcpyDeviceToHost, s));
+        HIPCHK(ctx, hipMemcpyAsync(hk.data(), dok, nb * 4, hipMemcpyDeviceToHost, s));
+        HIPCHK(ctx, hipStreamSynchronize(s));
+        for (long k = 0; k < nb; ++k) {
+            const long b = good[first + k];
+            ok[b] = hk[k];
+            if (hk[k]) {
+                q[2 * b] = hq[2 * k];
+                q[2 * b + 1] = hq[2 * k + 1];
+            }
+        }
+    }
+    return STAQ_OK;
+}
+
 }  // extern "C"
 
 // ---------------------------------------------------------------------------

@@ -122,6 +122,13 @@ int xcorr_full(const FrameDev &fr, const long box_f[4], const long box_g[4], voi
 // Levels >= 2: nb boxes (device arrays: bf, bg 4 per box; tile0 prefix sums of
 // xcorr_box_tiles), part = ntile * xcorr_box_part_doubles(pad) doubles
 size_t xcorr_box_part_doubles(int pad);
+// Quality factors (pce, ppe) of the whole normxcorr2 map of each box
+// (qfactor.hip): boxes_lds run with C and D in lds_bytes of LDS, boxes_hbm
+// read them from the frame; map holds 2 doubles per placement from map0[b].
+size_t qfactor_lds_bytes(long n1, long n2, long d1, long d2);
+hipError_t qfactor_launch(const FrameDev &fr, const long *bf, const long *bg, const long *map0,
+                          double *map, double *q, int *ok, const long *boxes_lds, long n_lds,
+                          size_t lds_bytes, const long *boxes_hbm, long n_hbm, hipStream_t s);
 int xcorr_box_tiles(long n1, long n2);
 hipError_t xcorr_boxes_launch(const FrameDev &fr, int pad, long nb, long ntile, const long *bf,
                               const long *bg, const int *tile0, double *part, double *out, int *ok,

@@ -176,10 +176,12 @@ def _split(lo_hi, on):
     return first, second
 
 
-def mg_scatter(c, slot, H, W, gridx, gridy, winsize):
+def mg_scatter(c, slot, H, W, gridx, gridy, winsize, last_boxes=None):
     """The level loop of funIntegerSearchMg (:3-246) on frame ``slot``:
     box centres, shifts and peak values of the last level, and the grid
-    limits after the level-1 border gap."""
+    limits after the level-1 border gap.  ``last_boxes``, a dict, receives
+    the last level's template boxes ('f') and g boxes with their +-5 ring
+    ('g', -1 where the box left the image)."""
     gx = np.array([gridx[0], gridx[-1]], dtype=float)
     gy = np.array([gridy[0], gridy[-1]], dtype=float)
     gap = 1 + _mround(0.75 * winsize)
@@ -241,6 +243,9 @@ def mg_scatter(c, slot, H, W, gridx, gridy, winsize):
         vn = np.where(ok, vn + (res[:, 0] - (PAD + 2)), np.nan)
         phi = np.where(ok, res[:, 2], 0.0)
         if wx < winsize and wy < winsize:
+            if last_boxes is not None:
+                last_boxes["f"] = bf
+                last_boxes["g"] = np.where(finite[:, None], bg + np.array([-PAD, PAD, -PAD, PAD]), -1)
             break
         xf, yf = xf2, yf2
         with np.errstate(invalid="ignore"):
@@ -267,21 +272,27 @@ def mg_scatter(c, slot, H, W, gridx, gridy, winsize):
     return tx, ty, un, vn, phi, gx, gy
 
 
-def funIntegerSearchMg(f, g, gridx, gridy, winsize, winstepsize, *varargin, ctx=None, slot=0):
+def funIntegerSearchMg(f, g, gridx, gridy, winsize, winstepsize, *varargin, ctx=None, slot=0,
+                       qfactors=True):
     """[xGrid, yGrid, uGrid, vGrid, cc] = funIntegerSearchMg(f, g, gridx, gridy,
     winsize, winstepsize), the InitFFTSearchMethod = 0 search of
-    IntegerSearch.m:122-125.  cc has 'max' (the fitted peak values) and 'A' = [];
-    the q-factors are not computed (see the package documentation)."""
+    IntegerSearch.m:122-125.  cc has 'max' (the fitted peak values), 'A' = []
+    and 'qfactors' (one row per box of the last level, :170-196)."""
     from .api import _c
     c = _c(ctx)
     c.set_frame(f, g, None, slot=slot)
     H, W = np.shape(f)
-    return mg_search_slot(c, slot, H, W, gridx, gridy, winsize, winstepsize)
+    return mg_search_slot(c, slot, H, W, gridx, gridy, winsize, winstepsize, qfactors=qfactors)
 
 
-def mg_search_slot(c, slot, H, W, gridx, gridy, winsize, winstepsize):
-    """funIntegerSearchMg on the frame pair resident in context slot ``slot``."""
-    tx, ty, tu, tv, phi, gx, gy = mg_scatter(c, slot, H, W, gridx, gridy, int(winsize))
+def mg_search_slot(c, slot, H, W, gridx, gridy, winsize, winstepsize, qfactors=False):
+    """funIntegerSearchMg on the frame pair resident in context slot ``slot``;
+    with ``qfactors`` cc['qfactors'] holds the normalised [pce ppe] of the
+    last level's boxes, NaN where a box failed (:185, :192-196)."""
+    from .api import xcorr_qfactors
+    last = {} if qfactors else None
+    tx, ty, tu, tv, phi, gx, gy = mg_scatter(c, slot, H, W, gridx, gridy, int(winsize),
+                                             last_boxes=last)
     step = np.broadcast_to(np.asarray(winstepsize, dtype=float).ravel(), (2,))
     xl = np.arange(gx[0], gx[1] + 1e-9, step[0])
     yl = np.arange(gy[0], gy[1] + 1e-9, step[1])
@@ -289,4 +300,7 @@ def mg_search_slot(c, slot, H, W, gridx, gridy, winsize, winstepsize):
     u, xg, yg = gridfit(tx[sel], ty[sel], tu[sel], xl, yl)
     v, _, _ = gridfit(tx[sel], ty[sel], tv[sel], xl, yl)
     p, _, _ = gridfit(tx[sel], ty[sel], phi[sel], xl, yl)
-    return xg, yg, u, v, {"max": p, "A": [], "qfactors": None}
+    qf = None
+    if qfactors:
+        qf = xcorr_qfactors(c, slot, last["f"], last["g"], fail=np.nan)
+    return xg, yg, u, v, {"max": p, "A": [], "qfactors": qf}

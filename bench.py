@@ -224,6 +224,7 @@ def local_step(ctx, tm, coords, win, truth, steps, warmup, slot=None, tune=True,
     conv, err, rms, rms_kept, kept = accuracy(U, st, truth, plain)
     n_conv = int(conv.sum())
     out = {"subsets": int(len(coords)), "ms_per_step": ms, "converged": n_conv,
+           "fp64_resolves": int(b.exact_count()),
            "converged_subsets_per_s": n_conv / (ms / 1e3), "rms_px": rms,
            "rms_px_kept": rms_kept, "kept": int(kept.sum()),
            "windows": {str(int(w)): int((win == w).sum()) for w in np.unique(win)},

@@ -144,7 +144,7 @@ int staq_xcorr_boxes(staq_ctx *ctx, int slot, long nbox, const long *box_f, cons
  * cc_min's histogram in 30 equal bins over [0, max(cc_min)].  ok = 1, or
  * ok = 0 and NaN where the reference's try fails (a box outside the image, a
  * g box smaller than the template, a constant template).  The callers are the
- * whole-field search (:123-125, boxes ii:ii+w by ii-r:ii+w+r) and the last
+ * whole-field search (:130-131, boxes ii:ii+w by ii-r:ii+w+r) and the last
  * level of the multigrid search (funIntegerSearchMg.m:164-173, g box +- 5). */
 int staq_xcorr_qfactors(staq_ctx *ctx, int slot, long nbox, const long *box_f, const long *box_g,
                         double *q, int *ok);

@@ -118,9 +118,9 @@ def xcorr_qfactors(c, slot, box_f, box_g, fail=np.inf, normalise=True):
     """compute_qFactor (funIntegerSearch.m:211-264) for each box on the frame
     pair in slot ``slot``: C = f(box_f), D = g(box_g), boxes (n, 4) 1-based
     inclusive [x0 x1 y0 y1].  A box the reference's try rejects takes
-    ``fail`` (Inf in funIntegerSearch, :143; NaN in funIntegerSearchMg,
+    ``fail`` (Inf in funIntegerSearch, :156; NaN in funIntegerSearchMg,
     :185).  With ``normalise``, each column becomes (q - min q) / max(q -
-    min q), NaN left out of min and max, as :151-154 and
+    min q), NaN left out of min and max, as :174-177 and
     funIntegerSearchMg.m:192-196 do."""
     bf = np.ascontiguousarray(np.asarray(box_f, dtype=np.int64).reshape(-1, 4))
     bg = np.ascontiguousarray(np.asarray(box_g, dtype=np.int64).reshape(-1, 4))
@@ -343,7 +343,7 @@ def LocalICGN(U0, coordinatesFEM, Df, ImgRef, ImgDef, DICpara, ICGNmethod="Gauss
     """[U,F,HtempPar,LocalTime,ConvItPerEle,LocalICGNBadPtNum] = LocalICGN(...) —
     the uniform-grid twin (func/LocalICGN.m): bad subsets NaN, then the
     plane fit per mask component, the F outlier removal and the nearest fill
-    of LocalICGN.m:143-287 (refill_plain.local_icgn_refill)."""
+    of LocalICGN.m:144-292 (refill_plain.local_icgn_refill)."""
     if ICGNmethod != "GaussNewton":
         raise NotImplementedError("only the Gauss-Newton scheme runs on this path")
     coords = np.asarray(coordinatesFEM, dtype=np.float64)

@@ -1,5 +1,5 @@
 // Correlation quality factors of the integer searches (compute_qFactor,
-// funIntegerSearch.m:211-264, called per search point at :123-125 and per box
+// funIntegerSearch.m:211-264, called per search point at :130-131 and per box
 // of the multigrid search's last level at funIntegerSearchMg.m:170-173), for
 // gfx950.
 //

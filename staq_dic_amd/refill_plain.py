@@ -1,7 +1,7 @@
 """Post-processing of the uniform-grid local steps on the host (FP64): the
 plain twins' counterparts of refill.refill_bad.
 
-LocalICGN (func/LocalICGN.m:117-287), after the bad subsets are set to NaN:
+LocalICGN (func/LocalICGN.m:117-292), after the bad subsets are set to NaN:
   1. for every 8-connected component of the reference mask, the nodes with a
      finite u whose rounded position lies in it are replaced by a local plane
      fit (funCompDefGrad2, func_quadtree/funCompDefGrad2.m:38-150): for each
@@ -12,10 +12,10 @@ LocalICGN (func/LocalICGN.m:117-287), after the bad subsets are set to NaN:
      [u_x, v_x, u_y, v_y].  With 3 or fewer such neighbours, or a mask window
      that leaves the image, the node keeps its u and its F becomes NaN;
   2. rmoutliers(F_k, 'movmedian', 1 + winstepsize) on each F component in node
-     order; a node flagged in any component loses all four (:195-209);
+     order; a node flagged in any component loses all four (:218-233);
   3. the NaN nodes take the value of their nearest node with a finite value,
      U from the finite u, F from the finite F (scatteredInterpolant 'nearest',
-     'nearest', :266-287).
+     'nearest', :267-292).
 funCompDefGrad2 also re-fills F by linear interpolation inside its node loop
 (:160-176), from the F that is finite at that moment: one value after the
 first node, more as the loop goes on.  What MATLAB's scatteredInterpolant
@@ -134,7 +134,7 @@ def _nearest_fill(coords, values, finite):
 
 
 def local_icgn_refill(U, F, coordinatesFEM, mask, winstepsize):
-    """Stages 1-3 of LocalICGN.m:143-287 on U (2N) and F (4N) whose bad nodes
+    """Stages 1-3 of LocalICGN.m:144-292 on U (2N) and F (4N) whose bad nodes
     are NaN already; returns new U, F."""
     c = np.asarray(coordinatesFEM, dtype=np.float64)
     N = c.shape[0]
@@ -171,7 +171,7 @@ def local_icgn_refill(U, F, coordinatesFEM, mask, winstepsize):
 
 
 def subpb1_gridfit(U, coordinatesFEM):
-    """Subpb1.m:185-199: gridfit of u and v through the nodes with a finite u
+    """Subpb1.m:186-199: gridfit of u and v through the nodes with a finite u
     on unique(x) x unique(y), read back at every node."""
     c = np.asarray(coordinatesFEM, dtype=np.float64)
     N = c.shape[0]

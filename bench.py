@@ -482,6 +482,180 @@ def al_loop_hole_leg(ctx, al_steps=5):
     return r
 
 
+def sequence_leg(ctx, n_frames=40, gn_steps=1, beta=1e-3):
+    """The large-deformation sequence end to end: frames 0..n_frames of the
+    4096^2 16-bit pattern under the cumulative stretch-and-bump field, each
+    frame made on the device (inverse map by Newton's method, Keys bicubic of
+    the pattern through staq_ba_interp2_dev, 16-bit quantisation, normalised
+    over the image); for every increment k-1 -> k (reference frame updated
+    each frame, as main_ALDIC_Quadtree_inc.m:64-73 does) the 6-DOF local step
+    as one captured hipGraph replayed on the new pair, then gn_steps
+    Gauss-Newton steps of the adjoint-fed global step (total ZNSSD plus the
+    reference's F regulariser, beta) on the uniform 16-px FE mesh (MeshSetUp,
+    64 516 nodes, windows 32).  Increment 1 starts from the rounded analytic
+    increment, every later one from the previous increment's result, rounded.
+    Per increment: local-step milliseconds, global-step seconds, RMS of both
+    against the analytic increment over the converged subsets (and of the
+    global step over every node, the non-converged ones included)."""
+    import ctypes as C
+
+    import torch
+    from staq_dic_amd import Batch, params, synth
+    from staq_dic_amd._lib import IN_U0
+    from staq_dic_amd.adjoint_global import AdjointGlobalStep
+    from staq_dic_amd.mesh import MeshSetUp, uniform_nodes
+    H = W = 4096
+    bits = 16
+    t0 = time.perf_counter()
+    f0 = synth.speckle_image(H, W, 192000, 1004, bits)
+    t_pattern = time.perf_counter() - t0
+    dev = torch.device("cuda")
+    ext = torch.cuda.ExternalStream(ctx.stream)
+    vp = lambda t: C.c_void_p(t.data_ptr())
+    xn, yn = uniform_nodes(H, W, 32, 16)
+    coords, el4 = MeshSetUp(xn, yn)
+    N = len(coords)
+    win = np.full(N, 32, dtype=np.int64)
+    mesh = {"coordinatesFEM": coords,
+            "elementsFEM": np.concatenate([el4, np.zeros((len(el4), 4), dtype=np.int64)], 1)}
+    px = np.round(coords[:, 0]).astype(np.int64)
+    py = np.round(coords[:, 1]).astype(np.int64)
+    with torch.cuda.stream(ext):
+        # images as x-fastest (W, H) tensors: the column-major H x W layout
+        F0 = torch.from_numpy(np.ascontiguousarray(f0.T)).to(dev)
+        del f0
+        xg = torch.arange(1, H + 1, dtype=torch.float64, device=dev).expand(W, H).contiguous()
+        yg = torch.arange(1, W + 1, dtype=torch.float64, device=dev)[:, None].expand(W, H)
+        yg = yg.contiguous()
+        ones = torch.ones((W, H), dtype=torch.float32, device=dev)
+
+        def frame(k):
+            s_k, amp, sig = 0.2 * k / 40, 40.0 * k / 40, 300.0
+            xc = H / 2 + 0.5
+            yc = W / 2 + 0.5
+            # the field moves x only (v = 0), so y is its own pre-image and
+            # Newton's method on X + u(X, y) = x is scalar
+            X = xg.clone()
+            for _ in range(30):
+                gb = torch.exp(-((X - xc) ** 2 + (yg - yc) ** 2) / (2 * sig * sig))
+                r = X + s_k * (X - xc) + amp * gb - xg
+                dX = r / (1.0 + s_k - amp * gb * (X - xc) / (sig * sig))
+                X = X - dX
+                if float(dX.abs().max()) < 1e-10:
+                    break
+            Z = torch.empty_like(X)
+            # ba_interp2(F, Xc, Yc): Xc the dim-2 (y) coordinate, Yc the dim-1 (x)
+            ctx.check(ctx.L.staq_ba_interp2_dev(ctx.h, vp(F0), H, W, 1, vp(yg), vp(X), H * W,
+                                                vp(Z), 0, C.c_void_p(ctx.stream)),
+                      "staq_ba_interp2_dev")
+            Z = torch.clamp(torch.round(Z), 0, (1 << bits) - 1)
+            return ((Z - Z.mean()) / Z.std()).to(torch.float32)
+
+        def set_pair(fa, fb):
+            ctx.check(ctx.L.staq_set_frame_dev(ctx.h, 0, vp(fa), vp(fb), vp(ones), H, W,
+                                               C.c_void_p(ctx.stream)), "staq_set_frame_dev")
+            ctx.frames[0] = (H, W)
+
+        def box_energy(fa):
+            # sum (f - mean f)^2 over each node's window, from integral images (FP64)
+            f64 = fa.to(torch.float64)
+            I1 = torch.nn.functional.pad(f64.cumsum(0).cumsum(1), (1, 0, 1, 0))
+            I2 = torch.nn.functional.pad((f64 * f64).cumsum(0).cumsum(1), (1, 0, 1, 0))
+            x0 = torch.from_numpy(px - 16 - 1).to(dev)
+            x1 = torch.from_numpy(px + 16).to(dev)
+            y0 = torch.from_numpy(py - 16 - 1).to(dev)
+            y1 = torch.from_numpy(py + 16).to(dev)
+            box = lambda I: I[y1, x1] - I[y0, x1] - I[y1, x0] + I[y0, x0]
+            s1, s2 = box(I1), box(I2)
+            return (s2 - s1 * s1 / 33.0 ** 2).cpu().numpy()
+
+        t0 = time.perf_counter()
+        prev = frame(0)
+        torch.cuda.synchronize()
+        t_first_frame = time.perf_counter() - t0
+        b = None
+        p6 = params(tol=1e-3)
+        rows = []
+        U_prev = None
+        for k in range(1, n_frames + 1):
+            t_f = time.perf_counter()
+            cur = frame(k)
+            set_pair(prev, cur)
+            torch.cuda.synchronize()
+            t_f = time.perf_counter() - t_f
+            fld = synth.IncrementalField(synth.stretch_bump_frame_field(k - 1, H, W),
+                                         synth.stretch_bump_frame_field(k, H, W))
+            truth = synth.truth_at(fld, coords)
+            U0 = np.round(truth[:, 4:6]).reshape(-1) if U_prev is None else np.round(U_prev)
+            if b is None:
+                b = Batch(ctx, coords, win)
+                b.set_input(IN_U0, U0)
+                b.icgn6(p6)
+                b.reorder()
+                b.graph_capture(p6, 6)
+            b.set_input(IN_U0, U0)
+            ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            ev0.record(ext)
+            b.graph_launch()
+            ev1.record(ext)
+            Ul, Fl, Hl, st = b.outputs()
+            local_ms = ev0.elapsed_time(ev1)
+            conv = (st >= 1) & (st <= 100)
+            t_g = time.perf_counter()
+            bf2 = box_energy(prev)
+            D = np.zeros((N, 2, 2))
+            D[:, 0, 0], D[:, 0, 1], D[:, 1, 1] = Hl[:, 18], Hl[:, 19], Hl[:, 20]
+            D[:, 1, 0] = D[:, 0, 1]
+            D = 2.0 * D / bf2[:, None, None]
+            # subsets that did not converge: their neighbours' mean carries them
+            Ug = np.where(np.repeat(conv, 2), Ul, np.nan).reshape(N, 2)
+            Fg = np.where(np.repeat(conv, 4), Fl, np.nan).reshape(N, 4)
+            bad = ~conv
+            if bad.any():
+                from scipy.spatial import cKDTree
+                tree = cKDTree(coords[conv])
+                for i in np.flatnonzero(bad):
+                    nb = tree.query_ball_point(coords[i], 24.0)
+                    if nb:
+                        Ug[i] = Ug[conv][nb].mean(0)
+                        Fg[i] = Fg[conv][nb].mean(0)
+                    else:
+                        Ug[i] = U0.reshape(N, 2)[i]
+                        Fg[i] = 0.0
+                D[bad] = np.median(D[conv], axis=0)
+            Ua, costs = AdjointGlobalStep(mesh, ctx, win, Ug.ravel(), Fg.ravel(), beta, D,
+                                          n_iter=gn_steps)
+            t_g = time.perf_counter() - t_g
+            err = lambda U: np.hypot(U[0::2] - truth[:, 4], U[1::2] - truth[:, 5])
+            rows.append({"frame": k, "frame_s": t_f, "local_ms": local_ms,
+                         "converged": int(conv.sum()), "global_s": t_g,
+                         "rms_local_px": float(np.sqrt(np.mean(err(Ul)[conv] ** 2))),
+                         "rms_global_px": float(np.sqrt(np.mean(err(Ua)[conv] ** 2))),
+                         "rms_global_px_all_nodes": float(np.sqrt(np.mean(err(Ua) ** 2))),
+                         "znssd_before_after": [costs[0], costs[-1]]})
+            log(f"[bench] sequence frame {k}: local {local_ms:.2f} ms, {int(conv.sum())}/{N} "
+                f"converged, rms local {rows[-1]['rms_local_px']:.4f} px, global "
+                f"{rows[-1]['rms_global_px']:.4f} px ({rows[-1]['rms_global_px_all_nodes']:.4f} "
+                f"px over all nodes, {t_g:.2f} s)")
+            U_prev = Ua
+            prev = cur
+        b.close()
+    loc = np.array([r["local_ms"] for r in rows])
+    return {"workload": f"large-deformation-4096 sequence, frames 0-{n_frames} made on the "
+                        "device, incremental pairs, uniform 16-px FE mesh (64 516 nodes, "
+                        f"windows 32), 6-DOF local step as one replayed hipGraph, {gn_steps} "
+                        f"Gauss-Newton step(s) of the adjoint-fed global step (beta {beta})",
+            "frames": n_frames, "nodes": int(N), "seconds_pattern_cpu": t_pattern,
+            "seconds_first_frame": t_first_frame,
+            "local_ms_mean": float(loc.mean()), "local_ms_max": float(loc.max()),
+            "local_subsets_per_s": float(N / (loc.mean() / 1e3)),
+            "global_s_mean": float(np.mean([r["global_s"] for r in rows])),
+            "frame_s_mean": float(np.mean([r["frame_s"] for r in rows])),
+            "rms_local_px_max": float(max(r["rms_local_px"] for r in rows)),
+            "rms_global_px_max": float(max(r["rms_global_px"] for r in rows)),
+            "per_frame": rows}
+
+
 def adjoint_bytes(w):
     """Algorithmic bytes of one forward + backward pass over a subset of
     window w: f ((w+1)^2 FP32) and the bicubic support of g ((w+4)^2 FP32)
@@ -560,7 +734,9 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-extras", action="store_true")
-    ap.add_argument("--leg", choices=["adjoint", "al"], default=None,
+    ap.add_argument("--no-sequence", action="store_true",
+                    help="skip the 40-increment large-deformation sequence (about a minute)")
+    ap.add_argument("--leg", choices=["adjoint", "al", "sequence"], default=None,
                     help="run one leg alone and print its JSON (a profiling aid)")
     ap.add_argument("--copy-calib", action="store_true",
                     help="first run a 256 MiB elementwise copy (read 256 MiB, write 256 MiB) "
@@ -587,6 +763,8 @@ def main():
         ctx = Context(0)
         if args.leg == "adjoint":
             out = adjoint_leg(ctx, Timer(torch.cuda.ExternalStream(ctx.stream)), args.steps)
+        elif args.leg == "sequence":
+            out = sequence_leg(ctx)
         else:
             out = al_loop_hole_leg(ctx)
         print(json.dumps(out))
@@ -664,6 +842,16 @@ def main():
         result["al_frame"] = al_frame_leg(ctx)
         result["al_loop_hole_1024"] = al_loop_hole_leg(ctx)
         ctx.close()
+        if not args.no_sequence:
+            # the 40 increments of the large-deformation sequence, local steps
+            # replayed from one graph, adjoint-fed global steps on the host
+            ctx = Context(0)
+            seq = sequence_leg(ctx)
+            ctx.close()
+            seq["per_frame"] = [{k: (round(v, 6) if isinstance(v, float) else v)
+                                 for k, v in r.items() if k != "znssd_before_after"}
+                                for r in seq["per_frame"]]
+            result["sequence_large_4096"] = seq
 
     if not args.no_cpu:
         from oracle import pyoracle

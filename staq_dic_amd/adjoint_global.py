@@ -28,7 +28,7 @@ import scipy.sparse as sp
 import scipy.sparse.linalg as spla
 
 from ._lib import d, ip, lp
-from .global_step import assemble_subpb2
+from .global_step import _cached, _digest, assemble_subpb2
 
 __all__ = ["znssd_cost_grad", "znssd_curvature", "AdjointGlobalStep"]
 
@@ -100,12 +100,28 @@ def AdjointGlobalStep(DICmesh, ctx, winsize, U, F, beta, D, n_iter=1, GaussPtOrd
     Uh = np.asarray(U, dtype=np.float64).reshape(N, 2).copy()
     # the regulariser's operator and its F term (global_step.assemble_subpb2
     # with mu = 0); the element Jacobians of MeshSetUp's corner order have a
-    # negative determinant, which flips A and b together: take K positive
-    K, bF = assemble_subpb2(coords, el, GaussPtOrder, 1.0, 0.0, np.zeros(2 * N), F.ravel())
-    if K.diagonal()[:2 * N].sum() < 0:
-        K, bF = -K, -bF
-    K = K[:2 * N, :2 * N].tocsr()
-    bF = bF[:2 * N]
+    # negative determinant, which flips A and b together: take K positive.
+    # K depends on the mesh alone and is kept per mesh; b_F is assembled here.
+    # The F term of a node's DOF takes that node's own F only
+    # (Subpb2Quadtree.m:259-262): b_F[u_a] = cx_a F11_a + cy_a F12_a and
+    # b_F[v_a] = cx_a F21_a + cy_a F22_a, with cx, cy the node's sums of the
+    # weighted shape-function derivatives, kept per mesh with K.
+    def build():
+        z2 = np.zeros(2 * N)
+        K0, _ = assemble_subpb2(coords, el, GaussPtOrder, 1.0, 0.0, z2, np.zeros(4 * N))
+        sign = -1.0 if K0.diagonal()[:2 * N].sum() < 0 else 1.0
+        e11 = np.zeros((N, 4))
+        e11[:, 0] = 1.0
+        e12 = np.zeros((N, 4))
+        e12[:, 2] = 1.0
+        _, bx = assemble_subpb2(coords, el, GaussPtOrder, 1.0, 0.0, z2, e11.ravel(), need_A=False)
+        _, by = assemble_subpb2(coords, el, GaussPtOrder, 1.0, 0.0, z2, e12.ravel(), need_A=False)
+        return (sign * K0)[:2 * N, :2 * N].tocsr(), sign * bx[0:2 * N:2], sign * by[0:2 * N:2]
+
+    K, cx, cy = _cached(_digest("adjoint K", coords, el, GaussPtOrder), build)
+    bF = np.empty(2 * N)
+    bF[0::2] = cx * F[:, 0] + cy * F[:, 2]
+    bF[1::2] = cx * F[:, 1] + cy * F[:, 3]
     involved = np.unique(el[:, :4]) - 1
     free = np.sort(np.concatenate([2 * involved, 2 * involved + 1]))
     rows = np.repeat(np.arange(2 * N).reshape(N, 2), 2, axis=1).ravel()
@@ -121,7 +137,8 @@ def AdjointGlobalStep(DICmesh, ctx, winsize, U, F, beta, D, n_iter=1, GaussPtOrd
         rhs = -(g.ravel() + beta * (K @ u - bF))
         A = (Dm + beta * K)[free][:, free].tocsc()
         du = np.zeros(2 * N)
-        du[free] = spla.spsolve(A, rhs[free])
+        # A is symmetric: a minimum-degree ordering of A + A^T fills in less
+        du[free] = spla.spsolve(A, rhs[free], permc_spec="MMD_AT_PLUS_A")
         Uh = (u + du).reshape(N, 2)
     cost, st, _ = znssd_cost_grad(ctx, coords, winsize, F, Uh)
     costs.append(float(cost[st == 0].sum()))

@@ -988,3 +988,37 @@ def test_full_size_multi_pair_and_adjoint(ctx, oracle):
     rel = np.max(np.abs(g[ok] - go[ok]) / scale)
     _print("   adjoint: largest gradient error relative to each parameter's scale", rel)
     assert rel < 2e-3
+
+
+def test_graph_replays_on_a_new_frame_pair(ctx, oracle):
+    """A local step captured once as a hipGraph and replayed after the frame
+    pair in its slot is replaced (same size: the slot keeps its buffers) gives
+    the same results as an eager step on the new pair, and those match the
+    oracle: the per-frame replay of an image sequence."""
+    from staq_dic_amd import Batch, params, synth
+    from staq_dic_amd._lib import IN_U0
+    fa, coords, win = synth.config_plain(seed=1101)
+    fb, _, _ = synth.config_plain(seed=1102)
+    p6 = params(tol=1e-3)
+    ctx.set_frame(fa.f, fa.g, fa.mask, slot=0)
+    b = Batch(ctx, coords, win)
+    ta = synth.truth_at(fa.field, coords)
+    b.set_input(IN_U0, np.round(ta[:, 4:6]).reshape(-1))
+    b.icgn6(p6)
+    b.reorder()
+    b.graph_capture(p6, 6)
+    ctx.set_frame(fb.f, fb.g, fb.mask, slot=0)
+    tb = synth.truth_at(fb.field, coords)
+    U0 = np.round(tb[:, 4:6]).reshape(-1)
+    b.set_input(IN_U0, U0)
+    b.graph_launch()
+    Ug, Fg, Hg, sg = b.outputs()
+    b.icgn6(p6)
+    Ue, Fe, He, se = b.outputs()
+    np.testing.assert_array_equal(sg, se)
+    np.testing.assert_array_equal(Ug, Ue)
+    np.testing.assert_array_equal(Fg, Fe)
+    b.close()
+    of = oracle.Frame(fb.f, fb.g, fb.mask)
+    Uo, Fo, Ho, so = oracle.local_icgn(of, U0, coords, win, tol=1e-3)
+    _cmp_icgn(sg, so, Ug, Uo, Fg, Fo, name="graph on a new pair")

@@ -798,6 +798,7 @@ def main():
         "kept": head["kept"], "rms_px_kept": head["rms_px_kept"],
         "accuracy_gate_0p01px": {"all_converged": bool(head["rms_px"] <= 0.01),
                                  "kept_by_reference_rule": bool(head["rms_px_kept"] <= 0.01)},
+        "fp64_resolves": head["fp64_resolves"],
         "schedule": head["schedule"],
         "schedule_note": "the measured graph's bin order (longest-first) comes from a prior "
                          "eager pass over the same inputs, and its kernel forms from the "

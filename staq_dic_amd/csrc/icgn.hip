@@ -54,6 +54,8 @@ struct Ctl {
     int rform;                // the window has no masked pixel: residual form of S1, S2
     int exact;                // a stopping test fell inside the guard band
     int tile_u0, tile_v0, tnx, tny, use_tile;
+    int tile_ok;       // the LDS tile holds g over (tile_u0, tile_v0, tnx, tny)
+    int reload;        // this iteration's support left the resident tile
     int xa, ya, wx, wy, nx, ny, npix, w0;
     int count, it, state, status, sum_bw2;
     long idx;
@@ -220,6 +222,7 @@ __global__ __launch_bounds__(BS) void icgn_kernel(IcgnArgs a)
             c.fsel = 0;
             c.kap = 1.0;
             c.exact = 0;
+            c.tile_ok = 0;
             c.rform = 0;
             const FrameDev fr = a.frames[c.slot];
             if (a.only_deferred && a.status[idx] != ST_DEFER && a.status[idx] != ST_EXACT) {
@@ -538,12 +541,24 @@ __global__ __launch_bounds__(BS) void icgn_kernel(IcgnArgs a)
                             c.state = 1;
                         } else {
                             const int tu0 = (int)floor(umin) - 2, tv0 = (int)floor(vmin) - 2;
-                            c.tile_u0 = tu0;
-                            c.tile_v0 = tv0;
-                            c.tnx = (int)floor(umax) - (int)floor(umin) + 6;
-                            c.tny = (int)floor(vmax) - (int)floor(vmin) + 6;
-                            c.use_tile = c.tnx * c.tny <= a.tile_cap;
-                            c.sw = split_warp(P, c.x0, c.y0, tu0, tv0);
+                            const int nxt = (int)floor(umax) - (int)floor(umin) + 6;
+                            const int nyt = (int)floor(vmax) - (int)floor(vmin) + 6;
+                            // the resident tile serves while it holds the new
+                            // support with the same margins (the samples are the
+                            // same, so the gather is unchanged)
+                            const bool keep = c.tile_ok && tu0 >= c.tile_u0 && tv0 >= c.tile_v0 &&
+                                              tu0 + nxt <= c.tile_u0 + c.tnx &&
+                                              tv0 + nyt <= c.tile_v0 + c.tny;
+                            if (!keep) {
+                                c.tile_u0 = tu0;
+                                c.tile_v0 = tv0;
+                                c.tnx = nxt;
+                                c.tny = nyt;
+                                c.use_tile = nxt * nyt <= a.tile_cap;
+                                c.tile_ok = c.use_tile;
+                            }
+                            c.reload = !keep && c.use_tile;
+                            c.sw = split_warp(P, c.x0, c.y0, c.tile_u0, c.tile_v0);
                         }
                     }
                 }
@@ -553,7 +568,7 @@ __global__ __launch_bounds__(BS) void icgn_kernel(IcgnArgs a)
 
             const int tu0 = c.tile_u0, tv0 = c.tile_v0, tnx = c.tnx, tny = c.tny;
             const bool use_tile = c.use_tile;
-            if (use_tile) {
+            if (use_tile && c.reload) {
                 // flat over the tile, 4 independent loads in flight per thread
                 const int total = tnx * tny;
                 const float rcpn = 1.0f / tnx;

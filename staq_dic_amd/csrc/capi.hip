@@ -1137,8 +1137,9 @@ static IcgnArgs base_args(staq_batch *b, const staq_params *p)
 // the subsets whose stopping test fell inside the guard band and of the
 // windows that outgrew their kernel. Launched unconditionally (capturable in
 // a graph); their workgroups leave at once when the list is empty. Two
-// launches: windows of up to 32 px (LDS for their grown windows only, so
-// that five subsets run per CU), then the larger ones, gathering from HBM.
+// launches: windows of up to 32 px (LDS for their grown windows: g in FP64,
+// DfDx and DfDy, the g tile; two subsets per CU), then the larger ones,
+// gathering from HBM.
 // q[0], q[1]: queue head and list length of the first; q[3], q[4]: of the
 // second (q[2] counts the FP64 re-solves).
 static hipError_t fallback_launches(IcgnArgs a, int mode, int *q, int fb_npix_cap, hipStream_t s)
@@ -1151,6 +1152,7 @@ static hipError_t fallback_launches(IcgnArgs a, int mode, int *q, int fb_npix_ca
     a.only_deferred = 1;
     a.npix_cap = std::min(kSmallFallbackPix, fb_npix_cap);
     a.tile_cap = tile_side(48) * tile_side(48);  // the warped support of a 48 x 48 window
+    a.grad_lds = 1;                              // DfDx, DfDy of the window in LDS
     a.work_counter = q;
     a.defer_flag = q + 1;
     a.list_back = 0;
@@ -1158,6 +1160,7 @@ static hipError_t fallback_launches(IcgnArgs a, int mode, int *q, int fb_npix_ca
     if (e != hipSuccess) return e;
     a.npix_cap = fb_npix_cap;
     a.tile_cap = 0;  // gathers from HBM
+    a.grad_lds = 0;
     a.work_counter = q + 3;
     a.defer_flag = q + 4;
     a.list_back = 1;

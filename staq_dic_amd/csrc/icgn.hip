@@ -139,14 +139,23 @@ __device__ void flood_fill(const unsigned char *bw, unsigned char *out, int nr, 
     }
 }
 
+// DfDx, DfDy at window pixel p (image offset gi): from the LDS stage when the
+// workgroup holds one (gl), else from HBM
+__device__ __forceinline__ float2 grad_px(bool gl, const float2 *gradf, const FrameDev &fr, int p, size_t gi)
+{
+    return gl ? gradf[p] : make_float2(gld(fr.dfdx, gi), gld(fr.dfdy, gi));
+}
+
 // EXACT: the FP64 re-solve of subsets whose stopping test fell inside the
 // guard band (and of windows that outgrew their bin): the bicubic gather is
 // evaluated in FP64 (from the FP32 image, as every path reads it) and every
 // iteration takes the two-pass FP64 statistics and the per-pixel residual
 // of funICGNQuadtree.m:269-371 / funICGN_Subpb1.m:270-379, so the iteration
-// count is the one an FP64 evaluation gives.
+// count is the one an FP64 evaluation gives.  The re-solve asks for two waves
+// per SIMD (256 VGPRs at most): left alone the compiler takes 276 and one
+// workgroup per CU, so that long re-solves queue behind each other.
 template <int BS, int MODE, bool EXACT>
-__global__ __launch_bounds__(BS) void icgn_kernel(IcgnArgs a)
+__global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(EXACT ? 2 : 1))) void icgn_kernel(IcgnArgs a)
 {
     constexpr int NW = BS / kWave;
     constexpr int NJ = MODE;  // steepest-descent components
@@ -158,7 +167,11 @@ __global__ __launch_bounds__(BS) void icgn_kernel(IcgnArgs a)
     __shared__ long item_sh;
 
     GT *gbuf = reinterpret_cast<GT *>(smem);
-    float *tile = reinterpret_cast<float *>(gbuf + a.npix_cap);
+    // the re-solve may stage the window's DfDx, DfDy after g (gl)
+    const bool gl = EXACT && a.grad_lds;
+    unsigned char *after_g = smem + sizeof(GT) * (size_t)a.npix_cap;
+    float2 *gradf = reinterpret_cast<float2 *>(after_g);
+    float *tile = reinterpret_cast<float *>(after_g + (gl ? sizeof(float2) * (size_t)a.npix_cap : 0));
     unsigned char *tm = reinterpret_cast<unsigned char *>(tile + a.tile_cap);
     unsigned char *sel0 = tm + a.npix_cap;
     unsigned char *sel = sel0 + a.npix_cap;
@@ -331,6 +344,32 @@ __global__ __launch_bounds__(BS) void icgn_kernel(IcgnArgs a)
             flood_fill<BS>(tm, sel0, nx, ny, seed_r, seed_c, 4, npix, rcp);
         }
 
+        if (gl && c.state == 0) {
+            // DfDx, DfDy of the window, read once: the window keeps its size
+            // over the iterations (a new size ends the subset), so every pass
+            // below reads them from LDS; 4 loads in flight per thread
+            for (int p0 = 0; p0 < npix; p0 += BS * 4) {
+                float2 q4[4];
+#pragma unroll
+                for (int q = 0; q < 4; ++q) {
+                    const int p = p0 + tid + BS * q;
+                    q4[q] = make_float2(0.f, 0.f);
+                    if (p < npix) {
+                        int i, j;
+                        pix_ij(p, nx, rcp, i, j);
+                        const size_t gi = (size_t)((xa + i - 4) + (ya + j - 4) * Hd);
+                        q4[q] = make_float2(gld(fr.dfdx, gi), gld(fr.dfdy, gi));
+                    }
+                }
+#pragma unroll
+                for (int q = 0; q < 4; ++q) {
+                    const int p = p0 + tid + BS * q;
+                    if (p < npix) gradf[p] = q4[q];
+                }
+            }
+            __syncthreads();
+        }
+
         // f-side sums in FP64 (:91-103, :119-120): statistics and the constant
         // parts of b first, the 21 Hessian entries in a second pass
         if (c.state == 0) {
@@ -345,8 +384,8 @@ __global__ __launch_bounds__(BS) void icgn_kernel(IcgnArgs a)
                 const int x = xa + i, y = ya + j;
                 const bool s = all || sel0[p];
                 const size_t gi = (size_t)((x - 4) + (y - 4) * Hd);
-                const double fx = s ? (double)gld(fr.dfdx, gi) : 0.0;
-                const double fy = s ? (double)gld(fr.dfdy, gi) : 0.0;
+                const float2 gq = s ? grad_px(gl, gradf, fr, p, gi) : make_float2(0.f, 0.f);
+                const double fx = gq.x, fy = gq.y;
                 double fv = fidx(fr.f, fr.H, x, y);
                 if (!(MODE == 6 && a.plain)) fv *= (double)fidx(fr.mask, fr.H, x, y);
                 if (!s) fv = 0.0;
@@ -422,8 +461,8 @@ __global__ __launch_bounds__(BS) void icgn_kernel(IcgnArgs a)
                 const int x = xa + i, y = ya + j;
                 const bool s = all || sel0[p];
                 const size_t gi = (size_t)((x - 4) + (y - 4) * Hd);
-                const double fx = s ? (double)gld(fr.dfdx, gi) : 0.0;
-                const double fy = s ? (double)gld(fr.dfdy, gi) : 0.0;
+                const float2 gq = s ? grad_px(gl, gradf, fr, p, gi) : make_float2(0.f, 0.f);
+                const double fx = gq.x, fy = gq.y;
                 const double dx = i - (double)hx, dy = j - (double)hy;
                 const double J[6] = {fx * dx, fy * dx, fx * dy, fy * dy, fx, fy};
                 int t = 0;
@@ -606,11 +645,11 @@ __global__ __launch_bounds__(BS) void icgn_kernel(IcgnArgs a)
                 const double cd = c.cf;
                 const bool whole = MODE == 6 && a.plain;
                 const bool all = c.sel0_all;
+                // v[K], v[K + 1]: the counts of zeros and NaNs of g (one reduction)
                 constexpr int K = 3 + NJ;
-                double v[K];
+                double v[K + 2];
 #pragma unroll
-                for (int k = 0; k < K; ++k) v[k] = 0.0;
-                float acc[2] = {0.f, 0.f};
+                for (int k = 0; k < K + 2; ++k) v[k] = 0.0;
                 for (int p = tid; p < npix; p += BS) {
                     int i, j;
                     pix_ij(p, nx, rcp, i, j);
@@ -647,8 +686,8 @@ __global__ __launch_bounds__(BS) void icgn_kernel(IcgnArgs a)
                         }
                     }
                     gbuf[p] = g;
-                    acc[0] += g == 0.0 ? 1.f : 0.f;
-                    acc[1] += isnan(g) ? 1.f : 0.f;
+                    v[K] += g == 0.0 ? 1.0 : 0.0;
+                    v[K + 1] += isnan(g) ? 1.0 : 0.0;
                     const bool nz = whole || fabs(g) > 1e-10;
                     const double e = g - cd;
                     v[0] += nz ? 1.0 : 0.0;
@@ -657,7 +696,8 @@ __global__ __launch_bounds__(BS) void icgn_kernel(IcgnArgs a)
                     const bool sf = all || sel0[p];
                     if (sf) {
                         const size_t gi = (size_t)((xa + i - 4) + (ya + j - 4) * Hd);
-                        const double fx = gld(fr.dfdx, gi), fy = gld(fr.dfdy, gi);
+                        const float2 gq = grad_px(gl, gradf, fr, p, gi);
+                        const double fx = gq.x, fy = gq.y;
                         if (MODE == 6) {
                             v[3] += fx * dx * e;
                             v[4] += fy * dx * e;
@@ -671,16 +711,14 @@ __global__ __launch_bounds__(BS) void icgn_kernel(IcgnArgs a)
                         }
                     }
                 }
-                float o[2];
-                double od[K];
-                block_sum<BS, 2>(acc, redf, o);
-                block_sum<BS, K>(v, redd, od);
+                double od[K + 2];
+                block_sum<BS, K + 2>(v, redd, od);
                 if (tid == 0) {
                     const bool quad = !(MODE == 6 && a.plain);
-                    if (quad && o[1] > 0) {
+                    if (quad && od[K + 1] > 0) {
                         c.status = ST_ERROR;  // logical(NaN) throws
                         c.state = 2;
-                    } else if (quad && (c.count > 0 || o[0] > 0)) {
+                    } else if (quad && (c.count > 0 || od[K] > 0)) {
                         c.path = 1;
                     } else if (MODE == 2 && c.fsel) {
                         c.path = 2;
@@ -883,7 +921,8 @@ __global__ __launch_bounds__(BS) void icgn_kernel(IcgnArgs a)
                         const bool s = fsel ? (bool)sel[p] : (all || sel0[p]);
                         if (s) {
                             const size_t gi = (size_t)((x - 4) + (y - 4) * Hd);
-                            const double fx = gld(fr.dfdx, gi), fy = gld(fr.dfdy, gi);
+                            const float2 gq = grad_px(gl, gradf, fr, p, gi);
+                            const double fx = gq.x, fy = gq.y;
                             if (MODE == 6) {
                                 const double dx = i - (double)hx, dy = j - (double)hy;
                                 v[4] += dx * fx * e;
@@ -990,8 +1029,8 @@ __global__ __launch_bounds__(BS) void icgn_kernel(IcgnArgs a)
                             const int x = xa + i, y = ya + j;
                             const bool s = sel[p];
                             const size_t gi = (size_t)((x - 4) + (y - 4) * Hd);
-                            const double fx = s ? (double)gld(fr.dfdx, gi) : 0.0;
-                            const double fy = s ? (double)gld(fr.dfdy, gi) : 0.0;
+                            const float2 gq = s ? grad_px(gl, gradf, fr, p, gi) : make_float2(0.f, 0.f);
+                            const double fx = gq.x, fy = gq.y;
                             const double fv = s ? (double)fidx(fr.f, fr.H, x, y) * (double)fidx(fr.mask, fr.H, x, y) : 0.0;
                             const bool fnz = fabs(fv) > 1e-10;
                             v[3] += fnz ? 1.0 : 0.0;
@@ -1078,8 +1117,8 @@ __global__ __launch_bounds__(BS) void icgn_kernel(IcgnArgs a)
                             if (!s) fv = 0.0;
                         }
                         const size_t gi = (size_t)((x - 4) + (y - 4) * Hd);
-                        const double fx = s ? (double)gld(fr.dfdx, gi) : 0.0;
-                        const double fy = s ? (double)gld(fr.dfdy, gi) : 0.0;
+                        const float2 gq = s ? grad_px(gl, gradf, fr, p, gi) : make_float2(0.f, 0.f);
+                        const double fx = gq.x, fy = gq.y;
                         const double r = (fv - mf) / bf - ((double)gbuf[p] - mg) / bg;
                         if (MODE == 6) {
                             const double dx = i - (double)hx, dy = j - (double)hy;
@@ -1193,10 +1232,12 @@ __global__ __launch_bounds__(BS) void icgn_kernel(IcgnArgs a)
 }
 
 // Host-side launch helpers (instantiated for the block sizes the autotuner may pick).
-size_t icgn_lds_bytes(int npix_cap, int tile_cap, bool exact)
+size_t icgn_lds_bytes(int npix_cap, int tile_cap, bool exact, bool grad)
 {
-    // g per pixel (FP64 in the re-solve), the g tile, four byte masks per pixel
+    // g per pixel (FP64 in the re-solve), DfDx and DfDy per pixel when staged,
+    // the g tile, four byte masks per pixel
     size_t b = (exact ? sizeof(double) : sizeof(float)) * (size_t)npix_cap +
+               (exact && grad ? sizeof(float2) * (size_t)npix_cap : 0) +
                sizeof(float) * (size_t)tile_cap + 4u * (size_t)npix_cap;
     return (b + 15) & ~(size_t)15;
 }
@@ -1204,7 +1245,7 @@ size_t icgn_lds_bytes(int npix_cap, int tile_cap, bool exact)
 template <int BS, int MODE, bool EXACT>
 static hipError_t launch_bs(const IcgnArgs &a, int grid, hipStream_t s)
 {
-    const size_t lds = icgn_lds_bytes(a.npix_cap, a.tile_cap, EXACT);
+    const size_t lds = icgn_lds_bytes(a.npix_cap, a.tile_cap, EXACT, a.grad_lds != 0);
     static bool attr_set = false;
     if (!attr_set) {
         hipFuncSetAttribute((const void *)icgn_kernel<BS, MODE, EXACT>,

@@ -37,6 +37,7 @@ struct IcgnArgs {
     double guard2;          // guard band of the relative stopping test, on squared norms (0 = none)
     double guard2_abs;      // guard band of the absolute stopping test |b| < tol (0 = none)
     int *exact_count;       // incremented for each subset left to the FP64 re-solve (may be null)
+    int grad_lds;           // FP64 re-solve: stage the window's DfDx, DfDy in LDS once per subset
 };
 
 struct ZnssdArgs {
@@ -95,7 +96,7 @@ hipError_t icgn_wave_launch(const IcgnArgs &a, int mode, int ppt, int nw, int gr
 // icgn_wave_ppt up to four wavefronts (0: too large)
 int icgn_lean_ppt(int npix_cap, int nw);
 hipError_t icgn_lean_launch(const IcgnArgs &a, int mode, int ppt, int nw, int grid, hipStream_t s);
-size_t icgn_lds_bytes(int npix_cap, int tile_cap, bool exact = false);
+size_t icgn_lds_bytes(int npix_cap, int tile_cap, bool exact = false, bool grad = false);
 // FP64 re-solve (workgroup kernel, 256 threads) of the subsets a launch left
 // with status ST_EXACT or ST_DEFER (only_deferred set); no tile (tile_cap 0)
 hipError_t icgn_exact_launch(const IcgnArgs &a, int mode, int bs, int grid, hipStream_t s);

@@ -197,6 +197,41 @@ __device__ __forceinline__ float wave_sum_dpp(float v)
     return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), 63));
 }
 
+// Eight sums over the 64 lanes of a wavefront at once, by a butterfly that
+// halves the data at every step: the halves swap (lanes 0-31 go on with
+// values 0-3, lanes 32-63 with 4-7), then the rows of 16 (value 2r' + j in
+// row r of pair j), then the 8-lane halves of each row, so that value q sums
+// in lanes 8q..8q+7; three DPP steps finish each group.  26 instructions for
+// the eight sums instead of seven per sum.  On return f[q] holds the total of
+// value q in every lane.
+__device__ __forceinline__ void wave_sum8(float (&f)[8])
+{
+    float s[4];
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+        // lanes 32-63 of f[j] trade places with lanes 0-31 of f[j + 4]
+        const auto r = __builtin_amdgcn_permlane32_swap(__float_as_uint(f[j]), __float_as_uint(f[j + 4]),
+                                                        false, false);
+        s[j] = __uint_as_float(r[0]) + __uint_as_float(r[1]);
+    }
+    float t[2];
+#pragma unroll
+    for (int j = 0; j < 2; ++j) {
+        // odd rows of s[j] trade places with even rows of s[j + 2]
+        const auto r = __builtin_amdgcn_permlane16_swap(__float_as_uint(s[j]), __float_as_uint(s[j + 2]),
+                                                        false, false);
+        t[j] = __uint_as_float(r[0]) + __uint_as_float(r[1]);
+    }
+    const bool lo = (threadIdx.x & 15) < 8;
+    const float x = lo ? t[1] : t[0], y = lo ? t[0] : t[1];
+    float z = y + __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(x), 0x128, 0xf, 0xf, false));  // row_ror:8
+    z += __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(z), 0x141, 0xf, 0xf, false));  // row_half_mirror
+    z += __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(z), 0x4e, 0xf, 0xf, false));   // quad_perm 2,3,0,1
+    z += __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(z), 0xb1, 0xf, 0xf, false));   // quad_perm 1,0,3,2
+#pragma unroll
+    for (int q = 0; q < 8; ++q) f[q] = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(z), 8 * q));
+}
+
 // Keys cubic-convolution weights, a = -0.5 (ba_interp2.cpp:204-213).
 template <typename T>
 __device__ __forceinline__ void keys_w(T t, T w[4])
@@ -231,11 +266,10 @@ __device__ __forceinline__ void keys_w2(float t, float w[4])
 }
 
 // Keys gather of window offset (dx, dy) from a tile of row pitch tnx (the
-// taps of the 4 x 4 support, weights from the split coordinates).  The taps
-// are combined with 2 w in both directions and the sum scaled by 1/4, which
-// is exact: the result is the gather with the weights w themselves.
-__device__ __forceinline__ float split_gather(const SplitWarp &s, const float *tile, int tnx,
-                                              float dx, float dy)
+// taps of the 4 x 4 support, weights from the split coordinates), at four
+// times its value: the taps are combined with 2 w in both directions.
+__device__ __forceinline__ float split_gather4(const SplitWarp &s, const float *tile, int tnx,
+                                               float dx, float dy)
 {
     const float su = fmaf(s.p1, dx, fmaf(s.p3, dy, s.fu));
     const float sv = fmaf(s.p2, dx, fmaf(s.p4, dy, s.fv));
@@ -251,7 +285,14 @@ __device__ __forceinline__ float split_gather(const SplitWarp &s, const float *t
         const float *c4 = b + q * tnx;
         g = fmaf(wv[q], fmaf(wu[0], c4[0], fmaf(wu[1], c4[1], fmaf(wu[2], c4[2], wu[3] * c4[3]))), g);
     }
-    return 0.25f * g;
+    return g;
+}
+
+// The same gather at its value (the 1/4 is exact).
+__device__ __forceinline__ float split_gather(const SplitWarp &s, const float *tile, int tnx,
+                                              float dx, float dy)
+{
+    return 0.25f * split_gather4(s, tile, tnx, dx, dy);
 }
 
 template <typename T>

@@ -13,10 +13,12 @@
 //   * per pixel slot the lane keeps df/dx, df/dy and the window offsets
 //     (x - x0, y - y0) in registers, so an iteration is the warp, the Keys
 //     gather from the resident g tile, eight sums, and the FP64 step;
-//   * the sums, the statistics, the Gauss-Newton step and the compose are the
-//     same operations in the same order as in icgn_wave.hip (shared through
-//     wave_common.h), so for these subsets both kernels give bit-identical
-//     results.
+//   * the sums, the statistics, the Gauss-Newton step and the compose are
+//     those of icgn_wave.hip (shared through wave_common.h) up to rounding:
+//     this kernel forms r and q at four times their value (exact), takes the
+//     warped window's extremes in closed form and multiplies by 1/n and
+//     1/bottomf instead of dividing, so both kernels hold the same bar
+//     against the FP64 evaluation, not bit-identical results.
 //
 // Semantics: funICGNQuadtree.m:40-451 / funICGN.m:40-323 (6-DOF) and
 // funICGN_Subpb1.m:40-474 (2-DOF) on windows without masked pixels.
@@ -295,6 +297,8 @@ void icgn_lean_kernel(IcgnArgs a)
         }
 
         // ---------------------------------------------------------- iterate
+        // 1/n and 1/bottomf, FP64 divisions once per subset
+        const double inv_n = 1.0 / (double)npix, inv_bf = 1.0 / bottomf;
         int it = 0;
         int tu0 = 0, tv0 = 0, tnx = 0, tny = 0;  // resident g tile (1-based origin, extents)
         bool have_tile = false;
@@ -334,27 +338,21 @@ void icgn_lean_kernel(IcgnArgs a)
             double umin, umax, vmin, vmax;
             bool out;
             {
-                // the four corners, alike in every lane (no cross-lane step
-                // on the iteration's critical path)
-                double u[4], v[4];
-#pragma unroll
-                for (int q = 0; q < 4; ++q) {
-                    const double dx = (q & 1) ? hx : -hx, dy = (q & 2) ? hy : -hy;
-                    u[q] = (1 + P[0]) * dx + P[2] * dy + (x0 + P[4]);
-                    v[q] = P[1] * dx + (1 + P[3]) * dy + (y0 + P[5]);
-                }
-                umin = fmin(fmin(u[0], u[1]), fmin(u[2], u[3]));
-                umax = fmax(fmax(u[0], u[1]), fmax(u[2], u[3]));
-                vmin = fmin(fmin(v[0], v[1]), fmin(v[2], v[3]));
-                vmax = fmax(fmax(v[0], v[1]), fmax(v[2], v[3]));
-                // NaN corners compare false, as in the reference's test
-                bool o = false, edge = false;
-#pragma unroll
-                for (int q = 0; q < 4; ++q) {
-                    o |= (u[q] < 3) | (u[q] > fr.H - 2) | (v[q] < 3) | (v[q] > fr.W - 2);
-                    edge |= near_edge(u[q], 3.0, fr.H - 2.0) | near_edge(v[q], 3.0, fr.W - 2.0);
-                }
-                out = o;
+                // the corners (x0 +- hx, y0 +- hy) warp to c +- A +- B, so the
+                // extremes of each coordinate are c +- (|A| + |B|); alike in
+                // every lane (no cross-lane step on the critical path)
+                const double cu = x0 + P[4], cv = y0 + P[5];
+                const double eu = fabs((1 + P[0]) * hx) + fabs(P[2] * hy);
+                const double ev = fabs(P[1] * hx) + fabs((1 + P[3]) * hy);
+                umin = cu - eu;
+                umax = cu + eu;
+                vmin = cv - ev;
+                vmax = cv + ev;
+                out = (umin < 3) | (umax > fr.H - 2) | (vmin < 3) | (vmax > fr.W - 2);
+                // the test can only come out otherwise under the FP32 sums'
+                // effect on P where an extreme lies near its own bound
+                const bool edge = (fabs(umin - 3.0) < 1e-3) | (fabs(umax - (fr.H - 2.0)) < 1e-3) |
+                                  (fabs(vmin - 3.0) < 1e-3) | (fabs(vmax - (fr.W - 2.0)) < 1e-3);
                 if (edge && a.guard2 > 0) {
                     state = 3;  // the bounds test decided within FP32 reach: FP64 re-solve
                     exact = true;
@@ -426,6 +424,10 @@ void icgn_lean_kernel(IcgnArgs a)
             const float kf = (float)kap;
             const float ck = (float)((1.0 - (double)kf) * (double)cf);
             const float c1k = (float)((1.0 + (double)kf) * (double)cf);
+            // the gather returns 4 g (taps combined with 2 w both ways), so r
+            // and q are formed at four times their value: every FP32 operand
+            // scaled by 4, which rounds alike; the sums are scaled back in FP64
+            const float kf4 = 4.f * kf, ck4 = 4.f * ck, c1k4 = 4.f * c1k;
             float acc[2 + NJ];
 #pragma unroll
             for (int q = 0; q < 2 + NJ; ++q) acc[q] = 0.f;
@@ -440,11 +442,11 @@ void icgn_lean_kernel(IcgnArgs a)
                 const bool valid = (vbits >> k) & 1u;
                 float dx, dy;
                 unpack_dd(ddr[k], dx, dy);
-                const float g = split_gather(sw, tile, tnx, dx, dy);
+                const float g4 = split_gather4(sw, tile, tnx, dx, dy);
                 const float fv = fwin[pixel(k)];
-                small |= valid && !(fabsf(g) > 1e-10f);
-                const float r = valid ? fmaf(-kf, fv, g) - ck : 0.f;
-                const float qv = fmaf(kf, fv, g) - c1k;
+                small |= valid && !(fabsf(g4) > 4.f * 1e-10f);  // |g| > 1e-10
+                const float r = valid ? fmaf(-kf4, fv, g4) - ck4 : 0.f;
+                const float qv = fmaf(kf4, fv, g4) - c1k4;
                 acc[0] += r;
                 acc[1] = fmaf(r, qv, acc[1]);
                 const float u = fxr[k] * r, v = fyr[k] * r;
@@ -466,6 +468,8 @@ void icgn_lean_kernel(IcgnArgs a)
                 state = 3;  // the general kernel (mask regions, statistics, NaN status)
                 break;
             }
+            // sum r, sum J r at 4 x, sum r q at 16 x their value
+            const double s_r = 0.25 * (double)acc[0], s_rq = 0.0625 * (double)acc[1];
             // every window pixel is in the nonzero set of g
             const double n = npix;
             const double bf = bottomf;
@@ -477,10 +481,10 @@ void icgn_lean_kernel(IcgnArgs a)
                 const double corr = (double)ck - (1.0 - k) * (double)cf;
                 const double corr1 = (double)c1k - (1.0 + k) * (double)cf;
                 const double F1c = L.F1c, F2c = L.F2c;
-                const double S1 = (double)acc[0] + k * F1c + n * corr;
-                const double S2 = (double)acc[1] + k * k * F2c + corr1 * (S1 - k * F1c) +
+                const double S1 = s_r + k * F1c + n * corr;
+                const double S2 = s_rq + k * k * F2c + corr1 * (S1 - k * F1c) +
                                   corr * (S1 + k * F1c) - n * corr * corr1;
-                dg = S1 / n;
+                dg = S1 * inv_n;
                 var = S2 - S1 * dg;
             }
             if (!(n >= 2) || !(bf > 0) || !(var > 0) || !isfinite(bf) || !isfinite(var) ||
@@ -489,7 +493,7 @@ void icgn_lean_kernel(IcgnArgs a)
                 break;
             }
             const double sv = sqrt(var);
-            kap = sv / bf;
+            kap = sv * inv_bf;
             // steepest-descent vector b = FJ - (bottomf/bottomg) (T - (gbar-c) SJ),
             // norm test (with its guard band), Gauss-Newton step and
             // inverse-compositional update; returns 0 to go on, 1 when the
@@ -503,7 +507,7 @@ void icgn_lean_kernel(IcgnArgs a)
                     const double ratio = bf / sv;
 #pragma unroll
                     for (int q = 0; q < NJ; ++q) {
-                        const double Tq = (double)acc[2 + q] + k * L.FJc[q] + corr * L.SJ[q];
+                        const double Tq = 0.25 * (double)acc[2 + q] + k * L.FJc[q] + corr * L.SJ[q];
                         b[q] = L.FJ[q] - ratio * (Tq - dg * L.SJ[q]);
                     }
                 }

@@ -80,8 +80,12 @@ __device__ __forceinline__ void solve_chol6(const double *Lp, const double *Ld, 
 template <int NW, int NF, int ND, int NI, typename Lds>
 __device__ __forceinline__ void group_reduce(Lds &L, int &par, float *f, double *d, int *iv)
 {
+    if constexpr (NF == 8) {
+        wave_sum8(*reinterpret_cast<float(*)[8]>(f));
+    } else {
 #pragma unroll
-    for (int q = 0; q < NF; ++q) f[q] = wave_sum_dpp(f[q]);
+        for (int q = 0; q < NF; ++q) f[q] = wave_sum_dpp(f[q]);
+    }
 #pragma unroll
     for (int q = 0; q < ND; ++q) d[q] = wave_sum_dpp_f64(d[q]);
     if constexpr (NW > 1) {

@@ -560,18 +560,22 @@ __global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(EXACT ? 2 : 
                     if (c.state == 0) {
                         double umin = INFINITY, umax = -INFINITY, vmin = INFINITY, vmax = -INFINITY;
                         bool out = false, edge = false;
+                        // the near-edge hand-over only where a guard band is set
+                        // (not in the FP64 re-solve, which decides every test)
+                        const bool guard = a.guard2 > 0;
                         for (int q = 0; q < 4; ++q) {
                             const double dx = (q & 1) ? hx : -hx, dy = (q & 2) ? hy : -hy;
                             const double u = (1 + P[0]) * dx + P[2] * dy + (c.x0 + P[4]);
                             const double w = P[1] * dx + (1 + P[3]) * dy + (c.y0 + P[5]);
                             if (u < 3 || u > fr.H - 2 || w < 3 || w > fr.W - 2) out = true;
-                            edge |= near_edge(u, 3.0, fr.H - 2.0) | near_edge(w, 3.0, fr.W - 2.0);
+                            if (guard)
+                                edge |= near_edge(u, 3.0, fr.H - 2.0) | near_edge(w, 3.0, fr.W - 2.0);
                             umin = fmin(umin, u);
                             umax = fmax(umax, u);
                             vmin = fmin(vmin, w);
                             vmax = fmax(vmax, w);
                         }
-                        if (edge && a.guard2 > 0) {
+                        if (edge) {
                             // the bounds test decided within FP32 reach: FP64 re-solve
                             c.exact = 1;
                             c.state = 3;
@@ -597,7 +601,9 @@ __global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(EXACT ? 2 : 
                                 c.tile_ok = c.use_tile;
                             }
                             c.reload = !keep && c.use_tile;
-                            c.sw = split_warp(P, c.x0, c.y0, c.tile_u0, c.tile_v0);
+                            // FP32 warp coordinates for the FP32 gather (the
+                            // re-solve gathers in FP64 from P itself)
+                            if constexpr (!EXACT) c.sw = split_warp(P, c.x0, c.y0, c.tile_u0, c.tile_v0);
                         }
                     }
                 }

@@ -98,7 +98,8 @@ int icgn_lean_ppt(int npix_cap, int nw);
 hipError_t icgn_lean_launch(const IcgnArgs &a, int mode, int ppt, int nw, int grid, hipStream_t s);
 size_t icgn_lds_bytes(int npix_cap, int tile_cap, bool exact = false, bool grad = false);
 // FP64 re-solve (workgroup kernel, 256 threads) of the subsets a launch left
-// with status ST_EXACT or ST_DEFER (only_deferred set); no tile (tile_cap 0)
+// with status ST_EXACT or ST_DEFER (only_deferred set); g from an LDS tile
+// when tile_cap > 0, else from HBM
 hipError_t icgn_exact_launch(const IcgnArgs &a, int mode, int bs, int grid, hipStream_t s);
 hipError_t ba_interp2_launch(const void *F, long M, long N, long P, const void *X, const void *Y,
                              long nd, void *out, bool single, hipStream_t s);

@@ -73,8 +73,8 @@ __device__ __forceinline__ void solve_chol6(const double *Lp, const double *Ld, 
 }
 
 // Subset-wide sums over the NW wavefronts of a workgroup: f and d hold each
-// lane's partials (reduced within the wavefront by DPP), iv each wavefront's
-// own totals.  Every wavefront adds the same partials in the same order, so
+// lane's partials (reduced within the wavefront by DPP; eight floats together
+// by the lane-swap butterfly of wave_sum8), iv each wavefront's own totals.  Every wavefront adds the same partials in the same order, so
 // all of them hold bit-identical totals afterwards; the buffers alternate, so
 // one barrier per exchange suffices.
 template <int NW, int NF, int ND, int NI, typename Lds>

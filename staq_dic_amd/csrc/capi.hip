@@ -1141,8 +1141,11 @@ static IcgnArgs base_args(staq_batch *b, const staq_params *p)
 // DfDx and DfDy, the g tile; two subsets per CU), then the larger ones,
 // gathering from HBM.
 // q[0], q[1]: queue head and list length of the first; q[3], q[4]: of the
-// second (q[2] counts the FP64 re-solves).
-static hipError_t fallback_launches(IcgnArgs a, int mode, int *q, int fb_npix_cap, hipStream_t s)
+// second (q[2] counts the FP64 re-solves).  A subset is listed by its own
+// window size, so with no window above kSmallFallbackW px (wmax) the second
+// list stays empty and its launch is left out.
+static hipError_t fallback_launches(IcgnArgs a, int mode, int *q, int fb_npix_cap, int wmax,
+                                    hipStream_t s)
 {
     a.item0 = 0;
     a.n_items = a.N;
@@ -1157,7 +1160,7 @@ static hipError_t fallback_launches(IcgnArgs a, int mode, int *q, int fb_npix_ca
     a.defer_flag = q + 1;
     a.list_back = 0;
     hipError_t e = icgn_exact_launch(a, mode, 256, (int)std::min<long>(a.N, 1280), s);
-    if (e != hipSuccess) return e;
+    if (e != hipSuccess || wmax <= kSmallFallbackW) return e;
     a.npix_cap = fb_npix_cap;
     a.tile_cap = 0;  // gathers from HBM
     a.grad_lds = 0;
@@ -1262,7 +1265,7 @@ static int enqueue_step(staq_batch *b, const staq_params *p, int mode, hipStream
     if (bin_only < 0) {
         // fallback over the whole batch
         if (timed) HIPCHK(ctx, hipEventRecord(b->tev[2 * nb], s));
-        HIPCHK(ctx, fallback_launches(base_args(b, p), mode, ints + nb, b->fb_npix_cap, s));
+        HIPCHK(ctx, fallback_launches(base_args(b, p), mode, ints + nb, b->fb_npix_cap, b->wmax, s));
         if (timed) HIPCHK(ctx, hipEventRecord(b->tev[2 * nb + 1], s));
     }
     return STAQ_OK;
@@ -1608,7 +1611,7 @@ int staq_local_icgn_noalloc(staq_ctx *ctx, const staq_params *p, int slot, long 
         a.tile_cap = tile_side(gw + 1) * tile_side(gw + 1);
         HIPCHK(ctx, icgn_launch(a, 6, 256, (int)N, s));
     }
-    HIPCHK(ctx, fallback_launches(a, 6, q, gnpix, s));
+    HIPCHK(ctx, fallback_launches(a, 6, q, gnpix, wmax, s));
     return STAQ_OK;
 }
 

@@ -1093,6 +1093,18 @@ extern "C" int staq_batch_bin_of(staq_batch *b, int *bin)
 }
 
 // Stopping rule, damping, iteration cap and guard band of a local step.
+// The guard bands as limits on the squared norms, with the operations of
+// near_tol (common.h), so that the lean kernel's compares decide alike.
+static void set_bands(IcgnArgs &a)
+{
+    const double tol2 = a.tol * a.tol;
+    const bool r = a.guard2 > 0, b = a.guard2_abs > 0;
+    a.rel_lo2 = r ? tol2 * (1.0 - a.guard2) : INFINITY;
+    a.rel_hi2 = r ? tol2 * (1.0 + a.guard2) : -INFINITY;
+    a.abs_lo2 = b ? tol2 * (1.0 - a.guard2_abs) : INFINITY;
+    a.abs_hi2 = b ? tol2 * (1.0 + a.guard2_abs) : -INFINITY;
+}
+
 static void set_step_params(IcgnArgs &a, const staq_params *p)
 {
     a.tol = p->tol;
@@ -1106,6 +1118,7 @@ static void set_step_params(IcgnArgs &a, const staq_params *p)
     // larger fraction of it: a band four times as wide
     const double ga = 4.0 * g;
     a.guard2_abs = ga > 0 ? ga * (2.0 + ga) : 0.0;
+    set_bands(a);
 }
 
 static IcgnArgs base_args(staq_batch *b, const staq_params *p)
@@ -1152,6 +1165,7 @@ static hipError_t fallback_launches(IcgnArgs a, int mode, int *q, int fb_npix_ca
     a.defer = 0;
     a.guard2 = 0;
     a.guard2_abs = 0;
+    set_bands(a);
     a.only_deferred = 1;
     a.npix_cap = std::min(kSmallFallbackPix, fb_npix_cap);
     a.tile_cap = tile_side(48) * tile_side(48);  // the warped support of a 48 x 48 window
@@ -1329,6 +1343,7 @@ int staq_batch_icgn6_per_subset(staq_batch *b, const staq_params *p, double *ker
         a.npix_cap = npix_cap;
         a.tile_cap = tile_cap;
         a.guard2 = a.guard2_abs = 0;  // no fallback launch follows: every subset is finished here
+        set_bands(a);
         if (kernel_ms_sum) HIPCHK(ctx, hipEventRecord(e0, s));
         HIPCHK(ctx, icgn_launch(a, 6, bs, 1, s));
         if (kernel_ms_sum) {

@@ -528,7 +528,9 @@ void icgn_lean_kernel(IcgnArgs a)
                     inv_init2 = 1.0 / nb2;
                 }
                 rel2 = init_ok ? nb2 * inv_init2 : 0.0;
-                if (near_tol(abs2, tol2, a.guard2_abs) || (init_ok && near_tol(rel2, tol2, a.guard2)))
+                // near_tol against the band limits set once per launch
+                if ((abs2 > a.abs_lo2 && abs2 < a.abs_hi2) ||
+                    (init_ok && rel2 > a.rel_lo2 && rel2 < a.rel_hi2))
                     return 2;
                 if (rel2 < tol2 || abs2 < tol2) return 1;
                 double d[6] = {0, 0, 0, 0, 0, 0};

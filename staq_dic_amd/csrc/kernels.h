@@ -38,6 +38,9 @@ struct IcgnArgs {
     double guard2_abs;      // guard band of the absolute stopping test |b| < tol (0 = none)
     int *exact_count;       // incremented for each subset left to the FP64 re-solve (may be null)
     int grad_lds;           // FP64 re-solve: stage the window's DfDx, DfDy in LDS once per subset
+    // the guard bands as limits on the squared norms, tol^2 (1 -+ guard2) and
+    // tol^2 (1 -+ guard2_abs) (lean kernel; +inf / -inf when the guard is off)
+    double rel_lo2, rel_hi2, abs_lo2, abs_hi2;
 };
 
 struct ZnssdArgs {

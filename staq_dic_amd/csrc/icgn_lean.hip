@@ -81,10 +81,20 @@ void icgn_lean_kernel(IcgnArgs a)
     float *fwin = tile + a.tile_cap;
     const int tid = threadIdx.x;
     const int lane = tid & 63;
-    const int wv = tid >> 6;
+    const int wv = __builtin_amdgcn_readfirstlane(tid >> 6);  // scalar: alike in every lane
     const bool quad = !(MODE == 6 && a.plain);
     int par = 0;
-    auto pixel = [&](int k) { return lane + 64 * (k * NW + wv); };
+    // pixel of slot k for lane index l.  The lane index reaches each use
+    // through fresh_lane(), an empty asm the compiler cannot see through, so
+    // that what derives from it (pixel offsets, LDS addresses) is recomputed
+    // where it is used instead of being kept live, and spilled, across the
+    // iteration loop; the asm emits no instruction.
+    auto pixel_of = [&](int l, int k) { return l + 64 * (k * NW + wv); };
+    auto fresh_lane = [&]() {
+        int l = lane;
+        asm volatile("" : "+v"(l));
+        return l;
+    };
 
     for (long pass = 0;; ++pass) {
         if (tid == 0)
@@ -94,6 +104,8 @@ void icgn_lean_kernel(IcgnArgs a)
         const long item = item_sh;
         if (item >= a.item0 + a.n_items) break;
         const long idx = a.order ? gld(a.order, (size_t)(item)) : item;
+        const int lane_s = fresh_lane();  // for this subset's set-up
+        auto pixel = [&](int k) { return pixel_of(lane_s, k); };
         const int slot = a.slot ? gld(a.slot, (size_t)(idx)) : 0;
         const FrameDev fr = load_frame(a.frames, slot);
         const int Hd = fr.H - 6;
@@ -435,6 +447,7 @@ void icgn_lean_kernel(IcgnArgs a)
             // 1e-10 in magnitude (outside the nonzero set the statistics
             // take) or NaN (logical(NaN) throws): the general kernel decides
             bool small = false;
+            const int lane_i = fresh_lane();  // for this iteration's f reads
 #pragma unroll
             for (int k = 0; k < PPT; ++k) {
                 // no lane of this wavefront holds a pixel of slot k
@@ -443,7 +456,7 @@ void icgn_lean_kernel(IcgnArgs a)
                 float dx, dy;
                 unpack_dd(ddr[k], dx, dy);
                 const float g4 = split_gather4(sw, tile, tnx, dx, dy);
-                const float fv = fwin[pixel(k)];
+                const float fv = fwin[pixel_of(lane_i, k)];
                 small |= valid && !(fabsf(g4) > 4.f * 1e-10f);  // |g| > 1e-10
                 const float r = valid ? fmaf(-kf4, fv, g4) - ck4 : 0.f;
                 const float qv = fmaf(kf4, fv, g4) - c1k4;

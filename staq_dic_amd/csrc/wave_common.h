@@ -89,7 +89,8 @@ __device__ __forceinline__ void group_reduce(Lds &L, int &par, float *f, double 
 #pragma unroll
     for (int q = 0; q < ND; ++q) d[q] = wave_sum_dpp_f64(d[q]);
     if constexpr (NW > 1) {
-        const int w = threadIdx.x >> 6;
+        // the wavefront's index in a scalar register (alike in every lane)
+        const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
         if ((threadIdx.x & 63) == 0) {
 #pragma unroll
             for (int q = 0; q < NF; ++q) L.xf[par][w][q] = f[q];

@@ -715,6 +715,9 @@ static int grown_window(int w)
 
 static int tile_side(int n) { return (int)std::ceil(n * 1.12) + 8; }
 
+// bins from this many subsets take one wavefront per subset up to 1152 pixels
+static constexpr long kOneWaveMin = 2048;
+
 static int default_bs(int npix)
 {
     if (npix <= 400) return 64;
@@ -793,15 +796,19 @@ static int build_bins(staq_batch *b)
         // (or grows although classified clear) defers to the fallback launch
         bin.wave_npix = std::max(bin.wave_npix, bin.npix_cap);
         bin.ppt = icgn_wave_ppt(bin.wave_npix, 1);
-        // default form (measured on the hole and large-deformation
+        // default form (measured on the hole, plain and large-deformation
         // workloads): bins without masked pixels take the lean kernel, one
-        // wavefront per subset while five pixel slots hold the window, else
-        // two (up to 1152 pixels: the 33 x 33 windows), four or eight; bins
-        // for the general kernel take four (two, one) wavefronts; windows
-        // beyond that take the workgroup kernel
+        // wavefront per subset while five pixel slots hold the window, and
+        // up to 1152 pixels (18 slots per lane: the 33 x 33 windows; no FP64
+        // step taken twice) in bins of at least kOneWaveMin subsets, where
+        // throughput decides; else two (shorter chains for small bins),
+        // four or eight; bins for the general kernel take four (two, one)
+        // wavefronts; windows beyond that take the workgroup kernel
         int form = default_bs(bin.wave_npix);
         const int g = bin.masked ? -1 : 1;
         if (!bin.masked && bin.ppt && bin.ppt <= 5)
+            form = 1;
+        else if (!bin.masked && bin.n >= kOneWaveMin && icgn_lean_ppt(bin.wave_npix, 1))
             form = 1;
         else if (!bin.masked && icgn_lean_ppt(bin.wave_npix, 2))
             form = 2;

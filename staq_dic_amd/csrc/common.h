@@ -530,7 +530,8 @@ __device__ inline bool ic_compose(double P[6], const double d[6])
 template <int BS, int K, typename T>
 __device__ __forceinline__ void block_sum(T (&v)[K], T *red, T (&out)[K])
 {
-    const int lane = threadIdx.x & (kWave - 1), wid = threadIdx.x / kWave;
+    // the wavefront's index in a scalar register (alike in every lane)
+    const int lane = threadIdx.x & (kWave - 1), wid = __builtin_amdgcn_readfirstlane(threadIdx.x / kWave);
 #pragma unroll
     for (int k = 0; k < K; ++k) {
         T s;
